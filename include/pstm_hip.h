@@ -1,0 +1,144 @@
+/*
+ * pstm_hip.h — C ABI of the MI355X (gfx950) PSTM migration backend.
+ *
+ * The entry points replace, for the `hip` backend, the device boundary the
+ * reference crosses inside MigrationKernel.migrate_tile
+ * (pstm/kernels/base.py:290-318): the registered Numba CPU backend calls
+ * _migrate_tile_kernel_optimized (pstm/kernels/numba_cpu_optimized.py:93-233)
+ * after _precompute_time_terms (:358-401); the Metal precedent crosses it with
+ * the pybind11 call pstm_metal.migrate_tile (pstm/metal/src/bindings.cpp:48-159).
+ *
+ * Conventions (the reference's pybind module raises; here nothing crosses the
+ * C ABI but status codes):
+ *   - every function returns 0 on success, non-zero on failure; the message of
+ *     the last failure on the calling thread is pstm_hip_last_error();
+ *   - all pointers in PstmMigrateArgs are DEVICE pointers on the current HIP
+ *     device, the stream argument is a hipStream_t (NULL = default stream);
+ *   - image and fold are ACCUMULATED (+=), as the Numba backend does
+ *     (numba_cpu_optimized.py:229,231);
+ *   - launches are asynchronous on `stream`; pstm_hip_stream_sync() waits.
+ */
+#ifndef PSTM_HIP_H
+#define PSTM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSTM_HIP_ABI_VERSION 1
+
+/* traveltime: KernelConfig.kernel_type (pstm/kernels/base.py:220-229) */
+#define PSTM_TT_STRAIGHT 0 /* DSR, numba_cpu_optimized.py:189-194            */
+#define PSTM_TT_CURVED 1   /* V(z)=V0+kz, pstm/algorithm/curved_ray.py:19-52 */
+#define PSTM_TT_VTI 2      /* eta, pstm/algorithm/anisotropy_vti.py:148-220 */
+
+/* anti-aliasing (the reference's registered kernel has none) */
+#define PSTM_AA_NONE 0
+#define PSTM_AA_TRIANGLE 1 /* triangle filter on double-integrated traces */
+
+/* velocity slice layout: VelocitySlice.is_1d (pstm/kernels/base.py:180-194) */
+#define PSTM_VEL_1D 0 /* vrms (nt,)        */
+#define PSTM_VEL_3D 1 /* vrms (nx, ny, nt) */
+
+/* eta layout for PSTM_TT_VTI (KernelConfig.vti_eta_*, base.py:231-235) */
+#define PSTM_ETA_CONST 0
+#define PSTM_ETA_1D 1
+#define PSTM_ETA_3D 2
+
+/*
+ * One tile migration: TraceBlock (base.py:27-116) x OutputTile (base.py:
+ * 119-177) x VelocitySlice (base.py:180-194) x KernelConfig (base.py:197-265).
+ * Plain old data; mirrored field for field by a ctypes.Structure and checked
+ * with pstm_hip_args_size() when the library is loaded.
+ */
+typedef struct PstmMigrateArgs {
+    /* --- trace block (device) --- */
+    const float* amplitudes;      /* (rows, n_samples) row-major, f32     */
+    const double* source_x;       /* (rows,) absolute coordinates, f64    */
+    const double* source_y;
+    const double* receiver_x;
+    const double* receiver_y;
+    const double* midpoint_x;
+    const double* midpoint_y;
+    const double* trace_weights;  /* (rows,) or NULL = all 1.0            */
+    const int32_t* trace_index;   /* (n_traces,) rows to use, or NULL     */
+    /* --- output tile (device) --- */
+    const double* x_axis;         /* (nx,)                                */
+    const double* y_axis;         /* (ny,)                                */
+    const double* t_axis_ms;      /* (nt,)                                */
+    /* --- velocity slice (device) --- */
+    const double* vrms;           /* (nt,) or (nx, ny, nt)                */
+    const double* eta;            /* NULL, (nt,) or (nx, ny, nt)          */
+    /* --- outputs (device, accumulated) --- */
+    float* image;                 /* (nx, ny, nt) f32, +=                 */
+    int32_t* fold;                /* (nx, ny) pillar fold, +=             */
+    unsigned long long* n_effective; /* NULL or one counter, += E         */
+    /* --- sizes and switches --- */
+    int32_t n_traces;             /* traces used (length of trace_index)  */
+    int32_t n_samples;            /* samples per trace                    */
+    int32_t nx, ny, nt;
+    int32_t vel_mode;             /* PSTM_VEL_*                           */
+    int32_t eta_mode;             /* PSTM_ETA_*                           */
+    int32_t traveltime;           /* PSTM_TT_*                            */
+    int32_t aa_mode;              /* PSTM_AA_*                            */
+    int32_t apply_spreading;      /* 1/(v t) weight, t > 1 ms             */
+    int32_t apply_obliquity;      /* t0/t weight, t > 1 ms                */
+    int32_t aa_max_half_width;    /* triangle half width clamp (samples)  */
+    /* --- scalars --- */
+    double dt_in_ms;              /* TraceBlock.sample_rate_ms            */
+    double t_start_in_ms;         /* TraceBlock.start_time_ms             */
+    double max_aperture_m;
+    double min_aperture_m;
+    double max_dip_deg;
+    double taper_fraction;
+    double grid_dx;               /* output bin size x (AA operator dip)  */
+    double grid_dy;               /* output bin size y                    */
+    double curved_v0;             /* PSTM_TT_CURVED: V(z) = v0 + k z      */
+    double curved_k;
+    double eta_constant;          /* PSTM_ETA_CONST                       */
+} PstmMigrateArgs;
+
+/* ABI identity, for the loader's layout check. */
+int pstm_hip_abi_version(void);
+size_t pstm_hip_args_size(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* pstm_hip_last_error(void);
+
+/* Select the device and create per-device state. Replaces the reference's
+ * kernel initialize() (pstm/kernels/base.py:304-305). */
+int pstm_hip_init(int device);
+
+/* Release device workspaces. Replaces cleanup() (base.py:317-318). */
+int pstm_hip_release(void);
+
+/* Wait for all work queued on `stream`. Replaces synchronize()
+ * (base.py:314-315). */
+int pstm_hip_stream_sync(void* stream);
+
+/* Migrate one output tile: fold (f64 aperture test, bit-exact with the
+ * oracle), image (f32 gather, no atomics) and optionally the effective
+ * contribution count E. Replaces migrate_tile (base.py:307-313) and the call
+ * of _migrate_tile_kernel_optimized at numba_cpu_optimized.py:593-618. */
+int pstm_hip_migrate_tile(const PstmMigrateArgs* args, void* stream);
+
+/* Fold only: per-pillar count of traces with midpoint distance <=
+ * max_aperture_m (numba_cpu_optimized.py:167-175,231). */
+int pstm_hip_fold_tile(const PstmMigrateArgs* args, void* stream);
+
+/* Work-queue migration of several trace groups (common-offset bins) into one
+ * shared image in a single persistent launch. groups[g] gives the trace rows
+ * [group_start[g], group_start[g+1]) of args->trace_index (or of the block).
+ * Replaces the executor's per-bin, per-tile loop
+ * (pstm/pipeline/executor.py:977-1358, 1691-1745). */
+int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_start,
+                          int32_t n_groups, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSTM_HIP_H */

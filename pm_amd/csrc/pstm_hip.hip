@@ -1,0 +1,856 @@
+// pstm_hip.hip — MI355X (gfx950, CDNA4) PSTM tile migration behind a C ABI.
+//
+// The reference computes one output tile with a Numba loop nest
+// (pstm/kernels/numba_cpu_optimized.py:93-233): pillars in parallel, every
+// trace, every output sample, FP64.  Here the same sum is a gather laid out for
+// a 64-wide wavefront machine:
+//
+//   * one workgroup owns NPIL output pillars x a window of output samples and
+//     is the only writer of those image cells (no atomics on the image);
+//   * lanes of a wave map to consecutive output samples, so the DSR positions
+//     of neighbouring lanes land on neighbouring trace samples in LDS
+//     (conflict-free reads) while the per-(pillar, trace) terms are uniform;
+//   * each wave holds PW pillars x MW sample blocks; the per-(pillar, sample)
+//     velocity / aperture / weight terms stay in VGPRs for the whole launch,
+//     the image accumulates in VGPRs (f32 within a trace batch, f64 across
+//     batches) and is written once;
+//   * trace batches are staged HBM -> LDS with coalesced loads, shared by the
+//     16 waves of the workgroup; the (pillar, trace) pair table is built once
+//     per batch in LDS and a batch no pillar of the workgroup sees is skipped;
+//   * the fold is a separate f64 pass so that it is bit-exact with the FP64
+//     oracle; absolute coordinates are reduced to tile-relative f32 offsets
+//     in f64 before any f32 arithmetic.
+//
+// Built with -Xclang -target-feature -Xclang -packed-fp32-ops.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pstm_hip.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            return fail(3, std::string(#expr " failed: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------------
+constexpr int kPW = 2;                 // pillars per wave
+constexpr int kTW = 4;                 // time waves per workgroup
+constexpr int kPG = 4;                 // pillar waves per workgroup
+constexpr int kWaves = kTW * kPG;      // 16 waves
+constexpr int kThreads = 64 * kWaves;  // 1024 threads
+constexpr int kNPil = kPW * kPG;       // 8 pillars per workgroup
+constexpr int kMaxBatch = 16;          // traces per LDS batch (upper bound)
+constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS uses the rest
+
+// per-(pillar, sample) terms, f32, built once per call in f64
+//   x : traveltime slowness^2 s = 1/v^2       (curved ray: v(z) at depth)
+//   y : weight constant  (spreading ? 1/v : 1) * (obliquity ? t0 : 1)
+//   z : taper slope  1/(ap - ts)              (ts = ap (1 - taper))
+//   w : taper offset -ts/(ap - ts)  => x_t = d*z + w, in-aperture x_t <= 1
+struct KParams {
+    const float* amp;            // (rows, ns)
+    const int32_t* tidx;         // (ntr) or null
+    const float4* geo;           // (ntr, 2): {sx, sy, rx, ry}, {mx, my, w, 0}
+    const double* prow;          // AA: (ntr, rowlen) double-integrated rows
+    const float4* kc;            // (npv, nt)
+    const float* qk;             // (nt) tau^2 (curved: v(z)^2)
+    const float2* ek;            // VTI (nev, nt): {2 eta, 1 + 2 eta}
+    const float* t0k;            // (nt) t0 seconds
+    const double* xax;
+    const double* yax;
+    float* image;                // (nx, ny, nt)
+    unsigned long long* cnt_part;  // (gridDim.x) or null
+    int ns, ntr, nx, ny, nt;
+    int kc_pstride, ek_pstride;  // nt or 0
+    int rowlen, lo;              // AA row length / index offset
+    int batch;                   // traces per LDS batch
+    int pair_off;                // byte offset of the pair table in LDS
+    int n_items, n_pgroups;      // work items = pillar groups x time windows
+    int wpow;                    // 0, 1 or 2: power of 1/t in the weight
+    float idt, uoff, umax;       // u = t*idt + uoff, valid u in [0, umax)
+    float max_ap;
+    float gdx_idt, gdy_idt, lmax;  // AA
+    float ck2, ik_ln2;           // curved: k^2, ln2/k
+    float v0;                    // curved: surface velocity
+};
+
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+
+// 0.5 (1 + cos(pi x)) for x in [0, 1]: with y = x - 1/2, cos(pi x) = -sin(pi y);
+// odd Taylor series of sin(pi y) through y^11 (|pi y| <= pi/2, error < 6e-8).
+__device__ __forceinline__ float taper_weight(float x) {
+    const float y = x - 0.5f;
+    const float z = y * y;
+    float p = -7.370430945714348e-03f;        // -pi^11/11!
+    p = fmaf(p, z, 8.214588661112819e-02f);   //  pi^9/9!
+    p = fmaf(p, z, -5.992645293207919e-01f);  // -pi^7/7!
+    p = fmaf(p, z, 2.550164039877345e+00f);   //  pi^5/5!
+    p = fmaf(p, z, -5.167712780049969e+00f);  // -pi^3/3!
+    p = fmaf(p, z, 3.141592653589793e+00f);   //  pi
+    return fmaf(-0.5f, p * y, 0.5f);
+}
+
+// ---------------------------------------------------------------------------
+// preparation kernels (f64)
+// ---------------------------------------------------------------------------
+
+// absolute f64 survey coordinates -> tile-relative f32 offsets
+__global__ void k_prep_geometry(PstmMigrateArgs a, float4* geo) {
+    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= a.n_traces) return;
+    const int j = a.trace_index ? a.trace_index[jj] : jj;
+    const double x0 = a.x_axis[0], y0 = a.y_axis[0];
+    const float w = a.trace_weights ? (float)a.trace_weights[j] : 1.0f;
+    geo[2 * jj] = make_float4((float)(a.source_x[j] - x0), (float)(a.source_y[j] - y0),
+                              (float)(a.receiver_x[j] - x0), (float)(a.receiver_y[j] - y0));
+    geo[2 * jj + 1] = make_float4((float)(a.midpoint_x[j] - x0), (float)(a.midpoint_y[j] - y0), w, 0.0f);
+}
+
+// per-(pillar|0, sample) terms: _precompute_time_terms
+// (numba_cpu_optimized.py:375-401) plus the weight and taper constants.
+// tt_eff: traveltime actually evaluated (curved with |k| < 1e-10 becomes
+// straight ray in v0, curved_ray.py:35-38).
+__global__ void k_prep_time_terms(PstmMigrateArgs a, int tt_eff, int npv, double tan_dip, float4* kc,
+                                  float* qk, float* t0k) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = a.nt;
+    if (e >= (long)npv * nt) return;
+    const int k = (int)(e % nt);
+    const double t_s = a.t_axis_ms[k] / 1000.0;
+    const double v = a.vrms[e];
+    double apv = v * t_s * 0.5 * tan_dip;
+    if (apv < a.min_aperture_m) apv = a.min_aperture_m;
+    else if (apv > a.max_aperture_m) apv = a.max_aperture_m;
+    double s, wv, q;
+    if (a.traveltime == PSTM_TT_CURVED) {
+        const double v0 = a.curved_v0, kg = a.curved_k;
+        const double tau = t_s / 2.0;
+        const double z = fabs(kg) < 1e-10 ? v0 * tau : (v0 / kg) * (exp(kg * tau) - 1.0);
+        const double vz = v0 + kg * z;
+        wv = vz;
+        if (tt_eff == PSTM_TT_STRAIGHT) {  // k ~ 0: straight ray in v0
+            s = 1.0 / (v0 * v0);
+            q = tau * tau;
+        } else {
+            s = vz;        // v(z) at the image depth
+            q = vz * vz;   // v(z)^2
+        }
+    } else {
+        s = 1.0 / (v * v);
+        wv = 1.0 / sqrt(s);
+        q = (t_s * 0.5) * (t_s * 0.5);
+    }
+    double wc = 1.0;
+    if (a.apply_spreading) wc *= 1.0 / wv;
+    if (a.apply_obliquity) wc *= t_s;
+    const double ts = apv * (1.0 - a.taper_fraction);
+    const double width = apv - ts;
+    double slope, off;
+    if (width > 0.0) {
+        slope = 1.0 / width;
+        off = -ts / width;
+    } else {  // no taper: x_t <= 0 inside the aperture, > 1 outside
+        slope = 1e30;
+        off = -apv * 1e30;
+    }
+    kc[e] = make_float4((float)s, (float)wc, (float)slope, (float)off);
+    if (e < nt) {
+        qk[k] = (float)q;
+        t0k[k] = (float)t_s;
+    }
+}
+
+__global__ void k_prep_eta(PstmMigrateArgs a, int nev, float2* ek) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nev * a.nt) return;
+    const double eta = a.eta_mode == PSTM_ETA_CONST ? a.eta_constant : a.eta[e];
+    ek[e] = make_float2((float)(2.0 * eta), (float)(1.0 + 2.0 * eta));
+}
+
+// Fold: count of traces with f64 midpoint distance <= max aperture, one
+// workgroup per pillar (numba_cpu_optimized.py:167-175,231).  The products and
+// sums are kept unfused so the test is the oracle's, bit for bit.
+__global__ __launch_bounds__(256) void k_fold(PstmMigrateArgs a) {
+    const int pil = blockIdx.x;
+    const int ix = pil / a.ny, iy = pil % a.ny;
+    const double ox = a.x_axis[ix], oy = a.y_axis[iy];
+    int c = 0;
+    for (int jj = threadIdx.x; jj < a.n_traces; jj += blockDim.x) {
+        const int j = a.trace_index ? a.trace_index[jj] : jj;
+        const double dx = __dsub_rn(ox, a.midpoint_x[j]);
+        const double dy = __dsub_rn(oy, a.midpoint_y[j]);
+        const double d = __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
+        c += (d <= a.max_aperture_m) ? 1 : 0;
+    }
+    __shared__ int red[256];
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.fold[pil] += red[0];
+}
+
+// Double running sum rows for triangle anti-aliasing, in f64:
+//   S[n] = sum_{m<=n} A[m],  P[n] = sum_{m<n} S[m],
+//   row[n + lo] = P[n] for n in [-lo, ns + lmax], P = 0 for n <= 0,
+//   P[n] = P[ns] + (n - ns) S[ns-1] for n > ns (zero trace outside).
+__global__ __launch_bounds__(256) void k_integrate(PstmMigrateArgs a, int rowlen, int lo, double* prow) {
+    const int jj = blockIdx.x;
+    const int j = a.trace_index ? a.trace_index[jj] : jj;
+    const int ns = a.n_samples;
+    const float* tr = a.amplitudes + (size_t)j * ns;
+    double* out = prow + (size_t)jj * rowlen;
+    const int chunk = (ns + 255) / 256;
+    const int b = threadIdx.x * chunk, e = min(b + chunk, ns);
+    __shared__ double sh[256];
+    __shared__ double tot_s, tot_p;
+    // pass 1: S
+    double ls = 0.0;
+    for (int n = b; n < e; ++n) ls += (double)tr[n];
+    sh[threadIdx.x] = ls;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {  // inclusive Hillis-Steele scan
+        double v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const double s_before = sh[threadIdx.x] - ls;
+    if (threadIdx.x == 255) tot_s = sh[255];
+    __syncthreads();
+    // pass 2: sum of S over the chunk, scan
+    double s = s_before, lp = 0.0;
+    for (int n = b; n < e; ++n) {
+        s += (double)tr[n];
+        lp += s;
+    }
+    sh[threadIdx.x] = lp;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        double v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    double p = sh[threadIdx.x] - lp;
+    if (threadIdx.x == 255) tot_p = sh[255];
+    s = s_before;
+    for (int n = b; n < e; ++n) {
+        s += (double)tr[n];
+        p += s;
+        out[n + 1 + lo] = p;  // P[n+1]
+    }
+    __syncthreads();
+    // P[n] for n <= 0
+    for (int n = threadIdx.x; n <= lo; n += blockDim.x) out[n] = 0.0;
+    // linear continuation beyond the trace
+    const double pend = tot_p, send = tot_s;
+    for (int n = ns + 1 + (int)threadIdx.x; n + lo < rowlen; n += blockDim.x)
+        out[n + lo] = pend + (double)(n - ns) * send;
+}
+
+// ---------------------------------------------------------------------------
+// the gather
+// ---------------------------------------------------------------------------
+template <int MW, int TT, bool AA>
+__global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using Row = typename std::conditional<AA, double, float>::type;
+    constexpr int kPairVec = (AA || TT == PSTM_TT_CURVED) ? 2 : 1;
+    Row* rows = reinterpret_cast<Row*>(smem);
+    float4* pairs = reinterpret_cast<float4*>(smem + P.pair_off);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int tw = wave % kTW;
+    const int pg = wave / kTW;
+    const int npil = P.nx * P.ny;
+    const int win = 64 * kTW * MW;
+    unsigned long long cnt = 0;
+
+    for (int item = blockIdx.x; item < P.n_items; item += gridDim.x) {
+        const int pgroup = item % P.n_pgroups;
+        const int kwin = item / P.n_pgroups;
+        const int pil0 = pgroup * kNPil;
+        const int kbase = kwin * win;
+
+        // register-resident per-(pillar, sample) terms
+        float q[MW], cs[kPW][MW], cw[kPW][MW], ci[kPW][MW], cn[kPW][MW];
+        float e2[kPW][MW], e1[kPW][MW], t0r[MW];
+        float acc[kPW][MW];
+        double tot[kPW][MW];
+#pragma unroll
+        for (int m = 0; m < MW; ++m) {
+            const int k = kbase + 64 * (tw + kTW * m) + lane;
+            const bool kv = k < P.nt;
+            q[m] = kv ? P.qk[k] : 0.0f;
+            t0r[m] = kv ? P.t0k[k] : 0.0f;
+#pragma unroll
+            for (int p = 0; p < kPW; ++p) {
+                const int pil = pil0 + pg * kPW + p;
+                acc[p][m] = 0.0f;
+                tot[p][m] = 0.0;
+                if (kv && pil < npil) {
+                    const float4 c = P.kc[(size_t)pil * P.kc_pstride + k];
+                    cs[p][m] = c.x;
+                    cw[p][m] = c.y;
+                    ci[p][m] = c.z;
+                    cn[p][m] = c.w;
+                    if (TT == PSTM_TT_VTI) {
+                        const float2 ee = P.ek[(size_t)pil * P.ek_pstride + k];
+                        e2[p][m] = ee.x;
+                        e1[p][m] = ee.y;
+                    }
+                } else {
+                    cs[p][m] = 0.0f;
+                    cw[p][m] = 0.0f;
+                    ci[p][m] = 0.0f;
+                    cn[p][m] = 2.0f;  // x_t = 2 > 1: never in aperture
+                    e2[p][m] = 0.0f;
+                    e1[p][m] = 1.0f;
+                }
+            }
+        }
+
+        // tile-relative pillar coordinates for the pair-table threads
+        float pox = 0.0f, poy = 0.0f;
+        const int pt_p = tid / P.batch, pt_j = tid % P.batch;
+        const bool pt_thread = tid < kNPil * P.batch;
+        if (pt_thread) {
+            const int pil = pil0 + pt_p;
+            if (pil < npil) {
+                pox = (float)(P.xax[pil / P.ny] - P.xax[0]);
+                poy = (float)(P.yax[pil % P.ny] - P.yax[0]);
+            }
+        }
+
+        for (int j0 = 0; j0 < P.ntr; j0 += P.batch) {
+            const int nb = min(P.batch, P.ntr - j0);
+            // (pillar, trace) pair table
+            int inap = 0;
+            if (pt_thread) {
+                const int pil = pil0 + pt_p;
+                float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
+                float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (pt_j < nb && pil < npil) {
+                    const float4 g0 = P.geo[2 * (j0 + pt_j)];
+                    const float4 g1 = P.geo[2 * (j0 + pt_j) + 1];
+                    const float dxs = pox - g0.x, dys = poy - g0.y;
+                    const float dxr = pox - g0.z, dyr = poy - g0.w;
+                    const float dxm = pox - g1.x, dym = poy - g1.y;
+                    const float dm = sqrtf(dxm * dxm + dym * dym);
+                    if (dm <= P.max_ap) {
+                        inap = 1;
+                        const float ds2 = dxs * dxs + dys * dys;
+                        const float dr2 = dxr * dxr + dyr * dyr;
+                        if (TT == PSTM_TT_CURVED) {
+                            const float a_s = P.ck2 * ds2, a_r = P.ck2 * dr2;
+                            const float den_s = P.v0 + sqrtf(P.v0 * P.v0 + a_s);
+                            const float den_r = P.v0 + sqrtf(P.v0 * P.v0 + a_r);
+                            e = make_float4(a_s, a_r, dm, g1.z);
+                            f = make_float4(1.0f / (den_s * den_r), 0.0f, 0.0f, 0.0f);
+                        } else {
+                            e = make_float4(ds2, dr2, dm, g1.z);
+                            f = make_float4(dxs, dys, dxr, dyr);
+                        }
+                    }
+                }
+                pairs[(pt_p * P.batch + pt_j) * kPairVec] = e;
+                if (kPairVec == 2) pairs[(pt_p * P.batch + pt_j) * kPairVec + 1] = f;
+            }
+            if (!__syncthreads_or(inap)) continue;
+
+            // stage the batch of trace rows in LDS (coalesced)
+            if (AA) {
+                const double* src = P.prow + (size_t)j0 * P.rowlen;
+                double* dst = reinterpret_cast<double*>(rows);
+                const int n = nb * P.rowlen;
+                for (int e = tid; e < n; e += kThreads) dst[e] = src[e];
+            } else {
+                float* dst = reinterpret_cast<float*>(rows);
+                for (int r = 0; r < nb; ++r) {
+                    const int row = P.tidx ? P.tidx[j0 + r] : j0 + r;
+                    const float* src = P.amp + (size_t)row * P.ns;
+                    for (int e = tid; e < P.ns; e += kThreads) dst[r * P.rowlen + e] = src[e];
+                }
+            }
+            __syncthreads();
+
+            for (int jj = 0; jj < nb; ++jj) {
+                const Row* row = rows + (size_t)jj * P.rowlen;
+#pragma unroll
+                for (int p = 0; p < kPW; ++p) {
+                    const int pi = (pg * kPW + p) * P.batch + jj;
+                    const float4 e = pairs[pi * kPairVec];
+                    if (!(e.z <= P.max_ap)) continue;  // pillar-trace pair outside max aperture
+                    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
+#pragma unroll
+                    for (int m = 0; m < MW; ++m) {
+                        const float xt = fmaf(e.z, ci[p][m], cn[p][m]);
+                        if (!(xt <= 1.0f)) continue;  // time-dependent aperture
+                        float t, y1 = 0.f, y2 = 0.f;
+                        if (TT == PSTM_TT_STRAIGHT) {
+                            const float a1 = fmaf(e.x, cs[p][m], q[m]);
+                            const float a2 = fmaf(e.y, cs[p][m], q[m]);
+                            if (AA) {
+                                y1 = a1 > 0.f ? frsq(a1) : 0.f;
+                                y2 = a2 > 0.f ? frsq(a2) : 0.f;
+                                t = a1 * y1 + a2 * y2;
+                            } else {
+                                t = fsqrt(a1) + fsqrt(a2);
+                            }
+                        } else if (TT == PSTM_TT_CURVED) {
+                            // t = ln[(vz + sqrt(vz^2 + k^2 hs^2)) (vz + sqrt(vz^2 + k^2 hr^2))
+                            //        / (den_s den_r)] / k   (curved_ray.py:43-52, both legs)
+                            const float ns_ = cs[p][m] + fsqrt(q[m] + e.x);
+                            const float nr_ = cs[p][m] + fsqrt(q[m] + e.y);
+                            t = flog2((ns_ * nr_) * f.x) * P.ik_ln2;
+                        } else {
+                            // VTI one-way legs (anisotropy_vti.py:181-220) with
+                            // s = 1/v^2: t^2 = tau^2 + h^2 s - 2 eta (h^2 s)^2 / (tau^2 + (1+2 eta) h^2 s)
+                            const float hs = e.x * cs[p][m], hr = e.y * cs[p][m];
+                            const float ts2 = q[m] + hs - e2[p][m] * hs * hs * frcp(fmaf(e1[p][m], hs, q[m]));
+                            const float tr2 = q[m] + hr - e2[p][m] * hr * hr * frcp(fmaf(e1[p][m], hr, q[m]));
+                            t = fsqrt(fmaxf(ts2, 0.f)) + fsqrt(fmaxf(tr2, 0.f));
+                            if (!(t0r[m] > 0.f)) t = t0r[m];
+                        }
+                        const float u = fmaf(t, P.idt, P.uoff);
+                        if (!(u >= 0.0f && u < P.umax)) continue;  // sample bounds
+                        const int i0 = (int)u;
+                        const float f0 = u - (float)i0;
+                        float amp;
+                        if (!AA) {
+                            const float s0 = row[i0];
+                            const float s1 = row[i0 + 1];
+                            amp = fmaf(f0, s1 - s0, s0);
+                        } else {
+                            // triangle half width from the operator time dip
+                            const float gx = cs[p][m] * fmaf(f.x, y1, f.z * y2);
+                            const float gy = cs[p][m] * fmaf(f.y, y1, f.w * y2);
+                            float L = fmaxf(fabsf(gx) * P.gdx_idt, fabsf(gy) * P.gdy_idt);
+                            L = fminf(fmaxf(L, 1.0f), P.lmax);
+                            // blend of the integer half widths lf and lf+1,
+                            // each a second difference of the f64 double
+                            // running sum, interpolated between i0 and i0+1
+                            const int lf = (int)L;
+                            const float lam = L - (float)lf;
+                            const Row* r0 = row + P.lo;
+                            const double am1 = r0[i0 - lf - 1], am0 = r0[i0 - lf], am1p = r0[i0 - lf + 1];
+                            const double c0 = r0[i0], c1 = r0[i0 + 1];
+                            const double b0 = r0[i0 + lf], b1 = r0[i0 + lf + 1], b2 = r0[i0 + lf + 2];
+                            const float dl0 = (float)((b0 - c0) - (c0 - am0));   // T_lf[i0] * lf^2
+                            const float dh0 = (float)((b1 - c0) - (c0 - am1));   // T_lf+1[i0] * (lf+1)^2
+                            const float dl1 = (float)((b1 - c1) - (c1 - am1p));  // T_lf[i0+1] * lf^2
+                            const float dh1 = (float)((b2 - c1) - (c1 - am0));   // T_lf+1[i0+1] * (lf+1)^2
+                            const float il2 = frcp((float)(lf * lf));
+                            const float ih2 = frcp((float)((lf + 1) * (lf + 1)));
+                            const float wl = (1.0f - lam) * il2, wh = lam * ih2;
+                            const float tr0 = fmaf(dl0, wl, dh0 * wh);
+                            const float tr1 = fmaf(dl1, wl, dh1 * wh);
+                            amp = fmaf(f0, tr1 - tr0, tr0);
+                        }
+                        float w = e.w;
+                        if (xt > 0.0f) w *= taper_weight(xt);
+                        if (P.wpow != 0 && t > 0.001f) {
+                            const float r = frcp(t);
+                            w *= cw[p][m] * (P.wpow == 2 ? r * r : r);
+                        }
+                        acc[p][m] = fmaf(amp, w, acc[p][m]);
+                        if (P.cnt_part) ++cnt;
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < kPW; ++p)
+#pragma unroll
+                for (int m = 0; m < MW; ++m) {
+                    tot[p][m] += (double)acc[p][m];
+                    acc[p][m] = 0.0f;
+                }
+            __syncthreads();
+        }
+
+        // the workgroup is the only writer of these cells
+#pragma unroll
+        for (int m = 0; m < MW; ++m) {
+            const int k = kbase + 64 * (tw + kTW * m) + lane;
+            if (k >= P.nt) continue;
+#pragma unroll
+            for (int p = 0; p < kPW; ++p) {
+                const int pil = pil0 + pg * kPW + p;
+                if (pil < npil) {
+                    float* dst = P.image + (size_t)pil * P.nt + k;
+                    *dst = *dst + (float)tot[p][m];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    if (P.cnt_part) {
+        __shared__ unsigned long long red[kWaves];
+        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+        if (lane == 0) red[wave] = cnt;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long s = 0;
+            for (int w = 0; w < kWaves; ++w) s += red[w];
+            P.cnt_part[blockIdx.x] = s;
+        }
+    }
+}
+
+__global__ void k_sum_counts(const unsigned long long* part, int n, unsigned long long* out) {
+    __shared__ unsigned long long red[256];
+    unsigned long long s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out += red[0];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct DeviceState {
+    int device = -1;
+    int n_cu = 0;
+    void* ws = nullptr;
+    size_t ws_size = 0;
+};
+
+DeviceState g_state;
+
+int ensure_workspace(size_t bytes) {
+    if (bytes <= g_state.ws_size) return 0;
+    if (g_state.ws) {
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(g_state.ws));
+        g_state.ws = nullptr;
+        g_state.ws_size = 0;
+    }
+    const size_t sz = std::max(bytes, (size_t)1 << 20);
+    HIP_TRY(hipMalloc(&g_state.ws, sz));
+    g_state.ws_size = sz;
+    return 0;
+}
+
+int ensure_init() {
+    if (g_state.device >= 0) {
+        int cur = -1;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur == g_state.device) return 0;
+        return fail(4, "pstm_hip: current device differs from the device passed to pstm_hip_init");
+    }
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    g_state.device = dev;
+    g_state.n_cu = prop.multiProcessorCount;
+    return 0;
+}
+
+int validate(const PstmMigrateArgs* a) {
+    if (!a) return fail(1, "args is NULL");
+    if (a->n_traces < 0 || a->n_samples < 2 || a->nx <= 0 || a->ny <= 0 || a->nt <= 0)
+        return fail(1, "invalid sizes (need n_traces >= 0, n_samples >= 2, nx, ny, nt > 0)");
+    if (!a->x_axis || !a->y_axis || !a->t_axis_ms || !a->vrms || !a->image || !a->fold)
+        return fail(1, "axis, velocity, image and fold pointers must be set");
+    if (a->n_traces > 0 && (!a->amplitudes || !a->source_x || !a->source_y || !a->receiver_x ||
+                            !a->receiver_y || !a->midpoint_x || !a->midpoint_y))
+        return fail(1, "trace pointers must be set");
+    if (a->vel_mode != PSTM_VEL_1D && a->vel_mode != PSTM_VEL_3D) return fail(1, "invalid vel_mode");
+    if (a->traveltime < PSTM_TT_STRAIGHT || a->traveltime > PSTM_TT_VTI) return fail(1, "invalid traveltime");
+    if (a->aa_mode != PSTM_AA_NONE && a->aa_mode != PSTM_AA_TRIANGLE) return fail(1, "invalid aa_mode");
+    if (a->aa_mode == PSTM_AA_TRIANGLE && a->traveltime != PSTM_TT_STRAIGHT)
+        return fail(1, "triangle anti-aliasing is implemented for the straight-ray traveltime only");
+    if (a->aa_mode == PSTM_AA_TRIANGLE && (a->aa_max_half_width < 1 || a->aa_max_half_width > 64))
+        return fail(1, "aa_max_half_width must be in [1, 64]");
+    if (a->traveltime == PSTM_TT_VTI) {
+        if (a->eta_mode < PSTM_ETA_CONST || a->eta_mode > PSTM_ETA_3D) return fail(1, "invalid eta_mode");
+        if (a->eta_mode != PSTM_ETA_CONST && !a->eta) return fail(1, "eta pointer must be set");
+    }
+    if (a->traveltime == PSTM_TT_CURVED && !(a->curved_v0 > 0.0))
+        return fail(1, "curved_v0 must be positive");
+    if (!(a->dt_in_ms > 0.0)) return fail(1, "dt_in_ms must be positive");
+    if (!(a->max_aperture_m > 0.0)) return fail(1, "max_aperture_m must be positive");
+    if ((long long)a->nx * a->ny * a->nt > (1LL << 40)) return fail(1, "tile too large");
+    return 0;
+}
+
+template <int MW, int TT, bool AA>
+int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
+    auto fn = k_migrate<MW, TT, AA>;
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, st, kp);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <int TT, bool AA>
+int dispatch_mw(int mw, const KParams& kp, int grid, size_t lds, hipStream_t st) {
+    switch (mw) {
+        case 1: return launch_migrate<1, TT, AA>(kp, grid, lds, st);
+        case 2: return launch_migrate<2, TT, AA>(kp, grid, lds, st);
+        case 3: return launch_migrate<3, TT, AA>(kp, grid, lds, st);
+        default: return launch_migrate<4, TT, AA>(kp, grid, lds, st);
+    }
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// shared by migrate_tile and migrate_bins: `ntr` rows starting at row
+// offset `row0` of args->trace_index (or of the block).
+int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hipStream_t st) {
+    int rc = validate(in);
+    if (rc) return rc;
+    rc = ensure_init();
+    if (rc) return rc;
+    PstmMigrateArgs a = *in;
+    a.n_traces = ntr;
+    if (a.trace_index) a.trace_index += row0;
+    else if (row0 > 0) {
+        // rows [row0, row0+ntr) of the block: shift the per-trace arrays
+        a.amplitudes += (size_t)row0 * a.n_samples;
+        a.source_x += row0; a.source_y += row0;
+        a.receiver_x += row0; a.receiver_y += row0;
+        a.midpoint_x += row0; a.midpoint_y += row0;
+        if (a.trace_weights) a.trace_weights += row0;
+    }
+    if (ntr == 0) return 0;
+    const int nt = a.nt, ns = a.n_samples;
+    const int npil = a.nx * a.ny;
+    const int npv = a.vel_mode == PSTM_VEL_3D ? npil : 1;
+    const int nev = a.eta_mode == PSTM_ETA_3D ? npil : 1;
+    int tt = a.traveltime;
+    if (tt == PSTM_TT_CURVED && fabs(a.curved_k) < 1e-10) tt = PSTM_TT_STRAIGHT;
+    const bool aa = a.aa_mode == PSTM_AA_TRIANGLE;
+    const int lmax = aa ? a.aa_max_half_width : 0;
+    const int lo = aa ? lmax + 1 : 0;
+    const int rowlen = aa ? ns + 2 * lmax + 2 : ns;
+    const size_t elem = aa ? sizeof(double) : sizeof(float);
+    const int pair_vec = (aa || tt == PSTM_TT_CURVED) ? 2 : 1;
+
+    // batch size from the LDS budget
+    int batch = kMaxBatch;
+    while (batch > 1 && (size_t)batch * rowlen * elem + (size_t)kNPil * batch * pair_vec * 16 > kLdsBudget)
+        --batch;
+    const size_t lds = align_up((size_t)batch * rowlen * elem, 16) + (size_t)kNPil * batch * pair_vec * 16;
+    if (lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
+    // time windows
+    const int nwin = (nt + 1023) / 1024;
+    const int per_win = (nt + nwin - 1) / nwin;
+    const int mw = std::min(4, std::max(1, (per_win + 255) / 256));
+    const int win = 256 * mw;
+    const int nwin2 = (nt + win - 1) / win;
+    const int n_pgroups = (npil + kNPil - 1) / kNPil;
+    const long n_items = (long)n_pgroups * nwin2;
+    if (n_items > (1L << 30)) return fail(1, "tile too large");
+    const int occupancy = lds <= kLdsBudget / 2 ? 2 : 1;  // workgroups per CU the LDS allows
+    const int grid = (int)std::min<long>(n_items, (long)g_state.n_cu * occupancy);
+
+    // workspace layout
+    size_t off = 0;
+    const size_t o_geo = off; off = align_up(off + (size_t)ntr * 32, 256);
+    const size_t o_kc = off; off = align_up(off + (size_t)npv * nt * 16, 256);
+    const size_t o_qk = off; off = align_up(off + (size_t)nt * 4, 256);
+    const size_t o_t0 = off; off = align_up(off + (size_t)nt * 4, 256);
+    const size_t o_ek = off; off = align_up(off + (tt == PSTM_TT_VTI ? (size_t)nev * nt * 8 : 0), 256);
+    const size_t o_cnt = off; off = align_up(off + (size_t)grid * 8, 256);
+    const size_t o_prow = off; off = align_up(off + (aa ? (size_t)ntr * rowlen * 8 : 0), 256);
+    rc = ensure_workspace(off);
+    if (rc) return rc;
+    char* ws = static_cast<char*>(g_state.ws);
+    float4* geo = reinterpret_cast<float4*>(ws + o_geo);
+    float4* kc = reinterpret_cast<float4*>(ws + o_kc);
+    float* qk = reinterpret_cast<float*>(ws + o_qk);
+    float* t0k = reinterpret_cast<float*>(ws + o_t0);
+    float2* ek = reinterpret_cast<float2*>(ws + o_ek);
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(ws + o_cnt);
+    double* prow = reinterpret_cast<double*>(ws + o_prow);
+
+    hipLaunchKernelGGL(k_prep_geometry, dim3((ntr + 255) / 256), dim3(256), 0, st, a, geo);
+    HIP_TRY(hipGetLastError());
+    const double tan_dip = tan(a.max_dip_deg * M_PI / 180.0);
+    const long nkc = (long)npv * nt;
+    hipLaunchKernelGGL(k_prep_time_terms, dim3((unsigned)((nkc + 255) / 256)), dim3(256), 0, st, a, tt, npv, tan_dip,
+                       kc, qk, t0k);
+    HIP_TRY(hipGetLastError());
+    if (tt == PSTM_TT_VTI) {
+        const long nek = (long)nev * nt;
+        hipLaunchKernelGGL(k_prep_eta, dim3((unsigned)((nek + 255) / 256)), dim3(256), 0, st, a, nev, ek);
+        HIP_TRY(hipGetLastError());
+    }
+    if (aa) {
+        hipLaunchKernelGGL(k_integrate, dim3(ntr), dim3(256), 0, st, a, rowlen, lo, prow);
+        HIP_TRY(hipGetLastError());
+    }
+    if (do_fold) {
+        hipLaunchKernelGGL(k_fold, dim3(npil), dim3(256), 0, st, a);
+        HIP_TRY(hipGetLastError());
+    }
+
+    KParams kp;
+    kp.amp = a.amplitudes;
+    kp.tidx = a.trace_index;
+    kp.geo = geo;
+    kp.prow = prow;
+    kp.kc = kc;
+    kp.qk = qk;
+    kp.ek = ek;
+    kp.t0k = t0k;
+    kp.xax = a.x_axis;
+    kp.yax = a.y_axis;
+    kp.image = a.image;
+    kp.cnt_part = a.n_effective ? cnt : nullptr;
+    kp.ns = ns;
+    kp.ntr = ntr;
+    kp.nx = a.nx;
+    kp.ny = a.ny;
+    kp.nt = nt;
+    kp.kc_pstride = npv > 1 ? nt : 0;
+    kp.ek_pstride = nev > 1 ? nt : 0;
+    kp.rowlen = rowlen;
+    kp.lo = lo;
+    kp.batch = batch;
+    kp.pair_off = (int)align_up((size_t)batch * rowlen * elem, 16);
+    kp.n_items = (int)n_items;
+    kp.n_pgroups = n_pgroups;
+    kp.wpow = (a.apply_spreading ? 1 : 0) + (a.apply_obliquity ? 1 : 0);
+    const double dt_s = a.dt_in_ms / 1000.0;
+    kp.idt = (float)(1.0 / dt_s);
+    kp.uoff = (float)(-(a.t_start_in_ms / 1000.0) / dt_s);
+    kp.umax = (float)(ns - 1);
+    kp.max_ap = (float)a.max_aperture_m;
+    kp.gdx_idt = (float)(a.grid_dx / dt_s);
+    kp.gdy_idt = (float)(a.grid_dy / dt_s);
+    kp.lmax = (float)lmax;
+    kp.ck2 = (float)(a.curved_k * a.curved_k);
+    kp.ik_ln2 = (float)(tt == PSTM_TT_CURVED ? M_LN2 / a.curved_k : 0.0);
+    kp.v0 = (float)a.curved_v0;
+
+    if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, kp, grid, lds, st);
+    else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, kp, grid, lds, st);
+    else if (tt == PSTM_TT_CURVED) rc = dispatch_mw<PSTM_TT_CURVED, false>(mw, kp, grid, lds, st);
+    else rc = dispatch_mw<PSTM_TT_VTI, false>(mw, kp, grid, lds, st);
+    if (rc) return rc;
+    if (a.n_effective) {
+        hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(256), 0, st, (const unsigned long long*)cnt, grid,
+                           a.n_effective);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pstm_hip_abi_version(void) { return PSTM_HIP_ABI_VERSION; }
+
+size_t pstm_hip_args_size(void) { return sizeof(PstmMigrateArgs); }
+
+const char* pstm_hip_last_error(void) { return g_last_error.c_str(); }
+
+int pstm_hip_init(int device) {
+    g_last_error.clear();
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(1, "pstm_hip_init: no such device");
+    HIP_TRY(hipSetDevice(device));
+    if (g_state.device >= 0 && g_state.device != device) {
+        return fail(4, "pstm_hip_init: library already bound to another device (call pstm_hip_release first)");
+    }
+    g_state.device = -1;
+    return ensure_init();
+}
+
+int pstm_hip_release(void) {
+    g_last_error.clear();
+    if (g_state.ws) {
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(g_state.ws));
+    }
+    g_state.ws = nullptr;
+    g_state.ws_size = 0;
+    g_state.device = -1;
+    return 0;
+}
+
+int pstm_hip_stream_sync(void* stream) {
+    HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return 0;
+}
+
+int pstm_hip_migrate_tile(const PstmMigrateArgs* args, void* stream) {
+    g_last_error.clear();
+    if (!args) return fail(1, "args is NULL");
+    return run_migration(args, 0, args->n_traces, 1, static_cast<hipStream_t>(stream));
+}
+
+int pstm_hip_fold_tile(const PstmMigrateArgs* args, void* stream) {
+    g_last_error.clear();
+    int rc = validate(args);
+    if (rc) return rc;
+    rc = ensure_init();
+    if (rc) return rc;
+    if (args->n_traces == 0) return 0;
+    hipLaunchKernelGGL(k_fold, dim3(args->nx * args->ny), dim3(256), 0, static_cast<hipStream_t>(stream), *args);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_start, int32_t n_groups, void* stream) {
+    g_last_error.clear();
+    if (!args || !group_start) return fail(1, "args and group_start must be set");
+    if (n_groups < 0) return fail(1, "n_groups must be >= 0");
+    if (n_groups == 0) return 0;
+    for (int g = 0; g < n_groups; ++g)
+        if (group_start[g + 1] < group_start[g]) return fail(1, "group_start must be non-decreasing");
+    if (group_start[0] < 0 || group_start[n_groups] > args->n_traces)
+        return fail(1, "group_start outside [0, n_traces]");
+    // every group accumulates into the same image and fold: one gather over
+    // the concatenated rows, each workgroup owning its pillars for all groups
+    return run_migration(args, group_start[0], group_start[n_groups] - group_start[0], 1,
+                         static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+"""ctypes mirror of the C ABI declared in include/pstm_hip.h.
+
+The reference marshals kernel parameters through ctypes structures mirrored
+by hand in Metal shader structs (pstm/kernels/metal_compiled.py:137-289); its
+CurvedRayParams and VTI bindings drifted out of sync with the shaders.  Here
+the layout is declared once in C, mirrored once below, and checked against the
+library's own ``sizeof`` at load time (``check_layout``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+ABI_VERSION = 1
+
+TT_STRAIGHT, TT_CURVED, TT_VTI = 0, 1, 2
+AA_NONE, AA_TRIANGLE = 0, 1
+VEL_1D, VEL_3D = 0, 1
+ETA_CONST, ETA_1D, ETA_3D = 0, 1, 2
+
+_vp = ctypes.c_void_p
+
+
+class PstmMigrateArgs(ctypes.Structure):
+    """Field-for-field mirror of ``struct PstmMigrateArgs``."""
+
+    _fields_ = [
+        ("amplitudes", _vp), ("source_x", _vp), ("source_y", _vp),
+        ("receiver_x", _vp), ("receiver_y", _vp),
+        ("midpoint_x", _vp), ("midpoint_y", _vp),
+        ("trace_weights", _vp), ("trace_index", _vp),
+        ("x_axis", _vp), ("y_axis", _vp), ("t_axis_ms", _vp),
+        ("vrms", _vp), ("eta", _vp),
+        ("image", _vp), ("fold", _vp), ("n_effective", _vp),
+        ("n_traces", ctypes.c_int32), ("n_samples", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nt", ctypes.c_int32),
+        ("vel_mode", ctypes.c_int32), ("eta_mode", ctypes.c_int32),
+        ("traveltime", ctypes.c_int32), ("aa_mode", ctypes.c_int32),
+        ("apply_spreading", ctypes.c_int32),
+        ("apply_obliquity", ctypes.c_int32),
+        ("aa_max_half_width", ctypes.c_int32),
+        ("dt_in_ms", ctypes.c_double), ("t_start_in_ms", ctypes.c_double),
+        ("max_aperture_m", ctypes.c_double),
+        ("min_aperture_m", ctypes.c_double),
+        ("max_dip_deg", ctypes.c_double), ("taper_fraction", ctypes.c_double),
+        ("grid_dx", ctypes.c_double), ("grid_dy", ctypes.c_double),
+        ("curved_v0", ctypes.c_double), ("curved_k", ctypes.c_double),
+        ("eta_constant", ctypes.c_double),
+    ]
+
+
+#: every symbol include/pstm_hip.h declares
+EXPORTED_SYMBOLS = (
+    "pstm_hip_abi_version", "pstm_hip_args_size", "pstm_hip_last_error",
+    "pstm_hip_init", "pstm_hip_release", "pstm_hip_stream_sync",
+    "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
+)
+
+PACKAGE_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PACKAGE_DIR / "lib" / "libpstm_hip.so"
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("PSTM_HIP_LIB", str(LIB_PATH)))
+
+
+def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
+    """Attach argument/return types to every exported entry point."""
+    P = ctypes.POINTER(PstmMigrateArgs)
+    lib.pstm_hip_abi_version.restype = ctypes.c_int
+    lib.pstm_hip_abi_version.argtypes = []
+    lib.pstm_hip_args_size.restype = ctypes.c_size_t
+    lib.pstm_hip_args_size.argtypes = []
+    lib.pstm_hip_last_error.restype = ctypes.c_char_p
+    lib.pstm_hip_last_error.argtypes = []
+    lib.pstm_hip_init.restype = ctypes.c_int
+    lib.pstm_hip_init.argtypes = [ctypes.c_int]
+    lib.pstm_hip_release.restype = ctypes.c_int
+    lib.pstm_hip_release.argtypes = []
+    lib.pstm_hip_stream_sync.restype = ctypes.c_int
+    lib.pstm_hip_stream_sync.argtypes = [_vp]
+    lib.pstm_hip_migrate_tile.restype = ctypes.c_int
+    lib.pstm_hip_migrate_tile.argtypes = [P, _vp]
+    lib.pstm_hip_fold_tile.restype = ctypes.c_int
+    lib.pstm_hip_fold_tile.argtypes = [P, _vp]
+    lib.pstm_hip_migrate_bins.restype = ctypes.c_int
+    lib.pstm_hip_migrate_bins.argtypes = [P, _vp, ctypes.c_int32, _vp]
+    return lib
+
+
+def check_layout(lib: ctypes.CDLL) -> None:
+    """Refuse a library whose ABI differs from this mirror."""
+    ver = lib.pstm_hip_abi_version()
+    if ver != ABI_VERSION:
+        raise RuntimeError(f"libpstm_hip ABI version {ver} != {ABI_VERSION}")
+    size = lib.pstm_hip_args_size()
+    if size != ctypes.sizeof(PstmMigrateArgs):
+        raise RuntimeError(
+            f"PstmMigrateArgs layout mismatch: C sizeof {size} != ctypes "
+            f"{ctypes.sizeof(PstmMigrateArgs)}")
+
+
+_LIB: ctypes.CDLL | None = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Load the built HIP library (no device work happens at load time).
+
+    Raises RuntimeError when the library has not been built: the backend has no
+    CPU fallback.
+    """
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not path.exists():
+        raise RuntimeError(
+            f"HIP backend library not built: {path} is missing "
+            "(run __graft_entry__.build() or python -m pm_amd.build)")
+    lib = declare(ctypes.CDLL(str(path)))
+    check_layout(lib)
+    _LIB = lib
+    return lib
+
+
+def last_error(lib: ctypes.CDLL) -> str:
+    msg = lib.pstm_hip_last_error()
+    return msg.decode() if msg else ""

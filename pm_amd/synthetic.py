@@ -1,0 +1,197 @@
+"""Seeded synthetic migration jobs: the workloads the backend is measured on.
+
+Geometry and event parameters come from ``numpy.random.default_rng(seed)``
+(PCG64) in float64; traces are synthesised as tensor arithmetic on the chosen
+device (the 200k x 1501 job is built on the GPU in well under a second) and
+cast to float32.  Ricker wavelet ``(1 - 2 pi^2 f^2 tau^2) exp(-pi^2 f^2 tau^2)``
+as the reference's synthetic generator uses (tests/fixtures/synthetic.py:21-44).
+
+Jobs:
+  * ``point_diffractor_job`` — one diffractor at (1000, 1000) m, t0 0.8 s,
+    v 2000 m/s, 4096 traces x 1001 samples, 64 x 64 x 500 output at 25 m,
+    aperture 1000 m, weights on, no anti-aliasing.
+  * ``reflector_job`` — 200,000 traces x 1501 samples, 40 dipping planes and
+    200 diffractors, 3-D v_rms cube 64 x 64 x 151 with +-10 % lateral bumps,
+    256 x 256 x 1000 output at 12.5 m in 64 x 64 tiles, aperture 2500 m, dip
+    60 deg, triangle anti-aliasing.
+Sizes are parameters so that tests can run the same recipes scaled down.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from pm_amd.kernels.base import KernelConfig
+from pm_amd.velocity import VelocityCube
+
+
+@dataclass
+class SyntheticJob:
+    """Everything one migration job needs, amplitudes as a torch tensor."""
+
+    name: str
+    amplitudes: Any                 # torch (n_traces, n_samples) f32
+    source_x: np.ndarray
+    source_y: np.ndarray
+    receiver_x: np.ndarray
+    receiver_y: np.ndarray
+    dt_ms: float
+    t_start_ms: float
+    x_axis: np.ndarray
+    y_axis: np.ndarray
+    t_axis_ms: np.ndarray
+    config: KernelConfig
+    vrms_1d: np.ndarray | None = None        # (nt,) when the velocity is 1-D
+    cube: VelocityCube | None = None         # 3-D v_rms model
+    tile: int = 0                            # output tile edge (0: one tile)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def midpoint_x(self) -> np.ndarray:
+        return (self.source_x + self.receiver_x) / 2.0
+
+    @property
+    def midpoint_y(self) -> np.ndarray:
+        return (self.source_y + self.receiver_y) / 2.0
+
+    @property
+    def n_traces(self) -> int:
+        return int(self.amplitudes.shape[0])
+
+    @property
+    def n_samples(self) -> int:
+        return int(self.amplitudes.shape[1])
+
+    def tiles(self) -> list[tuple[int, int, int, int]]:
+        """(x0, x1, y0, y1) index ranges of the output tiles."""
+        nx, ny = len(self.x_axis), len(self.y_axis)
+        t = self.tile or max(nx, ny)
+        return [(x0, min(x0 + t, nx), y0, min(y0 + t, ny))
+                for x0 in range(0, nx, t) for y0 in range(0, ny, t)]
+
+
+def ricker(tau, f_hz):
+    a = (np.pi * f_hz * tau) ** 2
+    return (1.0 - 2.0 * a) * np.exp(-a)
+
+
+def add_events(traces, t_event, amp, f_hz: float, dt_s: float, t_start_s: float = 0.0):
+    """traces[j, n] += amp[j] * ricker(t_start + n dt - t_event[j]) over the
+    samples where the wavelet is above 1e-7 of its peak."""
+    import torch
+
+    n_tr, ns = traces.shape
+    half = int(np.ceil(4.5 / (np.pi * f_hz * dt_s)))
+    rows = torch.arange(n_tr, device=traces.device)
+    centre = torch.round((t_event - t_start_s) / dt_s).long()
+    for w in range(-half, half + 1):
+        idx = centre + w
+        ok = (idx >= 0) & (idx < ns)
+        tau = t_start_s + idx.double() * dt_s - t_event
+        val = torch.where(ok, amp * ricker(tau, f_hz), torch.zeros_like(tau))
+        traces[rows, idx.clamp(0, ns - 1)] += val.to(traces.dtype)
+    return traces
+
+
+def _geometry(rng, n, lo, hi, max_offset):
+    mx = rng.uniform(lo, hi, n)
+    my = rng.uniform(lo, hi, n)
+    h = rng.uniform(0.0, max_offset, n)
+    az = rng.uniform(0.0, 2.0 * np.pi, n)
+    c, s = np.cos(az), np.sin(az)
+    return mx - 0.5 * h * c, my - 0.5 * h * s, mx + 0.5 * h * c, my + 0.5 * h * s
+
+
+def point_diffractor_job(seed: int = 1, n_traces: int = 4096, n_samples: int = 1001,
+                         n_out: int = 64, nt: int = 500, device: str = "cpu") -> SyntheticJob:
+    import torch
+
+    rng = np.random.default_rng(seed)
+    sx, sy, rx, ry = _geometry(rng, n_traces, 0.0, 2000.0, 1500.0)
+    dx_, dy_, t0d, v = 1000.0, 1000.0, 0.8, 2000.0
+    dt_s = 0.002
+    t_d = (np.sqrt((t0d / 2) ** 2 + ((sx - dx_) ** 2 + (sy - dy_) ** 2) / v ** 2)
+           + np.sqrt((t0d / 2) ** 2 + ((rx - dx_) ** 2 + (ry - dy_) ** 2) / v ** 2))
+    tr = torch.zeros((n_traces, n_samples), dtype=torch.float64, device=device)
+    add_events(tr, torch.as_tensor(t_d, device=device), torch.ones(n_traces, dtype=torch.float64,
+               device=device), 25.0, dt_s)
+    peak = float(tr.abs().max())
+    noise = torch.as_tensor(rng.standard_normal((n_traces, n_samples)), device=device)
+    tr += 0.01 * peak * noise
+    x = 200.0 + 25.0 * np.arange(n_out)
+    cfg = KernelConfig(max_aperture_m=1000.0, min_aperture_m=1000.0, max_dip_degrees=45.0,
+                       taper_fraction=0.1, apply_spreading=True, apply_obliquity=True,
+                       grid_dx=25.0, grid_dy=25.0)
+    return SyntheticJob("point_diffractor", tr.float().contiguous(), sx, sy, rx, ry, 2.0, 0.0, x,
+                        x.copy(), 2.0 * np.arange(nt), cfg, vrms_1d=np.full(nt, v),
+                        meta={"diffractor": (dx_, dy_, t0d)})
+
+
+def lateral_cube(x_lo: float = -500.0, x_hi: float = 3687.5, n_xy: int = 64, n_t: int = 151,
+                 dt_ms: float = 20.0) -> VelocityCube:
+    """v = (1500 + 600 t0)(1 + 0.1 (g1 - g2)), two Gaussian bumps of sigma
+    800 m centred at (1000, 1000) and (2500, 2200) m."""
+    x = np.linspace(x_lo, x_hi, n_xy)
+    t = dt_ms * np.arange(n_t)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    sig = 800.0
+    g1 = np.exp(-((X - 1000.0) ** 2 + (Y - 1000.0) ** 2) / (2 * sig ** 2))
+    g2 = np.exp(-((X - 2500.0) ** 2 + (Y - 2200.0) ** 2) / (2 * sig ** 2))
+    v = (1500.0 + 600.0 * t / 1000.0)[None, None, :] * (1.0 + 0.1 * (g1 - g2))[:, :, None]
+    return VelocityCube(v, x, x.copy(), t)
+
+
+def reflector_job(seed: int = 2, n_traces: int = 200_000, n_samples: int = 1501,
+                  n_out: int = 256, nt: int = 1000, d_out: float = 12.5, tile: int = 64,
+                  n_planes: int = 40, n_diffractors: int = 200, antialias: str = "triangle",
+                  device: str = "cpu") -> SyntheticJob:
+    import torch
+
+    rng = np.random.default_rng(seed)
+    x_hi = d_out * (n_out - 1)
+    lo, hi = -500.0, x_hi + 500.0
+    sx, sy, rx, ry = _geometry(rng, n_traces, lo, hi, 3000.0)
+    cube = lateral_cube(lo, hi)
+    dt_s = 0.002
+    f = 30.0
+    dev = device
+    tr = torch.zeros((n_traces, n_samples), dtype=torch.float64, device=dev)
+    S = [torch.as_tensor(a, device=dev) for a in (sx, sy, rx, ry)]
+    xc = yc = 0.5 * x_hi
+    # dipping planes: mirror-image time in the plane-centre velocity
+    dips = rng.uniform(0.0, np.deg2rad(30.0), n_planes)
+    strikes = rng.uniform(0.0, 2.0 * np.pi, n_planes)
+    t0c = rng.uniform(0.3, 2.8, n_planes)
+    amps = rng.uniform(0.3, 1.0, n_planes) * rng.choice([-1.0, 1.0], n_planes)
+    vc = cube.at(np.full(n_planes, xc), np.full(n_planes, yc), 1000.0 * t0c)
+    for i in range(n_planes):
+        n_vec = np.array([np.sin(dips[i]) * np.cos(strikes[i]),
+                          np.sin(dips[i]) * np.sin(strikes[i]), np.cos(dips[i])])
+        p0 = np.array([xc, yc, vc[i] * t0c[i] / 2.0])  # plane point under the centre
+        d_s = (S[0] - p0[0]) * n_vec[0] + (S[1] - p0[1]) * n_vec[1] + (0.0 - p0[2]) * n_vec[2]
+        mx_ = S[0] - 2 * d_s * n_vec[0]
+        my_ = S[1] - 2 * d_s * n_vec[1]
+        mz_ = -2 * d_s * n_vec[2]
+        t = torch.sqrt((mx_ - S[2]) ** 2 + (my_ - S[3]) ** 2 + mz_ ** 2) / vc[i]
+        add_events(tr, t, torch.full_like(t, amps[i]), f, dt_s)
+    # point diffractors: DSR time with the local velocity
+    xd = rng.uniform(0.0, x_hi, n_diffractors)
+    yd = rng.uniform(0.0, x_hi, n_diffractors)
+    t0d = rng.uniform(0.2, 2.8, n_diffractors)
+    ad = rng.uniform(0.5, 1.0, n_diffractors)
+    vd = cube.at(xd, yd, 1000.0 * t0d)
+    for i in range(n_diffractors):
+        q = (t0d[i] / 2) ** 2
+        t = (torch.sqrt(q + ((S[0] - xd[i]) ** 2 + (S[1] - yd[i]) ** 2) / vd[i] ** 2)
+             + torch.sqrt(q + ((S[2] - xd[i]) ** 2 + (S[3] - yd[i]) ** 2) / vd[i] ** 2))
+        add_events(tr, t, torch.full_like(t, ad[i]), f, dt_s)
+    x = d_out * np.arange(n_out)
+    cfg = KernelConfig(max_aperture_m=2500.0, min_aperture_m=500.0, max_dip_degrees=60.0,
+                       taper_fraction=0.1, apply_spreading=True, apply_obliquity=True,
+                       grid_dx=d_out, grid_dy=d_out, antialias=antialias)
+    return SyntheticJob("reflectors", tr.float().contiguous(), sx, sy, rx, ry, 2.0, 0.0, x,
+                        x.copy(), 2.0 * np.arange(nt), cfg, cube=cube, tile=tile,
+                        meta={"diffractors": np.stack([xd, yd, t0d], 1)})

@@ -1,0 +1,286 @@
+"""GPU parity: the HIP backend against the CPU FP64 oracle on the same inputs.
+
+Bar: fold bit-exact (np.array_equal); FP32 image within relative L2
+RTOL = 1e-5 of the FP64 oracle (||gpu - cpu||_2 / ||cpu||_2), every result
+printed with its error against the FP64 run.  Every test calls the C ABI
+through the ``hip`` backend.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from pm_amd.kernels import (KernelConfig, VelocitySlice, create_kernel, create_output_tile,
+                            create_trace_block)
+from pm_amd.kernels import _abi
+from pm_amd.kernels.base import OutputTile
+from tests.conftest import golden_inputs
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+GOLDEN_CASES = ["weights_on", "weights_off", "spreading_only", "taper_dense", "accumulate",
+                "t0_edge", "const_v_small_ap"]
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def kernel():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    k = create_kernel("hip")
+    k.initialize(KernelConfig())
+    yield k
+    k.cleanup()
+
+
+def _config_from(inp: orc.MigrateInputs, **kw) -> KernelConfig:
+    return KernelConfig(max_aperture_m=inp.max_aperture_m, min_aperture_m=inp.min_aperture_m,
+                        max_dip_degrees=inp.max_dip_deg, taper_fraction=inp.taper_fraction,
+                        apply_spreading=inp.apply_spreading, apply_obliquity=inp.apply_obliquity,
+                        grid_dx=inp.grid_dx, grid_dy=inp.grid_dy, curved_ray_v0=inp.curved_v0,
+                        curved_ray_k=inp.curved_k, vti_eta_constant=inp.eta_constant,
+                        aa_max_half_width=inp.aa_max_half_width,
+                        antialias="triangle" if inp.aa_mode == _abi.AA_TRIANGLE else "none", **kw)
+
+
+def _run_hip(kernel, inp: orc.MigrateInputs, image0=None, fold0=None, fold3d=True, method=None):
+    tb = create_trace_block(inp.amplitudes, inp.source_x, inp.source_y, inp.receiver_x,
+                            inp.receiver_y, inp.dt_in_ms, inp.t_start_in_ms, inp.trace_weights)
+    nx, ny, nt = len(inp.x_axis), len(inp.y_axis), len(inp.t_axis_ms)
+    img = np.zeros((nx, ny, nt)) if image0 is None else image0.copy()
+    fshape = (nx, ny, nt) if fold3d else (nx, ny)
+    fold = np.zeros(fshape, np.int32) if fold0 is None else fold0.copy()
+    out = OutputTile(img, fold, np.asarray(inp.x_axis, float), np.asarray(inp.y_axis, float),
+                     np.asarray(inp.t_axis_ms, float))
+    v = np.asarray(inp.vrms, float)
+    vel = VelocitySlice(v, is_1d=(v.ndim == 1))
+    cfg = _config_from(inp, vti_eta_array=inp.eta)
+    if method == "curved":
+        kernel.migrate_tile_curved_ray(tb, out, vel, cfg)
+    elif method == "vti":
+        kernel.migrate_tile_vti(tb, out, vel, cfg)
+    else:
+        kernel.migrate_tile(tb, out, vel, cfg)
+    return out
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_golden_cases(kernel, golden_dir, case):
+    g = np.load(golden_dir / f"migrate_{case}.npz")
+    inp = golden_inputs(g)
+    out = _run_hip(kernel, inp, image0=g["image_init"], fold0=g["fold_init"])
+    np.testing.assert_array_equal(out.fold, g["fold"])
+    ref = orc.migrate(inp, image_init=g["image_init"])
+    err_golden = rel_l2(out.image, g["image"])
+    err_oracle = rel_l2(out.image, ref.image)
+    print(f"[{case}] rel-L2 vs reference golden {err_golden:.3e}, vs FP64 oracle {err_oracle:.3e}")
+    assert err_golden <= RTOL and err_oracle <= RTOL
+
+
+def _random(seed, nx=9, ny=7, nt=150, ntr=300, ns=400, vel3d=False, **kw):
+    rng = np.random.default_rng(seed)
+    mx, my = rng.uniform(-300, 700, ntr), rng.uniform(-300, 600, ntr)
+    h, az = rng.uniform(0, 1200, ntr), rng.uniform(0, 2 * np.pi, ntr)
+    sx, sy = mx - h / 2 * np.cos(az), my - h / 2 * np.sin(az)
+    rx, ry = mx + h / 2 * np.cos(az), my + h / 2 * np.sin(az)
+    t = 4.0 * np.arange(nt)
+    v = 1700 + 400 * t / 1000
+    if vel3d:
+        v = v[None, None, :] * (1 + 0.12 * np.linspace(0, 1, nx)[:, None, None]
+                                - 0.08 * np.linspace(0, 1, ny)[None, :, None])
+    # UTM-like absolute coordinates: f64 reduction to tile-relative offsets matters
+    x0, y0 = 620000.0, 5100000.0
+    args = dict(amplitudes=rng.standard_normal((ntr, ns)).astype(np.float32),
+                source_x=sx + x0, source_y=sy + y0, receiver_x=rx + x0, receiver_y=ry + y0,
+                midpoint_x=(sx + rx) / 2 + x0, midpoint_y=(sy + ry) / 2 + y0,
+                x_axis=x0 + 40.0 * np.arange(nx), y_axis=y0 + 50.0 * np.arange(ny), t_axis_ms=t,
+                vrms=v, dt_in_ms=2.0, trace_weights=rng.uniform(0.5, 1.5, ntr),
+                max_aperture_m=600.0, min_aperture_m=150.0, max_dip_deg=50.0, taper_fraction=0.15)
+    args.update(kw)
+    return orc.MigrateInputs(**args)
+
+
+@pytest.mark.parametrize("vel3d", [False, True])
+@pytest.mark.parametrize("shape", [(9, 7, 150), (3, 5, 1300), (16, 16, 64), (1, 1, 1)])
+def test_random_tiles(kernel, shape, vel3d):
+    """Odd pillar counts (partial workgroups), nt below one wave, above one
+    time window, UTM-sized absolute coordinates."""
+    nx, ny, nt = shape
+    inp = _random(7 + nt, nx=nx, ny=ny, nt=nt, vel3d=vel3d)
+    out = _run_hip(kernel, inp, fold3d=False)
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(out.fold, ref.fold2d)
+    err = rel_l2(out.image, ref.image)
+    print(f"[{shape} vel3d={vel3d}] rel-L2 {err:.3e}, fold sum {int(ref.fold2d.sum())}")
+    if np.linalg.norm(ref.image) > 0:
+        assert err <= RTOL
+
+
+def test_accumulates_into_caller_arrays(kernel):
+    inp = _random(3)
+    rng = np.random.default_rng(0)
+    img0 = rng.standard_normal((9, 7, 150))
+    fold0 = rng.integers(0, 9, (9, 7, 150)).astype(np.int32)
+    out = _run_hip(kernel, inp, image0=img0, fold0=fold0)
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(out.fold - fold0, ref.fold3d)
+    assert rel_l2(out.image - img0, ref.image) <= RTOL
+
+
+def test_edge_cases(kernel):
+    # no traces: nothing changes
+    inp = _random(4, ntr=1)
+    tb = create_trace_block(np.zeros((0, 50), np.float32), np.zeros(0), np.zeros(0), np.zeros(0),
+                            np.zeros(0), 2.0)
+    out = create_output_tile(0, 100, 25, 0, 100, 25, 0, 100, 2)
+    kernel.migrate_tile(tb, out, VelocitySlice(np.full(out.nt, 2000.0)), KernelConfig())
+    assert not out.image.any() and not out.fold.any()
+    # traces far from the tile (reference tests/test_kernels.py:132-150)
+    far = _random(5)
+    for name in ("source_x", "receiver_x", "midpoint_x"):
+        setattr(far, name, getattr(far, name) + 50000.0)
+    o = _run_hip(kernel, far)
+    assert o.fold.sum() == 0 and not o.image.any()
+    # two-sample traces, one trace
+    tiny = _random(6, ntr=1, ns=2, nt=10)
+    o = _run_hip(kernel, tiny, fold3d=False)
+    r = orc.migrate(tiny)
+    np.testing.assert_array_equal(o.fold, r.fold2d)
+    assert np.abs(o.image - r.image).max() <= 1e-5 * max(np.abs(r.image).max(), 1e-30)
+
+
+def test_trace_index_and_bins(kernel):
+    """Subset/reordered trace rows and the multi-bin entry point equal one
+    migration over the concatenated rows."""
+    import torch
+
+    inp = _random(8, ntr=500)
+    dt = kernel.upload_traces(create_trace_block(
+        inp.amplitudes, inp.source_x, inp.source_y, inp.receiver_x, inp.receiver_y,
+        inp.dt_in_ms, 0.0, inp.trace_weights))
+    dev = kernel.device
+    perm = np.random.default_rng(1).permutation(500)[:420].astype(np.int32)
+    ax = [torch.as_tensor(np.asarray(a, float), device=dev)
+          for a in (inp.x_axis, inp.y_axis, inp.t_axis_ms, inp.vrms)]
+    cfg = _config_from(inp)
+    img = torch.zeros((9, 7, 150), dtype=torch.float32, device=dev)
+    fold = torch.zeros((9, 7), dtype=torch.int32, device=dev)
+    tidx = torch.as_tensor(perm, device=dev)
+    e = kernel.migrate_device(dt, img, fold, *ax, cfg, trace_index=tidx, count_effective=True)
+    inp.trace_index = perm
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(fold.cpu().numpy(), ref.fold2d)
+    assert rel_l2(img.cpu().numpy(), ref.image) <= RTOL
+    print(f"E gpu {e} oracle {ref.n_effective}")
+    assert abs(e - ref.n_effective) <= 1e-4 * ref.n_effective
+    # bins: groups [0,100) [100,100) [100,420) of the same index list
+    img2 = torch.zeros_like(img)
+    fold2 = torch.zeros_like(fold)
+    kernel.migrate_device(dt, img2, fold2, *ax, cfg, trace_index=tidx,
+                          group_start=np.array([0, 100, 100, 420]))
+    kernel.synchronize()
+    np.testing.assert_array_equal(fold2.cpu().numpy(), ref.fold2d)
+    assert rel_l2(img2.cpu().numpy(), ref.image) <= RTOL
+
+
+def test_point_diffractor_job(kernel):
+    """Full 4096-trace point-diffractor job: 64 x 64 x 500, aperture 1000 m."""
+    from pm_amd.synthetic import point_diffractor_job
+
+    job = point_diffractor_job()
+    inp = orc.MigrateInputs(
+        amplitudes=job.amplitudes.numpy(), source_x=job.source_x, source_y=job.source_y,
+        receiver_x=job.receiver_x, receiver_y=job.receiver_y, midpoint_x=job.midpoint_x,
+        midpoint_y=job.midpoint_y, x_axis=job.x_axis, y_axis=job.y_axis,
+        t_axis_ms=job.t_axis_ms, vrms=job.vrms_1d, dt_in_ms=job.dt_ms,
+        max_aperture_m=1000.0, min_aperture_m=1000.0, max_dip_deg=45.0)
+    out = _run_hip(kernel, inp, fold3d=True)
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(out.fold, ref.fold3d)
+    err = rel_l2(out.image, ref.image)
+    print(f"point diffractor: rel-L2 {err:.3e}, max-abs {np.abs(out.image - ref.image).max():.3e}, "
+          f"C = {int(ref.fold3d.sum())}, E = {ref.n_effective}")
+    assert err <= RTOL
+    ix, iy, k = np.unravel_index(np.argmax(out.image), out.image.shape)
+    assert (ix, iy) == (32, 32) and abs(job.t_axis_ms[k] - 800.0) <= 2.0
+
+
+@pytest.mark.parametrize("vel3d", [False, True])
+def test_triangle_antialiasing(kernel, vel3d):
+    inp = _random(11, vel3d=vel3d, aa_mode=_abi.AA_TRIANGLE, grid_dx=25.0, grid_dy=25.0,
+                  aa_max_half_width=12)
+    out = _run_hip(kernel, inp, fold3d=False)
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(out.fold, ref.fold2d)
+    err = rel_l2(out.image, ref.image)
+    print(f"triangle AA vel3d={vel3d}: rel-L2 {err:.3e}")
+    assert err <= RTOL
+
+
+def test_reflector_job_tile(kernel):
+    """One 32 x 32 tile of a scaled-down reflector job: 3-D velocity from the
+    cube, triangle anti-aliasing, aperture 2500 m, dip 60."""
+    from pm_amd.synthetic import reflector_job
+
+    job = reflector_job(n_traces=3000, n_samples=801, n_out=32, nt=400, d_out=25.0, tile=32,
+                        n_planes=8, n_diffractors=20)
+    v3 = job.cube.resample(job.x_axis, job.y_axis, job.t_axis_ms).numpy()
+    c = job.config
+    inp = orc.MigrateInputs(
+        amplitudes=job.amplitudes.numpy(), source_x=job.source_x, source_y=job.source_y,
+        receiver_x=job.receiver_x, receiver_y=job.receiver_y, midpoint_x=job.midpoint_x,
+        midpoint_y=job.midpoint_y, x_axis=job.x_axis, y_axis=job.y_axis,
+        t_axis_ms=job.t_axis_ms, vrms=v3, dt_in_ms=job.dt_ms, max_aperture_m=c.max_aperture_m,
+        min_aperture_m=c.min_aperture_m, max_dip_deg=c.max_dip_degrees,
+        taper_fraction=c.taper_fraction, aa_mode=_abi.AA_TRIANGLE, grid_dx=c.grid_dx,
+        grid_dy=c.grid_dy, aa_max_half_width=c.aa_max_half_width)
+    out = _run_hip(kernel, inp, fold3d=False)
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(out.fold, ref.fold2d)
+    err = rel_l2(out.image, ref.image)
+    print(f"reflector tile: rel-L2 {err:.3e}, E = {ref.n_effective}")
+    assert err <= RTOL
+
+
+@pytest.mark.parametrize("k", [0.0, 0.6])
+def test_curved_ray(kernel, k):
+    inp = _random(12, traveltime=_abi.TT_CURVED, curved_v0=1600.0, curved_k=k)
+    out = _run_hip(kernel, inp, fold3d=False, method="curved")
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(out.fold, ref.fold2d)
+    err = rel_l2(out.image, ref.image)
+    print(f"curved ray k={k}: rel-L2 {err:.3e}")
+    assert err <= RTOL
+
+
+@pytest.mark.parametrize("eta_kind", ["const", "1d", "3d"])
+def test_vti(kernel, eta_kind):
+    nx, ny, nt = 9, 7, 150
+    eta = None
+    kw = {}
+    if eta_kind == "const":
+        kw["eta_constant"] = 0.12
+    elif eta_kind == "1d":
+        eta = np.linspace(0.0, 0.2, nt)
+    else:
+        eta = 0.05 + 0.1 * np.random.default_rng(3).random((nx, ny, nt))
+    inp = _random(13, traveltime=_abi.TT_VTI, eta=eta, vel3d=(eta_kind == "3d"), **kw)
+    out = _run_hip(kernel, inp, fold3d=False, method="vti")
+    ref = orc.migrate(inp)
+    np.testing.assert_array_equal(out.fold, ref.fold2d)
+    err = rel_l2(out.image, ref.image)
+    print(f"VTI eta={eta_kind}: rel-L2 {err:.3e}")
+    assert err <= RTOL
+
+
+def test_invalid_combination_raises(kernel):
+    inp = _random(14, traveltime=_abi.TT_VTI, aa_mode=_abi.AA_TRIANGLE)
+    with pytest.raises(ValueError, match="straight-ray"):
+        _run_hip(kernel, inp, method="vti")
