@@ -119,6 +119,15 @@ int pstm_hip_release(void);
  * (base.py:314-315). */
 int pstm_hip_stream_sync(void* stream);
 
+/* Launch timing of the gather kernel: while enabled, every gather launch is
+ * bracketed by HIP events recorded on its own stream.  reset(1) clears and
+ * enables, reset(0) clears and disables; read() waits for the recorded
+ * launches and returns their summed duration and count.  Measurement aid
+ * with no counterpart in the reference (its benchmark times whole
+ * migrate_tile calls on the host, pstm/kernels/factory.py:219-309). */
+int pstm_hip_profile_reset(int enable);
+int pstm_hip_profile_read(double* total_ms, long long* launches);
+
 /* Migrate one output tile: fold (f64 aperture test, bit-exact with the
  * oracle), image (f32 gather, no atomics) and optionally the effective
  * contribution count E. Replaces migrate_tile (base.py:307-313) and the call

@@ -90,6 +90,8 @@ struct KParams {
     int rowlen, lo;              // AA row length / index offset
     int batch;                   // traces per LDS batch
     int pair_off;                // byte offset of the pair table in LDS
+    int tot_off;                 // byte offset of the f64 running totals in LDS
+    float taper_off;             // x_t = d * slope + taper_off (taper_off = -(1-tf)/tf)
     int n_items, n_pgroups;      // work items = pillar groups x time windows
     int wpow;                    // 0, 1 or 2: power of 1/t in the weight
     float idt, uoff, umax;       // u = t*idt + uoff, valid u in [0, umax)
@@ -97,6 +99,22 @@ struct KParams {
     float gdx_idt, gdy_idt, lmax;  // AA
     float ck2, ik_ln2;           // curved: k^2, ln2/k
     float v0;                    // curved: surface velocity
+    // f64 inputs for the exact decisions near a threshold (see exact_decision)
+    const double* sx64;
+    const double* sy64;
+    const double* rx64;
+    const double* ry64;
+    const double* mx64;
+    const double* my64;
+    const double* vrms64;        // (npv, nt)
+    const double* tax64;         // (nt) ms
+    const double* eta64;         // VTI: null (constant), (nt) or (npil, nt)
+    int eta_pstride;             // 0 or nt (eta64 per pillar)
+    int tt_req;                  // traveltime asked for (PSTM_TT_*)
+    int exact_ap;                // also re-decide the time-dependent aperture (E counting)
+    int lo_check;                // t_start > 0: u >= 0 can fail
+    float eps_u, eps_x;          // f32 bands around the sample bounds and the aperture edge
+    double tan_dip, min_ap64, max_ap64, t_start_s, dt_s, cv0, ck, eta_const;
 };
 
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -171,17 +189,11 @@ __global__ void k_prep_time_terms(PstmMigrateArgs a, int tt_eff, int npv, double
     double wc = 1.0;
     if (a.apply_spreading) wc *= 1.0 / wv;
     if (a.apply_obliquity) wc *= t_s;
-    const double ts = apv * (1.0 - a.taper_fraction);
-    const double width = apv - ts;
-    double slope, off;
-    if (width > 0.0) {
-        slope = 1.0 / width;
-        off = -ts / width;
-    } else {  // no taper: x_t <= 0 inside the aperture, > 1 outside
-        slope = 1e30;
-        off = -apv * 1e30;
-    }
-    kc[e] = make_float4((float)s, (float)wc, (float)slope, (float)off);
+    // taper coordinate x_t = (d - ap (1 - tf)) / (ap tf) = d / (ap tf) - (1 - tf) / tf:
+    // the slope depends on the aperture, the offset only on tf (a kernel scalar)
+    const double tf = a.taper_fraction;
+    const double slope = tf > 0.0 ? 1.0 / (apv * tf) : 1e30 / apv;
+    kc[e] = make_float4((float)s, (float)wc, (float)slope, 0.0f);
     if (e < nt) {
         qk[k] = (float)q;
         t0k[k] = (float)t_s;
@@ -280,8 +292,160 @@ __global__ __launch_bounds__(256) void k_integrate(PstmMigrateArgs a, int rowlen
 }
 
 // ---------------------------------------------------------------------------
+// exact decisions
+// ---------------------------------------------------------------------------
+// The reference's sample-bounds test (t_sample < 0 or >= n_samples-1 skips the
+// contribution, numba_cpu_optimized.py:199-201) is a discontinuity: an f32
+// position that lands on the other side of it than the f64 one drops or adds
+// a whole term.  When the f32 position is within eps of a threshold the
+// decision is taken again here in f64, with the oracle's formulas and
+// operation order, for the aperture test as well (dist_mid > aperture,
+// :185-187).  Returns whether the contribution counts and its f64 position.
+template <int TT>
+__device__ __forceinline__ bool exact_decision(const KParams& P, int pil, int row, int k, double* u_out) {
+    const int ix = pil / P.ny, iy = pil % P.ny;
+    const double ox = P.xax[ix], oy = P.yax[iy];
+    const double dxm = __dsub_rn(ox, P.mx64[row]), dym = __dsub_rn(oy, P.my64[row]);
+    const double dist_mid = __dsqrt_rn(__dadd_rn(__dmul_rn(dxm, dxm), __dmul_rn(dym, dym)));
+    const double v = P.vrms64[(size_t)pil * P.kc_pstride + k];
+    const double t_s = P.tax64[k] / 1000.0;
+    double ap = v * t_s * 0.5 * P.tan_dip;
+    if (ap < P.min_ap64) ap = P.min_ap64;
+    else if (ap > P.max_ap64) ap = P.max_ap64;
+    const double dxs = ox - P.sx64[row], dys = oy - P.sy64[row];
+    const double dxr = ox - P.rx64[row], dyr = oy - P.ry64[row];
+    const double ds2 = __dadd_rn(__dmul_rn(dxs, dxs), __dmul_rn(dys, dys));
+    const double dr2 = __dadd_rn(__dmul_rn(dxr, dxr), __dmul_rn(dyr, dyr));
+    double t;
+    if (TT == PSTM_TT_STRAIGHT) {
+        // straight ray (numba_cpu_optimized.py:192-194); a curved-ray request
+        // with |k| < 1e-10 is the straight ray in v0 (curved_ray.py:35-38)
+        const double vt = P.tt_req == PSTM_TT_CURVED ? P.cv0 : v;
+        const double sk = 1.0 / (vt * vt);
+        const double q = (t_s * 0.5) * (t_s * 0.5);
+        t = sqrt(q + ds2 * sk) + sqrt(q + dr2 * sk);
+    } else if (TT == PSTM_TT_CURVED) {
+        // curved ray (curved_ray.py:19-52, 239-262)
+        const double v0 = P.cv0, kg = P.ck, tau = t_s / 2.0;
+        const double hs = sqrt(ds2), hr = sqrt(dr2);
+        const double z = (v0 / kg) * (exp(kg * tau) - 1.0);
+        const double vz = v0 + kg * z;
+        const double k1 = kg * hs, k2 = kg * hr;
+        t = (1.0 / kg) * log((vz + sqrt(vz * vz + k1 * k1)) / (v0 + sqrt(v0 * v0 + k1 * k1))) +
+            (1.0 / kg) * log((vz + sqrt(vz * vz + k2 * k2)) / (v0 + sqrt(v0 * v0 + k2 * k2)));
+    } else {
+        // VTI one-way legs (anisotropy_vti.py:181-220)
+        const double eta = P.eta64 ? P.eta64[(size_t)pil * P.eta_pstride + k] : P.eta_const;
+        const double vv = 1.0 / sqrt(1.0 / (v * v));
+        const double tau = t_s / 2.0;
+        t = 0.0;
+        for (int leg = 0; leg < 2; ++leg) {
+            const double h = sqrt(leg == 0 ? ds2 : dr2);
+            double tl;
+            if (vv <= 0 || tau <= 0) {
+                tl = tau;
+            } else {
+                const double h2 = h * h, v2 = vv * vv, t02 = tau * tau;
+                const double t2h = t02 + h2 / v2;
+                const double den = v2 * (t02 * v2 + (1.0 + 2.0 * eta) * h2);
+                tl = fabs(den) < 1e-20 ? sqrt(fmax(t2h, 0.0))
+                                       : sqrt(fmax(t2h - (2.0 * eta * h2 * h2) / den, 0.0));
+            }
+            t += tl;
+        }
+    }
+    const double u = (t - P.t_start_s) / P.dt_s;
+    *u_out = u;
+    return !(dist_mid > ap) && !(u < 0.0 || u >= (double)(P.ns - 1));
+}
+
+// ---------------------------------------------------------------------------
 // the gather
 // ---------------------------------------------------------------------------
+
+// Traveltime of one contribution from the pair terms e/f (LDS) and the
+// (pillar, sample) terms.  y1/y2 return 1/t_src, 1/t_rec for anti-aliasing.
+template <int TT, bool AA>
+__device__ __forceinline__ float contrib_time(const KParams& P, const float4& e, const float4& f, float q, float cs,
+                                              float e2v, float e1v, float t0v, float& y1, float& y2) {
+    float t;
+    y1 = 0.f;
+    y2 = 0.f;
+    if (TT == PSTM_TT_STRAIGHT) {
+        const float a1 = fmaf(e.x, cs, q);
+        const float a2 = fmaf(e.y, cs, q);
+        if (AA) {
+            y1 = a1 > 0.f ? frsq(a1) : 0.f;
+            y2 = a2 > 0.f ? frsq(a2) : 0.f;
+            t = a1 * y1 + a2 * y2;
+        } else {
+            t = fsqrt(a1) + fsqrt(a2);
+        }
+    } else if (TT == PSTM_TT_CURVED) {
+        // t = ln[(vz + sqrt(vz^2 + k^2 hs^2)) (vz + sqrt(vz^2 + k^2 hr^2)) / (den_s den_r)] / k
+        // (curved_ray.py:43-52, both legs in one logarithm)
+        const float ns_ = cs + fsqrt(q + e.x);
+        const float nr_ = cs + fsqrt(q + e.y);
+        t = flog2((ns_ * nr_) * f.x) * P.ik_ln2;
+    } else {
+        // VTI one-way legs (anisotropy_vti.py:181-220) with s = 1/v^2:
+        // t^2 = tau^2 + h^2 s - 2 eta (h^2 s)^2 / (tau^2 + (1 + 2 eta) h^2 s)
+        const float hs = e.x * cs, hr = e.y * cs;
+        const float ts2 = q + hs - e2v * hs * hs * frcp(fmaf(e1v, hs, q));
+        const float tr2 = q + hr - e2v * hr * hr * frcp(fmaf(e1v, hr, q));
+        t = fsqrt(fmaxf(ts2, 0.f)) + fsqrt(fmaxf(tr2, 0.f));
+        if (!(t0v > 0.f)) t = t0v;
+    }
+    return t;
+}
+
+// Trace amplitude at sample position i0 + f0: linear interpolation
+// (interpolation.py:68-92), or, with anti-aliasing, the triangle filter of
+// half width L (from the operator time dip) on the f64 double running sum:
+// a blend of the integer half widths floor(L) and floor(L)+1, each a second
+// difference interpolated between i0 and i0+1.
+template <bool AA, typename Row>
+__device__ __forceinline__ float contrib_amp(const KParams& P, const Row* row, int i0, float f0, const float4& f,
+                                             float cs, float y1, float y2) {
+    if (!AA) {
+        const float s0 = row[i0];
+        const float s1 = row[i0 + 1];
+        return fmaf(f0, s1 - s0, s0);
+    } else {
+        const float gx = cs * fmaf(f.x, y1, f.z * y2);
+        const float gy = cs * fmaf(f.y, y1, f.w * y2);
+        float L = fmaxf(fabsf(gx) * P.gdx_idt, fabsf(gy) * P.gdy_idt);
+        L = fminf(fmaxf(L, 1.0f), P.lmax);
+        const int lf = (int)L;
+        const float lam = L - (float)lf;
+        const Row* r0 = row + P.lo;
+        const double am1 = r0[i0 - lf - 1], am0 = r0[i0 - lf], am1p = r0[i0 - lf + 1];
+        const double c0 = r0[i0], c1 = r0[i0 + 1];
+        const double b0 = r0[i0 + lf], b1 = r0[i0 + lf + 1], b2 = r0[i0 + lf + 2];
+        const float dl0 = (float)((b0 - c0) - (c0 - am0));   // T_lf[i0] * lf^2
+        const float dh0 = (float)((b1 - c0) - (c0 - am1));   // T_lf+1[i0] * (lf+1)^2
+        const float dl1 = (float)((b1 - c1) - (c1 - am1p));  // T_lf[i0+1] * lf^2
+        const float dh1 = (float)((b2 - c1) - (c1 - am0));   // T_lf+1[i0+1] * (lf+1)^2
+        const float wl = (1.0f - lam) * frcp((float)(lf * lf));
+        const float wh = lam * frcp((float)((lf + 1) * (lf + 1)));
+        const float tr0 = fmaf(dl0, wl, dh0 * wh);
+        const float tr1 = fmaf(dl1, wl, dh1 * wh);
+        return fmaf(f0, tr1 - tr0, tr0);
+    }
+}
+
+// Weight: trace weight x cosine taper x spreading 1/(v t) x obliquity t0/t,
+// the last two only for t > 1 ms (numba_cpu_optimized.py:209-226).
+__device__ __forceinline__ float contrib_weight(const KParams& P, float wtr, float xt, float t, float cw) {
+    float w = wtr;
+    if (xt > 0.0f) w *= taper_weight(fminf(xt, 1.0f));
+    if (P.wpow != 0 && t > 0.001f) {
+        const float r = frcp(t);
+        w *= cw * (P.wpow == 2 ? r * r : r);
+    }
+    return w;
+}
+
 template <int MW, int TT, bool AA>
 __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -306,10 +470,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         const int kbase = kwin * win;
 
         // register-resident per-(pillar, sample) terms
-        float q[MW], cs[kPW][MW], cw[kPW][MW], ci[kPW][MW], cn[kPW][MW];
+        float q[MW], cs[kPW][MW], cw[kPW][MW], ci[kPW][MW];
         float e2[kPW][MW], e1[kPW][MW], t0r[MW];
         float acc[kPW][MW];
-        double tot[kPW][MW];
+        double* tot = reinterpret_cast<double*>(smem + P.tot_off);  // [kPW*MW][kThreads], lane-private
 #pragma unroll
         for (int m = 0; m < MW; ++m) {
             const int k = kbase + 64 * (tw + kTW * m) + lane;
@@ -320,13 +484,14 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             for (int p = 0; p < kPW; ++p) {
                 const int pil = pil0 + pg * kPW + p;
                 acc[p][m] = 0.0f;
-                tot[p][m] = 0.0;
+                tot[(p * MW + m) * kThreads + tid] = 0.0;
+                e2[p][m] = 0.0f;
+                e1[p][m] = 1.0f;
                 if (kv && pil < npil) {
                     const float4 c = P.kc[(size_t)pil * P.kc_pstride + k];
                     cs[p][m] = c.x;
                     cw[p][m] = c.y;
                     ci[p][m] = c.z;
-                    cn[p][m] = c.w;
                     if (TT == PSTM_TT_VTI) {
                         const float2 ee = P.ek[(size_t)pil * P.ek_pstride + k];
                         e2[p][m] = ee.x;
@@ -335,43 +500,38 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 } else {
                     cs[p][m] = 0.0f;
                     cw[p][m] = 0.0f;
-                    ci[p][m] = 0.0f;
-                    cn[p][m] = 2.0f;  // x_t = 2 > 1: never in aperture
-                    e2[p][m] = 0.0f;
-                    e1[p][m] = 1.0f;
+                    ci[p][m] = -INFINITY;  // x_t = -inf * d: never in aperture (d >= 0)
                 }
             }
         }
 
-        // tile-relative pillar coordinates for the pair-table threads
-        float pox = 0.0f, poy = 0.0f;
         const int pt_p = tid / P.batch, pt_j = tid % P.batch;
         const bool pt_thread = tid < kNPil * P.batch;
-        if (pt_thread) {
-            const int pil = pil0 + pt_p;
-            if (pil < npil) {
-                pox = (float)(P.xax[pil / P.ny] - P.xax[0]);
-                poy = (float)(P.yax[pil % P.ny] - P.yax[0]);
-            }
-        }
 
         for (int j0 = 0; j0 < P.ntr; j0 += P.batch) {
             const int nb = min(P.batch, P.ntr - j0);
-            // (pillar, trace) pair table
+            // (pillar, trace) pair table; the max-aperture test is the
+            // oracle's f64 one (numba_cpu_optimized.py:167-173)
             int inap = 0;
             if (pt_thread) {
                 const int pil = pil0 + pt_p;
                 float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
                 float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (pt_j < nb && pil < npil) {
-                    const float4 g0 = P.geo[2 * (j0 + pt_j)];
-                    const float4 g1 = P.geo[2 * (j0 + pt_j) + 1];
-                    const float dxs = pox - g0.x, dys = poy - g0.y;
-                    const float dxr = pox - g0.z, dyr = poy - g0.w;
-                    const float dxm = pox - g1.x, dym = poy - g1.y;
-                    const float dm = sqrtf(dxm * dxm + dym * dym);
-                    if (dm <= P.max_ap) {
+                    const double pox64 = P.xax[pil / P.ny], poy64 = P.yax[pil % P.ny];
+                    const float pox = (float)(pox64 - P.xax[0]), poy = (float)(poy64 - P.yax[0]);
+                    const int trow = P.tidx ? P.tidx[j0 + pt_j] : j0 + pt_j;
+                    const double dx64 = __dsub_rn(pox64, P.mx64[trow]);
+                    const double dy64 = __dsub_rn(poy64, P.my64[trow]);
+                    const double dm64 = __dsqrt_rn(__dadd_rn(__dmul_rn(dx64, dx64), __dmul_rn(dy64, dy64)));
+                    if (!(dm64 > P.max_ap64)) {
                         inap = 1;
+                        const float4 g0 = P.geo[2 * (j0 + pt_j)];
+                        const float4 g1 = P.geo[2 * (j0 + pt_j) + 1];
+                        const float dxs = pox - g0.x, dys = poy - g0.y;
+                        const float dxr = pox - g0.z, dyr = poy - g0.w;
+                        const float dxm = pox - g1.x, dym = poy - g1.y;
+                        const float dm = sqrtf(dxm * dxm + dym * dym);
                         const float ds2 = dxs * dxs + dys * dys;
                         const float dr2 = dxr * dxr + dyr * dyr;
                         if (TT == PSTM_TT_CURVED) {
@@ -409,94 +569,80 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
 
             for (int jj = 0; jj < nb; ++jj) {
                 const Row* row = rows + (size_t)jj * P.rowlen;
+                unsigned nearmask = 0;  // contributions within eps of a threshold
 #pragma unroll
                 for (int p = 0; p < kPW; ++p) {
                     const int pi = (pg * kPW + p) * P.batch + jj;
                     const float4 e = pairs[pi * kPairVec];
-                    if (!(e.z <= P.max_ap)) continue;  // pillar-trace pair outside max aperture
+                    if (e.z == INFINITY) continue;  // pair outside max aperture
                     float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
 #pragma unroll
                     for (int m = 0; m < MW; ++m) {
-                        const float xt = fmaf(e.z, ci[p][m], cn[p][m]);
-                        if (!(xt <= 1.0f)) continue;  // time-dependent aperture
-                        float t, y1 = 0.f, y2 = 0.f;
-                        if (TT == PSTM_TT_STRAIGHT) {
-                            const float a1 = fmaf(e.x, cs[p][m], q[m]);
-                            const float a2 = fmaf(e.y, cs[p][m], q[m]);
-                            if (AA) {
-                                y1 = a1 > 0.f ? frsq(a1) : 0.f;
-                                y2 = a2 > 0.f ? frsq(a2) : 0.f;
-                                t = a1 * y1 + a2 * y2;
-                            } else {
-                                t = fsqrt(a1) + fsqrt(a2);
-                            }
-                        } else if (TT == PSTM_TT_CURVED) {
-                            // t = ln[(vz + sqrt(vz^2 + k^2 hs^2)) (vz + sqrt(vz^2 + k^2 hr^2))
-                            //        / (den_s den_r)] / k   (curved_ray.py:43-52, both legs)
-                            const float ns_ = cs[p][m] + fsqrt(q[m] + e.x);
-                            const float nr_ = cs[p][m] + fsqrt(q[m] + e.y);
-                            t = flog2((ns_ * nr_) * f.x) * P.ik_ln2;
-                        } else {
-                            // VTI one-way legs (anisotropy_vti.py:181-220) with
-                            // s = 1/v^2: t^2 = tau^2 + h^2 s - 2 eta (h^2 s)^2 / (tau^2 + (1+2 eta) h^2 s)
-                            const float hs = e.x * cs[p][m], hr = e.y * cs[p][m];
-                            const float ts2 = q[m] + hs - e2[p][m] * hs * hs * frcp(fmaf(e1[p][m], hs, q[m]));
-                            const float tr2 = q[m] + hr - e2[p][m] * hr * hr * frcp(fmaf(e1[p][m], hr, q[m]));
-                            t = fsqrt(fmaxf(ts2, 0.f)) + fsqrt(fmaxf(tr2, 0.f));
-                            if (!(t0r[m] > 0.f)) t = t0r[m];
-                        }
+                        const float xt = fmaf(e.z, ci[p][m], P.taper_off);
+                        if (!(xt <= 1.0f + P.eps_x)) continue;  // time-dependent aperture
+                        float y1, y2;
+                        const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m],
+                                                             y1, y2);
                         const float u = fmaf(t, P.idt, P.uoff);
-                        if (!(u >= 0.0f && u < P.umax)) continue;  // sample bounds
+                        const float du = u - P.umax;
+                        if ((fabsf(du) < P.eps_u) || (P.lo_check && fabsf(u) < P.eps_u) ||
+                            (P.exact_ap && fabsf(xt - 1.0f) < P.eps_x)) {
+                            nearmask |= 1u << (p * MW + m);
+                            continue;
+                        }
+                        if (!(xt <= 1.0f && u >= 0.0f && du < 0.0f)) continue;  // sample bounds
                         const int i0 = (int)u;
                         const float f0 = u - (float)i0;
-                        float amp;
-                        if (!AA) {
-                            const float s0 = row[i0];
-                            const float s1 = row[i0 + 1];
-                            amp = fmaf(f0, s1 - s0, s0);
-                        } else {
-                            // triangle half width from the operator time dip
-                            const float gx = cs[p][m] * fmaf(f.x, y1, f.z * y2);
-                            const float gy = cs[p][m] * fmaf(f.y, y1, f.w * y2);
-                            float L = fmaxf(fabsf(gx) * P.gdx_idt, fabsf(gy) * P.gdy_idt);
-                            L = fminf(fmaxf(L, 1.0f), P.lmax);
-                            // blend of the integer half widths lf and lf+1,
-                            // each a second difference of the f64 double
-                            // running sum, interpolated between i0 and i0+1
-                            const int lf = (int)L;
-                            const float lam = L - (float)lf;
-                            const Row* r0 = row + P.lo;
-                            const double am1 = r0[i0 - lf - 1], am0 = r0[i0 - lf], am1p = r0[i0 - lf + 1];
-                            const double c0 = r0[i0], c1 = r0[i0 + 1];
-                            const double b0 = r0[i0 + lf], b1 = r0[i0 + lf + 1], b2 = r0[i0 + lf + 2];
-                            const float dl0 = (float)((b0 - c0) - (c0 - am0));   // T_lf[i0] * lf^2
-                            const float dh0 = (float)((b1 - c0) - (c0 - am1));   // T_lf+1[i0] * (lf+1)^2
-                            const float dl1 = (float)((b1 - c1) - (c1 - am1p));  // T_lf[i0+1] * lf^2
-                            const float dh1 = (float)((b2 - c1) - (c1 - am0));   // T_lf+1[i0+1] * (lf+1)^2
-                            const float il2 = frcp((float)(lf * lf));
-                            const float ih2 = frcp((float)((lf + 1) * (lf + 1)));
-                            const float wl = (1.0f - lam) * il2, wh = lam * ih2;
-                            const float tr0 = fmaf(dl0, wl, dh0 * wh);
-                            const float tr1 = fmaf(dl1, wl, dh1 * wh);
-                            amp = fmaf(f0, tr1 - tr0, tr0);
-                        }
-                        float w = e.w;
-                        if (xt > 0.0f) w *= taper_weight(xt);
-                        if (P.wpow != 0 && t > 0.001f) {
-                            const float r = frcp(t);
-                            w *= cw[p][m] * (P.wpow == 2 ? r * r : r);
-                        }
+                        const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2);
+                        const float w = contrib_weight(P, e.w, xt, t, cw[p][m]);
                         acc[p][m] = fmaf(amp, w, acc[p][m]);
                         if (P.cnt_part) ++cnt;
                     }
+                }
+                // near a threshold: decide in f64 as the oracle does (rare)
+                while (nearmask) {
+                    const int b = __ffs(nearmask) - 1;
+                    nearmask &= nearmask - 1;
+                    const int p = b / MW, m = b % MW;
+                    const int pil = pil0 + pg * kPW + p;
+                    const int k = kbase + 64 * (tw + kTW * m) + lane;
+                    const int pi = (pg * kPW + p) * P.batch + jj;
+                    const float4 e = pairs[pi * kPairVec];
+                    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
+                    const float4 c = P.kc[(size_t)pil * P.kc_pstride + k];
+                    float e2v = 0.f, e1v = 1.f;
+                    if (TT == PSTM_TT_VTI) {
+                        const float2 ee = P.ek[(size_t)pil * P.ek_pstride + k];
+                        e2v = ee.x;
+                        e1v = ee.y;
+                    }
+                    const int trow = P.tidx ? P.tidx[j0 + jj] : j0 + jj;
+                    double u64;
+                    float val = 0.0f;
+                    if (exact_decision<TT>(P, pil, trow, k, &u64)) {
+                        float y1, y2;
+                        const float t = contrib_time<TT, AA>(P, e, f, P.qk[k], c.x, e2v, e1v, P.t0k[k], y1, y2);
+                        const int i0 = (int)u64;
+                        const float f0 = (float)(u64 - (double)i0);
+                        const float xt = fmaf(e.z, c.z, P.taper_off);
+                        val = contrib_amp<AA>(P, row, i0, f0, f, c.x, y1, y2) *
+                              contrib_weight(P, e.w, xt, t, c.y);
+                        if (P.cnt_part) ++cnt;
+                    }
+#pragma unroll
+                    for (int pp = 0; pp < kPW; ++pp)
+#pragma unroll
+                        for (int mm = 0; mm < MW; ++mm)
+                            if (pp * MW + mm == b) acc[pp][mm] += val;
                 }
             }
 #pragma unroll
             for (int p = 0; p < kPW; ++p)
 #pragma unroll
                 for (int m = 0; m < MW; ++m) {
-                    tot[p][m] += (double)acc[p][m];
+                    tot[(p * MW + m) * kThreads + tid] += (double)acc[p][m];
                     acc[p][m] = 0.0f;
                 }
             __syncthreads();
@@ -512,7 +658,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 const int pil = pil0 + pg * kPW + p;
                 if (pil < npil) {
                     float* dst = P.image + (size_t)pil * P.nt + k;
-                    *dst = *dst + (float)tot[p][m];
+                    *dst = *dst + (float)tot[(p * MW + m) * kThreads + tid];
                 }
             }
         }
@@ -531,6 +677,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         }
     }
 }
+
 
 __global__ void k_sum_counts(const unsigned long long* part, int n, unsigned long long* out) {
     __shared__ unsigned long long red[256];
@@ -556,6 +703,13 @@ struct DeviceState {
 };
 
 DeviceState g_state;
+
+// optional HIP-event timing of the gather launches (pstm_hip_profile_*)
+struct LaunchProfile {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+LaunchProfile g_prof;
 
 int ensure_workspace(size_t bytes) {
     if (bytes <= g_state.ws_size) return 0;
@@ -620,8 +774,18 @@ int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
     auto fn = k_migrate<MW, TT, AA>;
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (g_prof.on) {
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+        HIP_TRY(hipEventRecord(ev0, st));
+    }
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, st, kp);
     HIP_TRY(hipGetLastError());
+    if (g_prof.on) {
+        HIP_TRY(hipEventRecord(ev1, st));
+        g_prof.events.emplace_back(ev0, ev1);
+    }
     return 0;
 }
 
@@ -670,15 +834,19 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     const int pair_vec = (aa || tt == PSTM_TT_CURVED) ? 2 : 1;
 
     // batch size from the LDS budget
-    int batch = kMaxBatch;
-    while (batch > 1 && (size_t)batch * rowlen * elem + (size_t)kNPil * batch * pair_vec * 16 > kLdsBudget)
-        --batch;
-    const size_t lds = align_up((size_t)batch * rowlen * elem, 16) + (size_t)kNPil * batch * pair_vec * 16;
-    if (lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
     // time windows
     const int nwin = (nt + 1023) / 1024;
     const int per_win = (nt + nwin - 1) / nwin;
     const int mw = std::min(4, std::max(1, (per_win + 255) / 256));
+    // batch size from the LDS budget: trace rows + pair table + f64 totals
+    const size_t tot_bytes = (size_t)kPW * mw * kThreads * sizeof(double);
+    auto lds_for = [&](int b) {
+        return align_up(align_up((size_t)b * rowlen * elem, 16) + (size_t)kNPil * b * pair_vec * 16, 16) + tot_bytes;
+    };
+    int batch = kMaxBatch;
+    while (batch > 1 && lds_for(batch) > kLdsBudget) --batch;
+    const size_t lds = lds_for(batch);
+    if (lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
     const int win = 256 * mw;
     const int nwin2 = (nt + win - 1) / win;
     const int n_pgroups = (npil + kNPil - 1) / kNPil;
@@ -752,6 +920,8 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.lo = lo;
     kp.batch = batch;
     kp.pair_off = (int)align_up((size_t)batch * rowlen * elem, 16);
+    kp.tot_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * batch * pair_vec * 16, 16);
+    kp.taper_off = a.taper_fraction > 0.0 ? (float)(-(1.0 - a.taper_fraction) / a.taper_fraction) : -1e30f;
     kp.n_items = (int)n_items;
     kp.n_pgroups = n_pgroups;
     kp.wpow = (a.apply_spreading ? 1 : 0) + (a.apply_obliquity ? 1 : 0);
@@ -766,6 +936,29 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.ck2 = (float)(a.curved_k * a.curved_k);
     kp.ik_ln2 = (float)(tt == PSTM_TT_CURVED ? M_LN2 / a.curved_k : 0.0);
     kp.v0 = (float)a.curved_v0;
+    kp.sx64 = a.source_x;
+    kp.sy64 = a.source_y;
+    kp.rx64 = a.receiver_x;
+    kp.ry64 = a.receiver_y;
+    kp.mx64 = a.midpoint_x;
+    kp.my64 = a.midpoint_y;
+    kp.vrms64 = a.vrms;
+    kp.tax64 = a.t_axis_ms;
+    kp.eta64 = (a.traveltime == PSTM_TT_VTI && a.eta_mode != PSTM_ETA_CONST) ? a.eta : nullptr;
+    kp.eta_pstride = a.eta_mode == PSTM_ETA_3D ? nt : 0;
+    kp.tt_req = a.traveltime;
+    kp.exact_ap = a.n_effective ? 1 : 0;
+    kp.lo_check = a.t_start_in_ms > 0.0 ? 1 : 0;
+    kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
+    kp.eps_x = kp.exact_ap ? 1e-4f : 0.0f;
+    kp.tan_dip = tan_dip;
+    kp.min_ap64 = a.min_aperture_m;
+    kp.max_ap64 = a.max_aperture_m;
+    kp.t_start_s = a.t_start_in_ms / 1000.0;
+    kp.dt_s = a.dt_in_ms / 1000.0;
+    kp.cv0 = a.curved_v0;
+    kp.ck = a.curved_k;
+    kp.eta_const = a.eta_constant;
 
     if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, kp, grid, lds, st);
     else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, kp, grid, lds, st);
@@ -812,6 +1005,33 @@ int pstm_hip_release(void) {
     g_state.ws = nullptr;
     g_state.ws_size = 0;
     g_state.device = -1;
+    return 0;
+}
+
+int pstm_hip_profile_reset(int enable) {
+    g_last_error.clear();
+    for (auto& e : g_prof.events) {
+        HIP_TRY(hipEventSynchronize(e.second));
+        HIP_TRY(hipEventDestroy(e.first));
+        HIP_TRY(hipEventDestroy(e.second));
+    }
+    g_prof.events.clear();
+    g_prof.on = enable != 0;
+    return 0;
+}
+
+int pstm_hip_profile_read(double* total_ms, long long* launches) {
+    g_last_error.clear();
+    if (!total_ms || !launches) return fail(1, "output pointers must be set");
+    double sum = 0.0;
+    for (auto& e : g_prof.events) {
+        HIP_TRY(hipEventSynchronize(e.second));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (long long)g_prof.events.size();
     return 0;
 }
 

@@ -56,6 +56,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_abi_version", "pstm_hip_args_size", "pstm_hip_last_error",
     "pstm_hip_init", "pstm_hip_release", "pstm_hip_stream_sync",
     "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
+    "pstm_hip_profile_reset", "pstm_hip_profile_read",
 )
 
 PACKAGE_DIR = Path(__file__).resolve().parent.parent
@@ -87,6 +88,11 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_hip_fold_tile.argtypes = [P, _vp]
     lib.pstm_hip_migrate_bins.restype = ctypes.c_int
     lib.pstm_hip_migrate_bins.argtypes = [P, _vp, ctypes.c_int32, _vp]
+    lib.pstm_hip_profile_reset.restype = ctypes.c_int
+    lib.pstm_hip_profile_reset.argtypes = [ctypes.c_int]
+    lib.pstm_hip_profile_read.restype = ctypes.c_int
+    lib.pstm_hip_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_longlong)]
     return lib
 
 
