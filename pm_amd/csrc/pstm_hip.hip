@@ -98,6 +98,7 @@ struct KParams {
     float max_ap;
     float gdx_idt, gdy_idt, lmax;  // AA
     float ck2, ik_ln2;           // curved: k^2, ln2/k
+    float t_floor;               // curved: 0 for k > 0, -inf otherwise
     float v0;                    // curved: surface velocity
     // f64 inputs for the exact decisions near a threshold (see exact_decision)
     const double* sx64;
@@ -387,6 +388,9 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         const float ns_ = cs + fsqrt(q + e.x);
         const float nr_ = cs + fsqrt(q + e.y);
         t = flog2((ns_ * nr_) * f.x) * P.ik_ln2;
+        // k > 0: v(z) >= v0, so the exact time is >= 0 (and exactly 0 at t0 = 0,
+        // where the f32 ratio only rounds to 1 +- ulp)
+        t = fmaxf(t, P.t_floor);
     } else {
         // VTI one-way legs (anisotropy_vti.py:181-220) with s = 1/v^2:
         // t^2 = tau^2 + h^2 s - 2 eta (h^2 s)^2 / (tau^2 + (1 + 2 eta) h^2 s)
@@ -936,6 +940,7 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.ck2 = (float)(a.curved_k * a.curved_k);
     kp.ik_ln2 = (float)(tt == PSTM_TT_CURVED ? M_LN2 / a.curved_k : 0.0);
     kp.v0 = (float)a.curved_v0;
+    kp.t_floor = a.curved_k > 0.0 ? 0.0f : -INFINITY;
     kp.sx64 = a.source_x;
     kp.sy64 = a.source_y;
     kp.rx64 = a.receiver_x;
@@ -948,7 +953,9 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.eta_pstride = a.eta_mode == PSTM_ETA_3D ? nt : 0;
     kp.tt_req = a.traveltime;
     kp.exact_ap = a.n_effective ? 1 : 0;
-    kp.lo_check = a.t_start_in_ms > 0.0 ? 1 : 0;
+    // u >= 0 can only be decided differently from f64 when t can be near
+    // t_start: t_start > 0, or a curved ray with k < 0 (t <= 0 at shallow depth)
+    kp.lo_check = (a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0)) ? 1 : 0;
     kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
     kp.eps_x = kp.exact_ap ? 1e-4f : 0.0f;
     kp.tan_dip = tan_dip;
