@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 } else {
                     cs[p][m] = 0.0f;
                     cw[p][m] = 0.0f;
-                    ci[p][m] = -INFINITY;  // x_t = -inf * d: never in aperture (d >= 0)
+                    ci[p][m] = INFINITY;  // x_t = +inf (d > 0) or NaN (d = 0): never in aperture
                 }
             }
         }

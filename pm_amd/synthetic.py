@@ -75,7 +75,9 @@ class SyntheticJob:
 
 def ricker(tau, f_hz):
     a = (np.pi * f_hz * tau) ** 2
-    return (1.0 - 2.0 * a) * np.exp(-a)
+    if isinstance(a, np.ndarray) or np.isscalar(a):
+        return (1.0 - 2.0 * a) * np.exp(-a)
+    return (1.0 - 2.0 * a) * a.neg().exp()
 
 
 def add_events(traces, t_event, amp, f_hz: float, dt_s: float, t_start_s: float = 0.0):
