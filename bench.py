@@ -103,6 +103,10 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="reflectors", choices=["reflectors", "point_diffractor"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="time candidate launch shapes on the busiest tile first")
+    ap.add_argument("--candidates", default="",
+                    help="autotune candidates as MWxBATCH,MWxBATCH,... (default: a 4x3 grid)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0 = skip)")
     args = ap.parse_args()
@@ -131,6 +135,15 @@ def main() -> None:
     ex = TileExecutor(kernel, job)
     torch.cuda.synchronize()
     log(f"executor ready: {len(ex.tiles)} tiles, rows/tile {[int(t.rows.shape[0]) for t in ex.tiles][:4]}...")
+    tune = None
+    if args.autotune:
+        from pm_amd.autotune import autotune
+
+        cands = None
+        if args.candidates:
+            cands = [tuple(int(v) for v in c.split("x")) for c in args.candidates.split(",")]
+        tune = autotune(kernel, ex, candidates=cands)
+        log(f"autotune: mw={tune.mw} batch={tune.batch} ({tune.ms:.1f} ms on the busiest tile)")
 
     # E (counting pass, not timed) and C
     ex.reset()
@@ -188,7 +201,10 @@ def main() -> None:
                    "contributions": c_total, "effective_contributions": e_total,
                    "traveltime": "exact straight-ray DSR", "parallelism": "1 GPU"},
         "roofline": rl, "cpu_baseline": cpu,
+        "launch": _abi.last_launch(lib),
     }
+    if tune is not None:
+        out["autotune"] = tune.as_dict()
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(out), flush=True)

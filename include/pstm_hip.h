@@ -128,6 +128,17 @@ int pstm_hip_stream_sync(void* stream);
 int pstm_hip_profile_reset(int enable);
 int pstm_hip_profile_read(double* total_ms, long long* launches);
 
+/* Launch shape of the gather: mw_cap = sample blocks of 64 per wave
+ * (1..4; the workgroup covers 256*mw samples per time window) and batch_cap =
+ * traces staged in LDS per batch (1..16); 0 restores the built-in choice.
+ * Used by the autotuner; no counterpart in the reference (its tile size
+ * heuristic is host-side, pstm/pipeline/tile_planner.py:26-88). */
+int pstm_hip_set_tuning(int mw_cap, int batch_cap);
+
+/* Shape of the most recent gather launch: mw, traces per batch, workgroups
+ * launched and work items (pillar groups x time windows). */
+int pstm_hip_last_launch(int* mw, int* batch, int* grid, int* items);
+
 /* Migrate one output tile: fold (f64 aperture test, bit-exact with the
  * oracle), image (f32 gather, no atomics) and optionally the effective
  * contribution count E. Replaces migrate_tile (base.py:307-313) and the call

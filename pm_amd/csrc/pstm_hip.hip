@@ -112,8 +112,7 @@ struct KParams {
     const double* eta64;         // VTI: null (constant), (nt) or (npil, nt)
     int eta_pstride;             // 0 or nt (eta64 per pillar)
     int tt_req;                  // traveltime asked for (PSTM_TT_*)
-    int exact_ap;                // also re-decide the time-dependent aperture (E counting)
-    int lo_check;                // t_start > 0: u >= 0 can fail
+    float lo_eps;                // eps_u where u >= 0 can be decided differently from f64, else -inf
     float eps_u, eps_x;          // f32 bands around the sample bounds and the aperture edge
     double tan_dip, min_ap64, max_ap64, t_start_s, dt_s, cv0, ck, eta_const;
 };
@@ -450,7 +449,7 @@ __device__ __forceinline__ float contrib_weight(const KParams& P, float wtr, flo
     return w;
 }
 
-template <int MW, int TT, bool AA>
+template <int MW, int TT, bool AA, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     using Row = typename std::conditional<AA, double, float>::type;
@@ -578,30 +577,64 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 for (int p = 0; p < kPW; ++p) {
                     const int pi = (pg * kPW + p) * P.batch + jj;
                     const float4 e = pairs[pi * kPairVec];
-                    if (e.z == INFINITY) continue;  // pair outside max aperture
+                    if (e.z == INFINITY) continue;  // pair outside max aperture (wave-uniform)
                     float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
+                    // Every slot of the pair is evaluated without per-slot branches so
+                    // that the MW independent chains (sqrt, LDS reads) interleave;
+                    // slots outside the aperture or the trace use a safe sample
+                    // index and add zero.
+                    float xt[MW];
+                    bool any_in = false;
 #pragma unroll
                     for (int m = 0; m < MW; ++m) {
-                        const float xt = fmaf(e.z, ci[p][m], P.taper_off);
-                        if (!(xt <= 1.0f + P.eps_x)) continue;  // time-dependent aperture
-                        float y1, y2;
-                        const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m],
-                                                             y1, y2);
-                        const float u = fmaf(t, P.idt, P.uoff);
-                        const float du = u - P.umax;
-                        if ((fabsf(du) < P.eps_u) || (P.lo_check && fabsf(u) < P.eps_u) ||
-                            (P.exact_ap && fabsf(xt - 1.0f) < P.eps_x)) {
+                        xt[m] = fmaf(e.z, ci[p][m], P.taper_off);
+                        any_in |= xt[m] <= 1.0f + P.eps_x;
+                    }
+                    if (!any_in) continue;  // time-dependent aperture excludes every slot
+                    float t[MW], u[MW], y1[MW], y2[MW];
+                    bool ok[MW];
+#pragma unroll
+                    for (int m = 0; m < MW; ++m) {
+                        t[m] = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m], y1[m],
+                                                    y2[m]);
+                        u[m] = fmaf(t[m], P.idt, P.uoff);
+                        const float du = u[m] - P.umax;
+                        ok[m] = (xt[m] <= 1.0f) && (u[m] >= 0.0f) && (du < 0.0f);
+                        bool near = (fabsf(du) < P.eps_u) || (u[m] < P.lo_eps);
+                        if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
+                        near = near && (xt[m] <= 1.0f + P.eps_x);
+                        if (near) {
                             nearmask |= 1u << (p * MW + m);
-                            continue;
+                            ok[m] = false;
                         }
-                        if (!(xt <= 1.0f && u >= 0.0f && du < 0.0f)) continue;  // sample bounds
-                        const int i0 = (int)u;
-                        const float f0 = u - (float)i0;
-                        const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2);
-                        const float w = contrib_weight(P, e.w, xt, t, cw[p][m]);
-                        acc[p][m] = fmaf(amp, w, acc[p][m]);
-                        if (P.cnt_part) ++cnt;
+                    }
+                    float tw[MW];
+                    bool any_taper = false;
+#pragma unroll
+                    for (int m = 0; m < MW; ++m) {
+                        tw[m] = e.w;
+                        any_taper |= ok[m] && xt[m] > 0.0f;
+                    }
+                    if (any_taper) {
+#pragma unroll
+                        for (int m = 0; m < MW; ++m)
+                            if (xt[m] > 0.0f) tw[m] = e.w * taper_weight(fminf(xt[m], 1.0f));
+                    }
+#pragma unroll
+                    for (int m = 0; m < MW; ++m) {
+                        const float us = ok[m] ? u[m] : 0.0f;
+                        const int i0 = (int)us;
+                        const float f0 = us - (float)i0;
+                        const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m]);
+                        float w = tw[m];
+                        if (P.wpow != 0) {
+                            const float r = frcp(t[m]);
+                            const float wf = cw[p][m] * (P.wpow == 2 ? r * r : r);
+                            w = t[m] > 0.001f ? w * wf : w;
+                        }
+                        acc[p][m] = fmaf(ok[m] ? amp : 0.0f, w, acc[p][m]);
+                        if (COUNT) cnt += ok[m] ? 1 : 0;
                     }
                 }
                 // near a threshold: decide in f64 as the oracle does (rare)
@@ -633,7 +666,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         const float xt = fmaf(e.z, c.z, P.taper_off);
                         val = contrib_amp<AA>(P, row, i0, f0, f, c.x, y1, y2) *
                               contrib_weight(P, e.w, xt, t, c.y);
-                        if (P.cnt_part) ++cnt;
+                        if (COUNT) ++cnt;
                     }
 #pragma unroll
                     for (int pp = 0; pp < kPW; ++pp)
@@ -669,7 +702,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         __syncthreads();
     }
 
-    if (P.cnt_part) {
+    if (COUNT) {
         __shared__ unsigned long long red[kWaves];
         for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
         if (lane == 0) red[wave] = cnt;
@@ -707,6 +740,25 @@ struct DeviceState {
 };
 
 DeviceState g_state;
+
+// launch shape: time blocks per wave (MW) and traces per LDS batch; 0 = the
+// built-in choice.  Set by pstm_hip_set_tuning (the autotuner) or, for
+// experiments, the PSTM_HIP_MW / PSTM_HIP_BATCH environment variables.
+struct Tuning {
+    int mw_cap = 0;
+    int batch_cap = 0;
+};
+Tuning g_tune;
+
+struct LastLaunch {
+    int mw = 0, batch = 0, grid = 0, items = 0;
+};
+LastLaunch g_last;
+
+int env_int(const char* name) {
+    const char* v = getenv(name);
+    return v ? atoi(v) : 0;
+}
 
 // optional HIP-event timing of the gather launches (pstm_hip_profile_*)
 struct LaunchProfile {
@@ -773,9 +825,9 @@ int validate(const PstmMigrateArgs* a) {
     return 0;
 }
 
-template <int MW, int TT, bool AA>
+template <int MW, int TT, bool AA, bool COUNT>
 int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
-    auto fn = k_migrate<MW, TT, AA>;
+    auto fn = k_migrate<MW, TT, AA, COUNT>;
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -794,12 +846,20 @@ int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
 }
 
 template <int TT, bool AA>
-int dispatch_mw(int mw, const KParams& kp, int grid, size_t lds, hipStream_t st) {
+int dispatch_mw(int mw, bool count, const KParams& kp, int grid, size_t lds, hipStream_t st) {
+    if (count) {
+        switch (mw) {
+            case 1: return launch_migrate<1, TT, AA, true>(kp, grid, lds, st);
+            case 2: return launch_migrate<2, TT, AA, true>(kp, grid, lds, st);
+            case 3: return launch_migrate<3, TT, AA, true>(kp, grid, lds, st);
+            default: return launch_migrate<4, TT, AA, true>(kp, grid, lds, st);
+        }
+    }
     switch (mw) {
-        case 1: return launch_migrate<1, TT, AA>(kp, grid, lds, st);
-        case 2: return launch_migrate<2, TT, AA>(kp, grid, lds, st);
-        case 3: return launch_migrate<3, TT, AA>(kp, grid, lds, st);
-        default: return launch_migrate<4, TT, AA>(kp, grid, lds, st);
+        case 1: return launch_migrate<1, TT, AA, false>(kp, grid, lds, st);
+        case 2: return launch_migrate<2, TT, AA, false>(kp, grid, lds, st);
+        case 3: return launch_migrate<3, TT, AA, false>(kp, grid, lds, st);
+        default: return launch_migrate<4, TT, AA, false>(kp, grid, lds, st);
     }
 }
 
@@ -841,13 +901,17 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     // time windows
     const int nwin = (nt + 1023) / 1024;
     const int per_win = (nt + nwin - 1) / nwin;
-    const int mw = std::min(4, std::max(1, (per_win + 255) / 256));
+    int mw_cap = g_tune.mw_cap > 0 ? g_tune.mw_cap : env_int("PSTM_HIP_MW");
+    if (mw_cap <= 0) mw_cap = 4;
+    mw_cap = std::min(4, mw_cap);
+    int mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
     // batch size from the LDS budget: trace rows + pair table + f64 totals
     const size_t tot_bytes = (size_t)kPW * mw * kThreads * sizeof(double);
     auto lds_for = [&](int b) {
         return align_up(align_up((size_t)b * rowlen * elem, 16) + (size_t)kNPil * b * pair_vec * 16, 16) + tot_bytes;
     };
-    int batch = kMaxBatch;
+    int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
+    int batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
     while (batch > 1 && lds_for(batch) > kLdsBudget) --batch;
     const size_t lds = lds_for(batch);
     if (lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
@@ -858,6 +922,11 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     if (n_items > (1L << 30)) return fail(1, "tile too large");
     const int occupancy = lds <= kLdsBudget / 2 ? 2 : 1;  // workgroups per CU the LDS allows
     const int grid = (int)std::min<long>(n_items, (long)g_state.n_cu * occupancy);
+
+    g_last.mw = mw;
+    g_last.batch = batch;
+    g_last.grid = grid;
+    g_last.items = (int)n_items;
 
     // workspace layout
     size_t off = 0;
@@ -952,12 +1021,12 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.eta64 = (a.traveltime == PSTM_TT_VTI && a.eta_mode != PSTM_ETA_CONST) ? a.eta : nullptr;
     kp.eta_pstride = a.eta_mode == PSTM_ETA_3D ? nt : 0;
     kp.tt_req = a.traveltime;
-    kp.exact_ap = a.n_effective ? 1 : 0;
     // u >= 0 can only be decided differently from f64 when t can be near
     // t_start: t_start > 0, or a curved ray with k < 0 (t <= 0 at shallow depth)
-    kp.lo_check = (a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0)) ? 1 : 0;
+    const bool lo_check = a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0);
     kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
-    kp.eps_x = kp.exact_ap ? 1e-4f : 0.0f;
+    kp.lo_eps = lo_check ? kp.eps_u : -INFINITY;
+    kp.eps_x = a.n_effective ? 1e-4f : 0.0f;
     kp.tan_dip = tan_dip;
     kp.min_ap64 = a.min_aperture_m;
     kp.max_ap64 = a.max_aperture_m;
@@ -967,10 +1036,11 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.ck = a.curved_k;
     kp.eta_const = a.eta_constant;
 
-    if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, kp, grid, lds, st);
-    else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, kp, grid, lds, st);
-    else if (tt == PSTM_TT_CURVED) rc = dispatch_mw<PSTM_TT_CURVED, false>(mw, kp, grid, lds, st);
-    else rc = dispatch_mw<PSTM_TT_VTI, false>(mw, kp, grid, lds, st);
+    const bool count = a.n_effective != nullptr;
+    if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, count, kp, grid, lds, st);
+    else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, count, kp, grid, lds, st);
+    else if (tt == PSTM_TT_CURVED) rc = dispatch_mw<PSTM_TT_CURVED, false>(mw, count, kp, grid, lds, st);
+    else rc = dispatch_mw<PSTM_TT_VTI, false>(mw, count, kp, grid, lds, st);
     if (rc) return rc;
     if (a.n_effective) {
         hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(256), 0, st, (const unsigned long long*)cnt, grid,
@@ -1039,6 +1109,24 @@ int pstm_hip_profile_read(double* total_ms, long long* launches) {
     }
     *total_ms = sum;
     *launches = (long long)g_prof.events.size();
+    return 0;
+}
+
+int pstm_hip_set_tuning(int mw_cap, int batch_cap) {
+    g_last_error.clear();
+    if (mw_cap < 0 || mw_cap > 4 || batch_cap < 0 || batch_cap > kMaxBatch)
+        return fail(1, "pstm_hip_set_tuning: mw_cap in [0, 4], batch_cap in [0, 16] (0 = default)");
+    g_tune.mw_cap = mw_cap;
+    g_tune.batch_cap = batch_cap;
+    return 0;
+}
+
+int pstm_hip_last_launch(int* mw, int* batch, int* grid, int* items) {
+    if (!mw || !batch || !grid || !items) return fail(1, "output pointers must be set");
+    *mw = g_last.mw;
+    *batch = g_last.batch;
+    *grid = g_last.grid;
+    *items = g_last.items;
     return 0;
 }
 

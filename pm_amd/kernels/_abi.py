@@ -57,6 +57,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_init", "pstm_hip_release", "pstm_hip_stream_sync",
     "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
+    "pstm_hip_set_tuning", "pstm_hip_last_launch",
 )
 
 PACKAGE_DIR = Path(__file__).resolve().parent.parent
@@ -93,7 +94,18 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_hip_profile_read.restype = ctypes.c_int
     lib.pstm_hip_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_longlong)]
+    lib.pstm_hip_set_tuning.restype = ctypes.c_int
+    lib.pstm_hip_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.pstm_hip_last_launch.restype = ctypes.c_int
+    lib.pstm_hip_last_launch.argtypes = [ip, ip, ip, ip]
     return lib
+
+
+def last_launch(lib: ctypes.CDLL) -> dict:
+    vals = [ctypes.c_int() for _ in range(4)]
+    lib.pstm_hip_last_launch(*[ctypes.byref(v) for v in vals])
+    return dict(zip(("mw", "batch", "grid", "items"), (v.value for v in vals)))
 
 
 def check_layout(lib: ctypes.CDLL) -> None:
