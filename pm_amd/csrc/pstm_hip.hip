@@ -95,6 +95,7 @@ struct KParams {
     int n_items, n_pgroups;      // work items = pillar groups x time windows
     int wpow;                    // 0, 1 or 2: power of 1/t in the weight
     float idt, uoff, umax;       // u = t*idt + uoff, valid u in [0, umax)
+    float umax_lo, umax_hi;      // umax -+ eps_u: band decided in f64
     float max_ap;
     float gdx_idt, gdy_idt, lmax;  // AA
     float ck2, ik_ln2;           // curved: k^2, ln2/k
@@ -449,7 +450,7 @@ __device__ __forceinline__ float contrib_weight(const KParams& P, float wtr, flo
     return w;
 }
 
-template <int MW, int TT, bool AA, bool COUNT>
+template <int MW, int TT, bool AA, bool COUNT, bool LOB>
 __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     using Row = typename std::conditional<AA, double, float>::type;
@@ -507,6 +508,17 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 }
             }
         }
+
+        // sample blocks of this wave whose first t0 is <= 2 ms: only there can
+        // t <= 1 ms switch the spreading/obliquity weights off (wave-uniform)
+        unsigned early = 0;
+#pragma unroll
+        for (int m = 0; m < MW; ++m) {
+            const int kb = kbase + 64 * (tw + kTW * m);
+            if (kb < P.nt && P.t0k[kb] <= 0.002f) early |= 1u << m;
+        }
+        if (TT == PSTM_TT_CURVED) early = (1u << MW) - 1u;  // t may be < t0 on a curved ray
+        early = __builtin_amdgcn_readfirstlane(early);
 
         const int pt_p = tid / P.batch, pt_j = tid % P.batch;
         const bool pt_thread = tid < kNPil * P.batch;
@@ -572,7 +584,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
 
             for (int jj = 0; jj < nb; ++jj) {
                 const Row* row = rows + (size_t)jj * P.rowlen;
-                unsigned nearmask = 0;  // contributions within eps of a threshold
+                bool any_near = false;  // some slot within eps of a threshold (rare)
 #pragma unroll
                 for (int p = 0; p < kPW; ++p) {
                     const int pi = (pg * kPW + p) * P.batch + jj;
@@ -582,8 +594,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
                     // Every slot of the pair is evaluated without per-slot branches so
                     // that the MW independent chains (sqrt, LDS reads) interleave;
-                    // slots outside the aperture or the trace use a safe sample
-                    // index and add zero.
+                    // slots outside the aperture or the trace read a safe sample and
+                    // add zero.
                     float xt[MW];
                     bool any_in = false;
 #pragma unroll
@@ -599,15 +611,15 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         t[m] = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m], y1[m],
                                                     y2[m]);
                         u[m] = fmaf(t[m], P.idt, P.uoff);
-                        const float du = u[m] - P.umax;
-                        ok[m] = (xt[m] <= 1.0f) && (u[m] >= 0.0f) && (du < 0.0f);
-                        bool near = (fabsf(du) < P.eps_u) || (u[m] < P.lo_eps);
+                        // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
+                        const bool in_ap = xt[m] <= (COUNT ? 1.0f - P.eps_x : 1.0f);
+                        const bool below = u[m] < P.umax_lo;
+                        bool near = in_ap && !below && u[m] < P.umax_hi;
+                        if (LOB) near = near || (in_ap && u[m] < P.lo_eps && u[m] > -P.lo_eps);
                         if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
-                        near = near && (xt[m] <= 1.0f + P.eps_x);
-                        if (near) {
-                            nearmask |= 1u << (p * MW + m);
-                            ok[m] = false;
-                        }
+                        any_near |= near;
+                        ok[m] = in_ap && below && !near;
+                        if (LOB) ok[m] = ok[m] && u[m] >= 0.0f;
                     }
                     float tw[MW];
                     bool any_taper = false;
@@ -625,54 +637,55 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     for (int m = 0; m < MW; ++m) {
                         const float us = ok[m] ? u[m] : 0.0f;
                         const int i0 = (int)us;
-                        const float f0 = us - (float)i0;
+                        const float f0 = __builtin_amdgcn_fractf(us);
                         const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m]);
-                        float w = tw[m];
+                        float w = ok[m] ? tw[m] : 0.0f;
                         if (P.wpow != 0) {
                             const float r = frcp(t[m]);
                             const float wf = cw[p][m] * (P.wpow == 2 ? r * r : r);
-                            w = t[m] > 0.001f ? w * wf : w;
+                            // t > 1 ms is certain unless the block holds t0 <= 2 ms (early)
+                            if (early & (1u << m)) w = t[m] > 0.001f ? w * wf : w;
+                            else w *= wf;
                         }
-                        acc[p][m] = fmaf(ok[m] ? amp : 0.0f, w, acc[p][m]);
+                        acc[p][m] = fmaf(amp, w, acc[p][m]);
                         if (COUNT) cnt += ok[m] ? 1 : 0;
                     }
                 }
-                // near a threshold: decide in f64 as the oracle does (rare)
-                while (nearmask) {
-                    const int b = __ffs(nearmask) - 1;
-                    nearmask &= nearmask - 1;
-                    const int p = b / MW, m = b % MW;
-                    const int pil = pil0 + pg * kPW + p;
-                    const int k = kbase + 64 * (tw + kTW * m) + lane;
-                    const int pi = (pg * kPW + p) * P.batch + jj;
-                    const float4 e = pairs[pi * kPairVec];
-                    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
-                    const float4 c = P.kc[(size_t)pil * P.kc_pstride + k];
-                    float e2v = 0.f, e1v = 1.f;
-                    if (TT == PSTM_TT_VTI) {
-                        const float2 ee = P.ek[(size_t)pil * P.ek_pstride + k];
-                        e2v = ee.x;
-                        e1v = ee.y;
-                    }
-                    const int trow = P.tidx ? P.tidx[j0 + jj] : j0 + jj;
-                    double u64;
-                    float val = 0.0f;
-                    if (exact_decision<TT>(P, pil, trow, k, &u64)) {
-                        float y1, y2;
-                        const float t = contrib_time<TT, AA>(P, e, f, P.qk[k], c.x, e2v, e1v, P.t0k[k], y1, y2);
-                        const int i0 = (int)u64;
-                        const float f0 = (float)(u64 - (double)i0);
-                        const float xt = fmaf(e.z, c.z, P.taper_off);
-                        val = contrib_amp<AA>(P, row, i0, f0, f, c.x, y1, y2) *
-                              contrib_weight(P, e.w, xt, t, c.y);
-                        if (COUNT) ++cnt;
-                    }
+                // slots near a threshold: recompute which, decide in f64 as the
+                // oracle does (rare; the slots were left out above)
+                if (any_near) {
 #pragma unroll
-                    for (int pp = 0; pp < kPW; ++pp)
+                    for (int p = 0; p < kPW; ++p) {
+                        const int pi = (pg * kPW + p) * P.batch + jj;
+                        const float4 e = pairs[pi * kPairVec];
+                        if (e.z == INFINITY) continue;
+                        float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
 #pragma unroll
-                        for (int mm = 0; mm < MW; ++mm)
-                            if (pp * MW + mm == b) acc[pp][mm] += val;
+                        for (int m = 0; m < MW; ++m) {
+                            const float xt = fmaf(e.z, ci[p][m], P.taper_off);
+                            float y1, y2;
+                            const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m],
+                                                                 t0r[m], y1, y2);
+                            const float u = fmaf(t, P.idt, P.uoff);
+                            const bool in_ap = xt <= (COUNT ? 1.0f - P.eps_x : 1.0f);
+                            bool near = in_ap && u >= P.umax_lo && u < P.umax_hi;
+                            if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
+                            if (COUNT) near = near || (fabsf(xt - 1.0f) < P.eps_x);
+                            if (!near) continue;
+                            const int pil = pil0 + pg * kPW + p;
+                            const int k = kbase + 64 * (tw + kTW * m) + lane;
+                            const int trow = P.tidx ? P.tidx[j0 + jj] : j0 + jj;
+                            double u64;
+                            if (exact_decision<TT>(P, pil, trow, k, &u64)) {
+                                const int i0 = (int)u64;
+                                const float f0 = (float)(u64 - (double)i0);
+                                acc[p][m] = fmaf(contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2),
+                                                 contrib_weight(P, e.w, xt, t, cw[p][m]), acc[p][m]);
+                                if (COUNT) ++cnt;
+                            }
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -825,9 +838,9 @@ int validate(const PstmMigrateArgs* a) {
     return 0;
 }
 
-template <int MW, int TT, bool AA, bool COUNT>
+template <int MW, int TT, bool AA, bool COUNT, bool LOB>
 int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
-    auto fn = k_migrate<MW, TT, AA, COUNT>;
+    auto fn = k_migrate<MW, TT, AA, COUNT, LOB>;
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -845,22 +858,24 @@ int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
     return 0;
 }
 
-template <int TT, bool AA>
-int dispatch_mw(int mw, bool count, const KParams& kp, int grid, size_t lds, hipStream_t st) {
-    if (count) {
-        switch (mw) {
-            case 1: return launch_migrate<1, TT, AA, true>(kp, grid, lds, st);
-            case 2: return launch_migrate<2, TT, AA, true>(kp, grid, lds, st);
-            case 3: return launch_migrate<3, TT, AA, true>(kp, grid, lds, st);
-            default: return launch_migrate<4, TT, AA, true>(kp, grid, lds, st);
-        }
-    }
+template <int TT, bool AA, bool COUNT, bool LOB>
+int dispatch_mw3(int mw, const KParams& kp, int grid, size_t lds, hipStream_t st) {
     switch (mw) {
-        case 1: return launch_migrate<1, TT, AA, false>(kp, grid, lds, st);
-        case 2: return launch_migrate<2, TT, AA, false>(kp, grid, lds, st);
-        case 3: return launch_migrate<3, TT, AA, false>(kp, grid, lds, st);
-        default: return launch_migrate<4, TT, AA, false>(kp, grid, lds, st);
+        case 1: return launch_migrate<1, TT, AA, COUNT, LOB>(kp, grid, lds, st);
+        case 2: return launch_migrate<2, TT, AA, COUNT, LOB>(kp, grid, lds, st);
+        case 3: return launch_migrate<3, TT, AA, COUNT, LOB>(kp, grid, lds, st);
+        default: return launch_migrate<4, TT, AA, COUNT, LOB>(kp, grid, lds, st);
     }
+}
+
+template <int TT, bool AA>
+int dispatch_mw(int mw, bool count, bool lob, const KParams& kp, int grid, size_t lds, hipStream_t st) {
+    if (count) {
+        return lob ? dispatch_mw3<TT, AA, true, true>(mw, kp, grid, lds, st)
+                   : dispatch_mw3<TT, AA, true, false>(mw, kp, grid, lds, st);
+    }
+    return lob ? dispatch_mw3<TT, AA, false, true>(mw, kp, grid, lds, st)
+               : dispatch_mw3<TT, AA, false, false>(mw, kp, grid, lds, st);
 }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -1021,11 +1036,15 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.eta64 = (a.traveltime == PSTM_TT_VTI && a.eta_mode != PSTM_ETA_CONST) ? a.eta : nullptr;
     kp.eta_pstride = a.eta_mode == PSTM_ETA_3D ? nt : 0;
     kp.tt_req = a.traveltime;
-    // u >= 0 can only be decided differently from f64 when t can be near
-    // t_start: t_start > 0, or a curved ray with k < 0 (t <= 0 at shallow depth)
-    const bool lo_check = a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0);
+    // u >= 0 can only fail or be decided differently from f64 when t can be
+    // at or below t_start: t_start > 0, a curved ray with k < 0, or VTI
+    // VTI returns t = t0 for t0 <= 0 (negative output times give u < 0)
+    const bool lo_check = a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0) ||
+                          tt == PSTM_TT_VTI;
     kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
     kp.lo_eps = lo_check ? kp.eps_u : -INFINITY;
+    kp.umax_lo = kp.umax - kp.eps_u;
+    kp.umax_hi = kp.umax + kp.eps_u;
     kp.eps_x = a.n_effective ? 1e-4f : 0.0f;
     kp.tan_dip = tan_dip;
     kp.min_ap64 = a.min_aperture_m;
@@ -1037,10 +1056,10 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.eta_const = a.eta_constant;
 
     const bool count = a.n_effective != nullptr;
-    if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, count, kp, grid, lds, st);
-    else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, count, kp, grid, lds, st);
-    else if (tt == PSTM_TT_CURVED) rc = dispatch_mw<PSTM_TT_CURVED, false>(mw, count, kp, grid, lds, st);
-    else rc = dispatch_mw<PSTM_TT_VTI, false>(mw, count, kp, grid, lds, st);
+    if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, count, lo_check, kp, grid, lds, st);
+    else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, count, lo_check, kp, grid, lds, st);
+    else if (tt == PSTM_TT_CURVED) rc = dispatch_mw<PSTM_TT_CURVED, false>(mw, count, lo_check, kp, grid, lds, st);
+    else rc = dispatch_mw<PSTM_TT_VTI, false>(mw, count, lo_check, kp, grid, lds, st);
     if (rc) return rc;
     if (a.n_effective) {
         hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(256), 0, st, (const unsigned long long*)cnt, grid,
