@@ -107,6 +107,9 @@ def main() -> None:
                     help="time candidate launch shapes on the busiest tile first")
     ap.add_argument("--candidates", default="",
                     help="autotune candidates as MWxBATCH,MWxBATCH,... (default: a 4x3 grid)")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: one untimed copy of 256 MiB (traffic calibration) "
+                         "and the timed steps only, no counting pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0 = skip)")
     args = ap.parse_args()
@@ -145,13 +148,25 @@ def main() -> None:
         tune = autotune(kernel, ex, candidates=cands)
         log(f"autotune: mw={tune.mw} batch={tune.batch} ({tune.ms:.1f} ms on the busiest tile)")
 
-    # E (counting pass, not timed) and C
-    ex.reset()
-    e_total = ex.run(count_effective=True)
-    c_total = ex.contributions()
-    img, _ = ex.assemble()
-    assert bool(torch.isfinite(img).all()) and float(img.abs().max()) > 0, "empty or non-finite image"
-    log(f"C = {c_total:.4e} contributions, E = {e_total:.4e}")
+    if args.profile:
+        # plain copy of known size: reference point for the HBM byte counters
+        a = torch.empty(64 * 2**20, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
+        del a, b
+        e_total = 0
+        c_total = 0
+    else:
+        # E (counting pass, not timed) and C
+        ex.reset()
+        e_total = ex.run(count_effective=True)
+        c_total = ex.contributions()
+        img, _ = ex.assemble()
+        assert bool(torch.isfinite(img).all()) and float(img.abs().max()) > 0, "empty or non-finite image"
+        log(f"C = {c_total:.4e} contributions, E = {e_total:.4e}")
 
     for _ in range(args.warmup):
         ex.reset()
@@ -173,6 +188,10 @@ def main() -> None:
     lib.pstm_hip_profile_reset(0)
     ms_step = elapsed * 1000.0 / args.steps
     value = c_total / (elapsed / args.steps) / 1e9
+    if args.profile:
+        log(f"profile run: {ms_step:.1f} ms per step")
+        kernel.cleanup()
+        return
 
     cfg = job.config
     f = roofline.flops_per_contribution(cfg.kernel_type, cfg.antialias,
