@@ -157,6 +157,44 @@ int pstm_hip_fold_tile(const PstmMigrateArgs* args, void* stream);
 int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_start,
                           int32_t n_groups, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Traveltime surrogate (no counterpart in the reference: the learned part of
+ * this backend).  A 5-64-64-64-64-2 SiLU MLP; inputs are the normalised
+ * (tau, d_s, d_r, v, eta), outputs the anisotropic corrections rho_s, rho_r
+ * with t_leg = sqrt(tau^2 + d^2/v^2) (1 + rho).  Parameters: one f32 buffer of
+ * PSTM_MLP_PARAMS values, per layer W (out x in, row-major) then b (out).
+ * All pointers are device pointers; calls are asynchronous on `stream`.
+ * ------------------------------------------------------------------------- */
+#define PSTM_MLP_IN 5
+#define PSTM_MLP_HIDDEN 64
+#define PSTM_MLP_OUT 2
+#define PSTM_MLP_PARAMS 12994
+
+int pstm_mlp_param_count(void);
+const char* pstm_mlp_last_error(void);
+
+/* y (n, 2) = network(x (n, 5)), f32. */
+int pstm_mlp_forward(const float* params, const float* x, int n, float* y, void* stream);
+
+/* Mean squared error of network(x) against target (n, 2) into *loss (device
+ * f32) and its gradient w.r.t. every parameter into grads. */
+int pstm_mlp_loss_grad(const float* params, const float* x, const float* target, int n, float* grads,
+                       float* loss, void* stream);
+
+/* One Adam step (bias-corrected, step >= 1) on params with moments m, v. */
+int pstm_mlp_adam(float* params, const float* grads, float* m, float* v, float lr, float beta1, float beta2,
+                  float eps, int step, void* stream);
+
+/* loss_grad followed by adam. */
+int pstm_mlp_train_step(float* params, float* m, float* v, float* grads, const float* x, const float* target,
+                        int n, float lr, float beta1, float beta2, float eps, int step, float* loss,
+                        void* stream);
+
+/* Exact one-way VTI times in f64 (pstm/algorithm/anisotropy_vti.py:181-220):
+ * t_out[i] = t(tau[i], h[i], v[i], eta[i]); the surrogate's training targets. */
+int pstm_vti_leg_times(const double* tau, const double* h, const double* v, const double* eta, long long n,
+                       double* t_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

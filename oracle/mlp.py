@@ -1,0 +1,72 @@
+"""TEST INFRASTRUCTURE: FP64 NumPy restatement of the traveltime surrogate's
+network (5-64-64-64-64-2, SiLU), its mean-squared-error gradient and one Adam
+step, in the parameter layout of include/pstm_hip.h (per layer W (out x in)
+row-major, then b).  Used to check the HIP training kernels.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+SIZES = [(5, 64), (64, 64), (64, 64), (64, 64), (64, 2)]
+
+
+def unpack(p):
+    layers, off = [], 0
+    for fin, fout in SIZES:
+        W = p[off:off + fin * fout].reshape(fout, fin)
+        off += fin * fout
+        b = p[off:off + fout]
+        off += fout
+        layers.append((W, b))
+    assert off == p.size
+    return layers
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def forward(p, x):
+    a = np.asarray(x, np.float64)
+    ls = unpack(np.asarray(p, np.float64))
+    for i, (W, b) in enumerate(ls):
+        z = a @ W.T + b
+        a = silu(z) if i < len(ls) - 1 else z
+    return a
+
+
+def loss_grad(p, x, t):
+    """(mean squared error over all outputs, gradient in the packed layout)."""
+    p = np.asarray(p, np.float64)
+    x = np.asarray(x, np.float64)
+    ls = unpack(p)
+    acts, zs = [x], []
+    a = x
+    for i, (W, b) in enumerate(ls):
+        z = a @ W.T + b
+        zs.append(z)
+        a = silu(z) if i < len(ls) - 1 else z
+        acts.append(a)
+    y = acts[-1]
+    n = y.size
+    loss = float(np.sum((y - t) ** 2) / n)
+    d = 2.0 * (y - t) / n
+    grads = []
+    for i in range(len(ls) - 1, -1, -1):
+        W, b = ls[i]
+        if i < len(ls) - 1:
+            s = 1.0 / (1.0 + np.exp(-zs[i]))
+            d = d * s * (1.0 + zs[i] * (1.0 - s))
+        grads.append((d.T @ acts[i], d.sum(0)))
+        d = d @ W
+    g = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in reversed(grads)])
+    return loss, g
+
+
+def adam(p, g, m, v, lr, b1, b2, eps, step):
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** step)
+    vh = v / (1 - b2 ** step)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

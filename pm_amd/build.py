@@ -15,7 +15,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 REPO = PKG.parent
-SOURCES = [PKG / "csrc" / "pstm_hip.hip"]
+SOURCES = [PKG / "csrc" / "pstm_hip.hip", PKG / "csrc" / "pstm_mlp.hip"]
 OUT = PKG / "lib" / "libpstm_hip.so"
 
 HIPCC_FLAGS = [

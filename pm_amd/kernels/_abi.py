@@ -58,7 +58,11 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
     "pstm_hip_set_tuning", "pstm_hip_last_launch",
+    "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_forward", "pstm_mlp_loss_grad",
+    "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_vti_leg_times",
 )
+
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 5, 64, 2, 12994
 
 PACKAGE_DIR = Path(__file__).resolve().parent.parent
 LIB_PATH = PACKAGE_DIR / "lib" / "libpstm_hip.so"
@@ -99,6 +103,22 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     ip = ctypes.POINTER(ctypes.c_int)
     lib.pstm_hip_last_launch.restype = ctypes.c_int
     lib.pstm_hip_last_launch.argtypes = [ip, ip, ip, ip]
+    f = ctypes.c_float
+    lib.pstm_mlp_param_count.restype = ctypes.c_int
+    lib.pstm_mlp_param_count.argtypes = []
+    lib.pstm_mlp_last_error.restype = ctypes.c_char_p
+    lib.pstm_mlp_last_error.argtypes = []
+    lib.pstm_mlp_forward.restype = ctypes.c_int
+    lib.pstm_mlp_forward.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
+    lib.pstm_mlp_loss_grad.restype = ctypes.c_int
+    lib.pstm_mlp_loss_grad.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
+    lib.pstm_mlp_adam.restype = ctypes.c_int
+    lib.pstm_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, f, f, f, f, ctypes.c_int, _vp]
+    lib.pstm_mlp_train_step.restype = ctypes.c_int
+    lib.pstm_mlp_train_step.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, f, f, f, f,
+                                        ctypes.c_int, _vp, _vp]
+    lib.pstm_vti_leg_times.restype = ctypes.c_int
+    lib.pstm_vti_leg_times.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_longlong, _vp, _vp]
     return lib
 
 
@@ -113,6 +133,8 @@ def check_layout(lib: ctypes.CDLL) -> None:
     ver = lib.pstm_hip_abi_version()
     if ver != ABI_VERSION:
         raise RuntimeError(f"libpstm_hip ABI version {ver} != {ABI_VERSION}")
+    if lib.pstm_mlp_param_count() != MLP_PARAMS:
+        raise RuntimeError("libpstm_hip MLP parameter count mismatch")
     size = lib.pstm_hip_args_size()
     if size != ctypes.sizeof(PstmMigrateArgs):
         raise RuntimeError(

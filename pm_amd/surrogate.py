@@ -1,0 +1,246 @@
+"""Learned VTI traveltime surrogate: data, training and evaluation.
+
+The network (5 -> 64 -> 64 -> 64 -> 64 -> 2, SiLU) maps the normalised
+(tau, d_s, d_r, v_rms, eta) of a (pillar, trace, t0) triple to the anisotropic
+corrections (rho_s, rho_r) of the two one-way legs,
+
+    t_leg = sqrt(tau^2 + d^2 / v^2) * (1 + rho),
+
+where the exact leg time is the VTI one-way time of
+pstm/algorithm/anisotropy_vti.py:181-220.  Training samples are drawn on the
+job's own v_rms and eta fields (random x, y, t0 in the model, random source and
+receiver distances up to the aperture plus half the maximum offset) and their
+targets come from an f64 HIP kernel; forward, backward and Adam are HIP
+kernels (pm_amd/csrc/pstm_mlp.hip).  Tensors are device memory only.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from pm_amd.kernels import _abi
+
+
+def _p(t) -> int:
+    return t.data_ptr()
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check(lib, rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib.pstm_mlp_last_error().decode()
+        raise (ValueError if rc == 1 else RuntimeError)(f"{what}: {msg}")
+
+
+@dataclass
+class Normalizer:
+    """Affine maps of the five inputs to about [-1, 1]."""
+
+    tau_max: float
+    d_max: float
+    v_lo: float
+    v_hi: float
+    eta_max: float
+
+    def scales(self) -> tuple[np.ndarray, np.ndarray]:
+        """x_norm = x * scale + offset, per input (tau, d_s, d_r, v, eta)."""
+        sc = np.array([2.0 / self.tau_max, 2.0 / self.d_max, 2.0 / self.d_max,
+                       2.0 / (self.v_hi - self.v_lo), 2.0 / max(self.eta_max, 1e-6)])
+        off = np.array([-1.0, -1.0, -1.0, -1.0 - 2.0 * self.v_lo / (self.v_hi - self.v_lo), -1.0])
+        return sc, off
+
+    def apply(self, x):
+        """x (n, 5) torch f64/f32 -> normalised f32."""
+        import torch
+
+        sc, off = self.scales()
+        s = torch.as_tensor(sc, dtype=x.dtype, device=x.device)
+        o = torch.as_tensor(off, dtype=x.dtype, device=x.device)
+        return (x * s + o).float().contiguous()
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """Seeded He-uniform weights (SiLU ~ ReLU gain), zero biases, in the
+    library's layout (per layer W (out x in) then b)."""
+    rng = np.random.default_rng(seed)
+    sizes = [(5, 64), (64, 64), (64, 64), (64, 64), (64, 2)]
+    out = []
+    for li, (fin, fout) in enumerate(sizes):
+        lim = np.sqrt(6.0 / fin) if li < 4 else np.sqrt(1.0 / fin) * 0.1
+        out.append(rng.uniform(-lim, lim, (fout, fin)).ravel())
+        out.append(np.zeros(fout))
+    p = np.concatenate(out).astype(np.float32)
+    assert p.size == _abi.MLP_PARAMS
+    return p
+
+
+def vti_leg_times(tau, h, v, eta):
+    """Exact one-way VTI times on the device (f64 torch tensors)."""
+    import torch
+
+    lib = _abi.load_library()
+    tau, h, v, eta = (a.to(torch.float64).contiguous() for a in (tau, h, v, eta))
+    out = torch.empty_like(tau)
+    _check(lib, lib.pstm_vti_leg_times(_p(tau), _p(h), _p(v), _p(eta), int(tau.numel()), _p(out),
+                                       _stream()), "pstm_vti_leg_times")
+    return out
+
+
+def hyperbolic(tau, d, v):
+    return (tau * tau + (d / v) ** 2).sqrt()
+
+
+@dataclass
+class TrainingSet:
+    x_raw: object      # (n, 5) f64: tau, d_s, d_r, v, eta
+    x: object          # (n, 5) f32 normalised
+    y: object          # (n, 2) f32: rho_s, rho_r
+    t_exact: object    # (n, 2) f64 exact leg times
+
+
+def make_training_set(vcube, ecube, n: int, d_max: float, norm: Normalizer, seed: int,
+                      device="cuda", t_max_ms: float | None = None) -> TrainingSet:
+    """n samples on the job's own model: x, y uniform over the cube's extent,
+    t0 uniform over [0, t_max], d_s, d_r uniform over [0, d_max]; v and eta by
+    trilinear interpolation of the cubes."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    xs = rng.uniform(vcube.x_axis[0], vcube.x_axis[-1], n)
+    ys = rng.uniform(vcube.y_axis[0], vcube.y_axis[-1], n)
+    tmax = t_max_ms if t_max_ms is not None else vcube.t_axis_ms[-1]
+    ts = rng.uniform(0.0, tmax, n)
+    ds = rng.uniform(0.0, d_max, n)
+    dr = rng.uniform(0.0, d_max, n)
+    v = torch.as_tensor(_trilinear(vcube, xs, ys, ts), device=device)
+    e = torch.as_tensor(_trilinear(ecube, xs, ys, ts), device=device)
+    tau = torch.as_tensor(ts / 2000.0, device=device)
+    dst = torch.as_tensor(ds, device=device)
+    drt = torch.as_tensor(dr, device=device)
+    t_s = vti_leg_times(tau, dst, v, e)
+    t_r = vti_leg_times(tau, drt, v, e)
+    hs, hr = hyperbolic(tau, dst, v), hyperbolic(tau, drt, v)
+    ok = (hs > 0) & (hr > 0)
+    rho_s = torch.where(ok, t_s / hs.clamp_min(1e-30) - 1.0, torch.zeros_like(hs))
+    rho_r = torch.where(ok, t_r / hr.clamp_min(1e-30) - 1.0, torch.zeros_like(hr))
+    xr = torch.stack([tau, dst, drt, v, e], 1)
+    return TrainingSet(xr, norm.apply(xr), torch.stack([rho_s, rho_r], 1).float().contiguous(),
+                       torch.stack([t_s, t_r], 1))
+
+
+def _trilinear(cube, x, y, t):
+    """Trilinear interpolation of a VelocityCube-like grid at scattered points
+    (numpy, clamped), vectorised."""
+    def idx(axis, p):
+        axis = np.asarray(axis, dtype=np.float64)
+        p = np.clip(p, axis[0], axis[-1])
+        i = np.clip(np.searchsorted(axis, p, side="right") - 1, 0, len(axis) - 2)
+        return i, (p - axis[i]) / (axis[i + 1] - axis[i])
+
+    ix, fx = idx(cube.x_axis, x)
+    iy, fy = idx(cube.y_axis, y)
+    it, ft = idx(cube.t_axis_ms, t)
+    d = cube.data
+    out = np.zeros(len(x))
+    for a in (0, 1):
+        wa = fx if a else 1 - fx
+        for b in (0, 1):
+            wb = fy if b else 1 - fy
+            for c in (0, 1):
+                wc = ft if c else 1 - ft
+                out += wa * wb * wc * d[ix + a, iy + b, it + c]
+    return out
+
+
+@dataclass
+class TraveltimeSurrogate:
+    """Parameters (device f32), Adam moments and the input normaliser."""
+
+    norm: Normalizer
+    params: object = None
+    m: object = None
+    v: object = None
+    grads: object = None
+    step: int = 0
+    history: list = field(default_factory=list)
+
+    @classmethod
+    def create(cls, norm: Normalizer, seed: int = 0, device="cuda") -> "TraveltimeSurrogate":
+        import torch
+
+        p = torch.as_tensor(init_params(seed), device=device)
+        return cls(norm, p, torch.zeros_like(p), torch.zeros_like(p), torch.zeros_like(p))
+
+    def forward(self, x):
+        """Network output (n, 2) for normalised inputs x (n, 5) f32."""
+        import torch
+
+        lib = _abi.load_library()
+        x = x.float().contiguous()
+        y = torch.empty((x.shape[0], 2), dtype=torch.float32, device=x.device)
+        _check(lib, lib.pstm_mlp_forward(_p(self.params), _p(x), int(x.shape[0]), _p(y), _stream()),
+               "pstm_mlp_forward")
+        return y
+
+    def loss_grad(self, x, y):
+        import torch
+
+        lib = _abi.load_library()
+        loss = torch.zeros(1, dtype=torch.float32, device=x.device)
+        _check(lib, lib.pstm_mlp_loss_grad(_p(self.params), _p(x), _p(y), int(x.shape[0]), _p(self.grads),
+                                           _p(loss), _stream()), "pstm_mlp_loss_grad")
+        return loss, self.grads
+
+    def train(self, ts: TrainingSet, steps: int, batch: int = 8192, lr: float = 2e-3,
+              lr_final: float = 5e-5, seed: int = 0, log_every: int = 0):
+        """Adam with a cosine-decayed learning rate over `steps` minibatches."""
+        import torch
+
+        lib = _abi.load_library()
+        n = int(ts.x.shape[0])
+        g = torch.Generator(device=ts.x.device)
+        g.manual_seed(seed)
+        loss = torch.zeros(1, dtype=torch.float32, device=ts.x.device)
+        for i in range(steps):
+            idx = torch.randint(0, n, (batch,), device=ts.x.device, generator=g)
+            xb = ts.x.index_select(0, idx).contiguous()
+            yb = ts.y.index_select(0, idx).contiguous()
+            self.step += 1
+            a = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * i / max(steps - 1, 1)))
+            _check(lib, lib.pstm_mlp_train_step(_p(self.params), _p(self.m), _p(self.v), _p(self.grads),
+                                                _p(xb), _p(yb), batch, float(a), 0.9, 0.999, 1e-8,
+                                                self.step, _p(loss), _stream()), "pstm_mlp_train_step")
+            if log_every and (i % log_every == 0 or i == steps - 1):
+                self.history.append((i, float(loss.item())))
+        return self
+
+    def leg_times(self, ts: TrainingSet):
+        """Predicted (t_s, t_r) f64 for a sample set."""
+        rho = self.forward(ts.x).double()
+        tau, ds, dr, v = ts.x_raw[:, 0], ts.x_raw[:, 1], ts.x_raw[:, 2], ts.x_raw[:, 3]
+        import torch
+
+        return torch.stack([hyperbolic(tau, ds, v) * (1 + rho[:, 0]),
+                            hyperbolic(tau, dr, v) * (1 + rho[:, 1])], 1)
+
+    def error_samples(self, ts: TrainingSet, dt_ms: float) -> dict:
+        """Error of t_s + t_r against the exact sum, in input samples."""
+        pred = self.leg_times(ts).sum(1)
+        exact = ts.t_exact.sum(1)
+        err = (pred - exact).abs() / (dt_ms / 1000.0)
+        return {"max": float(err.max()), "p99": float(err.quantile(0.99)) if err.numel() <= 16_000_000 else None,
+                "mean": float(err.mean())}
+
+    def bf16_params(self):
+        """Parameters rounded to bfloat16 (the precision the migration uses)."""
+        import torch
+
+        return self.params.to(torch.bfloat16)

@@ -197,3 +197,65 @@ def reflector_job(seed: int = 2, n_traces: int = 200_000, n_samples: int = 1501,
     return SyntheticJob("reflectors", tr.float().contiguous(), sx, sy, rx, ry, 2.0, 0.0, x,
                         x.copy(), 2.0 * np.arange(nt), cfg, cube=cube, tile=tile,
                         meta={"diffractors": np.stack([xd, yd, t0d], 1)})
+
+
+def vti_model(n_xy: int = 128, n_t: int = 201, x_lo: float = -500.0, x_hi: float = 6887.5,
+              t_hi_ms: float = 4000.0) -> tuple[VelocityCube, VelocityCube]:
+    """Smooth 3-D v_rms (1500-4500 m/s) and eta (0-0.2) fields on one grid:
+    v = 1500 + 900 t0 (1 + 0.15 sin(2 pi x / Lx) cos(2 pi y / Ly)) and
+    eta = 0.1 + 0.08 sin(2 pi (x + y) / L) * min(1, t0), clipped."""
+    x = np.linspace(x_lo, x_hi, n_xy)
+    t = np.linspace(0.0, t_hi_ms, n_t)
+    X, Y, T = np.meshgrid(x, x, t / 1000.0, indexing="ij")
+    span = x_hi - x_lo
+    v = 1500.0 + 900.0 * T * (1.0 + 0.15 * np.sin(2 * np.pi * X / span) * np.cos(2 * np.pi * Y / span))
+    eta = 0.1 + 0.08 * np.sin(2 * np.pi * (X + Y) / (1.3 * span)) * np.minimum(1.0, T)
+    return (VelocityCube(np.clip(v, 1500.0, 4500.0), x, x.copy(), t),
+            VelocityCube(np.clip(eta, 0.0, 0.2), x, x.copy(), t))
+
+
+def _vti_leg(tau, h, v, eta):
+    """One-way VTI time (anisotropy_vti.py:181-220) as f64 tensor arithmetic,
+    for synthesising data (tau > 0, v > 0)."""
+    h2, v2, t02 = h * h, v * v, tau * tau
+    den = v2 * (t02 * v2 + (1.0 + 2.0 * eta) * h2)
+    return (t02 + h2 / v2 - 2.0 * eta * h2 * h2 / den).clamp_min(0.0).sqrt()
+
+
+def vti_diffractor_job(seed: int = 4, n_traces: int = 500_000, n_samples: int = 2001, n_out: int = 512,
+                       nt: int = 1500, d_out: float = 12.5, tile: int = 64, n_diffractors: int = 300,
+                       device: str = "cpu") -> SyntheticJob:
+    """VTI job: random diffractors at the exact two-leg VTI time with the local
+    v_rms and eta, 30 Hz Ricker; output n_out x n_out bins x nt samples."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    x_hi = d_out * (n_out - 1)
+    lo, hi = -500.0, x_hi + 500.0
+    sx, sy, rx, ry = _geometry(rng, n_traces, lo, hi, 3000.0)
+    vc, ec = vti_model(x_lo=lo, x_hi=hi)
+    dt_s = 0.002
+    tr = torch.zeros((n_traces, n_samples), dtype=torch.float64, device=device)
+    S = [torch.as_tensor(a, device=device) for a in (sx, sy, rx, ry)]
+    xd = rng.uniform(0.0, x_hi, n_diffractors)
+    yd = rng.uniform(0.0, x_hi, n_diffractors)
+    t0d = rng.uniform(0.2, 0.98 * nt * dt_s, n_diffractors)
+    ad = rng.uniform(0.5, 1.0, n_diffractors)
+    vd = vc.at(xd, yd, 1000.0 * t0d)
+    ed = ec.at(xd, yd, 1000.0 * t0d)
+    for i in range(n_diffractors):
+        tau = torch.full_like(S[0], t0d[i] / 2)
+        hs = ((S[0] - xd[i]) ** 2 + (S[1] - yd[i]) ** 2).sqrt()
+        hr = ((S[2] - xd[i]) ** 2 + (S[3] - yd[i]) ** 2).sqrt()
+        vv = torch.full_like(tau, vd[i])
+        ee = torch.full_like(tau, ed[i])
+        t = _vti_leg(tau, hs, vv, ee) + _vti_leg(tau, hr, vv, ee)
+        add_events(tr, t, torch.full_like(t, ad[i]), 30.0, dt_s)
+    x = d_out * np.arange(n_out)
+    cfg = KernelConfig(max_aperture_m=2500.0, min_aperture_m=500.0, max_dip_degrees=60.0,
+                       taper_fraction=0.1, apply_spreading=True, apply_obliquity=True,
+                       grid_dx=d_out, grid_dy=d_out, kernel_type="anisotropic_vti", vti_enabled=True)
+    job = SyntheticJob("vti_diffractors", tr.float().contiguous(), sx, sy, rx, ry, 2.0, 0.0, x, x.copy(),
+                       2.0 * np.arange(nt), cfg, cube=vc, tile=tile,
+                       meta={"eta_cube": ec, "diffractors": np.stack([xd, yd, t0d], 1)})
+    return job

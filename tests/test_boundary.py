@@ -107,7 +107,7 @@ def test_initialize_without_gpu_fails_loudly():
 
 def _declared_functions() -> list[str]:
     text = (ROOT / "include" / "pstm_hip.h").read_text()
-    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(pstm_hip_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(pstm_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
