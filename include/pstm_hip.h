@@ -28,12 +28,16 @@
 extern "C" {
 #endif
 
-#define PSTM_HIP_ABI_VERSION 1
+#define PSTM_HIP_ABI_VERSION 2
 
 /* traveltime: KernelConfig.kernel_type (pstm/kernels/base.py:220-229) */
 #define PSTM_TT_STRAIGHT 0 /* DSR, numba_cpu_optimized.py:189-194            */
 #define PSTM_TT_CURVED 1   /* V(z)=V0+kz, pstm/algorithm/curved_ray.py:19-52 */
 #define PSTM_TT_VTI 2      /* eta, pstm/algorithm/anisotropy_vti.py:148-220 */
+/* VTI with the learned traveltime surrogate evaluated inside the gather (BF16
+ * MFMA, f32 accumulation); no counterpart in the reference.  See the mlp_*
+ * fields below. */
+#define PSTM_TT_VTI_LEARNED 3
 
 /* anti-aliasing (the reference's registered kernel has none) */
 #define PSTM_AA_NONE 0
@@ -99,6 +103,25 @@ typedef struct PstmMigrateArgs {
     double curved_v0;             /* PSTM_TT_CURVED: V(z) = v0 + k z      */
     double curved_k;
     double eta_constant;          /* PSTM_ETA_CONST                       */
+    /* --- learned VTI traveltime (PSTM_TT_VTI_LEARNED only) ---
+     * Per (pillar, trace) pair the network is evaluated on a lattice of output
+     * samples k = 0, K, 2K, ... (K = mlp_node_spacing) with inputs
+     *   x = (2 tau / tau_max - 1, 2 s_s - 1, 2 s_r - 1,
+     *        2 (v - v_lo) / (v_hi - v_lo) - 1, 2 eta / eta_max - 1),
+     * tau = t0 / 2, v and eta at (pillar, t0), s = sine of the straight-ray
+     * angle of a leg, d / (v sqrt(tau^2 + d^2 / v^2)).  Outputs are
+     * out_scale * c for each leg; c is interpolated linearly between lattice
+     * samples and the leg time is sqrt(tau^2 + (d / v)^2 (1 + c)), with t = t0
+     * for t0 <= 0 as in the exact VTI time.                                 */
+    const float* mlp_params;      /* PSTM_MLP_PARAMS f32 (network layout below);
+                                     weights are rounded to bf16 on use       */
+    int32_t mlp_node_spacing;     /* K: 8, 16, 32 or 64 output samples        */
+    int32_t mlp_reserved;         /* 0                                        */
+    double mlp_tau_max;
+    double mlp_v_lo;
+    double mlp_v_hi;
+    double mlp_eta_max;
+    double mlp_out_scale;
 } PstmMigrateArgs;
 
 /* ABI identity, for the loader's layout check. */
@@ -159,10 +182,10 @@ int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_star
 
 /* ---------------------------------------------------------------------------
  * Traveltime surrogate (no counterpart in the reference: the learned part of
- * this backend).  A 5-64-64-64-64-2 SiLU MLP; inputs are the normalised
- * (tau, d_s, d_r, v, eta), outputs the anisotropic corrections rho_s, rho_r
- * with t_leg = sqrt(tau^2 + d^2/v^2) (1 + rho).  Parameters: one f32 buffer of
- * PSTM_MLP_PARAMS values, per layer W (out x in, row-major) then b (out).
+ * this backend).  A 5-64-64-64-64-2 SiLU MLP; inputs are the encoded
+ * (tau, s_s, s_r, v, eta) of PstmMigrateArgs.mlp_*, outputs out_scale * c per
+ * leg with t_leg = sqrt(tau^2 + (d/v)^2 (1 + c)).  Parameters: one f32 buffer
+ * of PSTM_MLP_PARAMS values, per layer W (out x in, row-major) then b (out).
  * All pointers are device pointers; calls are asynchronous on `stream`.
  * ------------------------------------------------------------------------- */
 #define PSTM_MLP_IN 5
@@ -173,8 +196,22 @@ int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_star
 int pstm_mlp_param_count(void);
 const char* pstm_mlp_last_error(void);
 
+/* Precision of pstm_mlp_forward / _loss_grad / _train_step for the calling
+ * process: 0 = f32 (default); 1 = the numerics of the inference fused into the
+ * migration (weights rounded to bf16; the input and every hidden activation
+ * carried as a bf16 hi+lo pair; f32 accumulation and biases), which makes
+ * training quantisation-aware with straight-through gradients. */
+int pstm_mlp_set_precision(int bf16);
+
 /* y (n, 2) = network(x (n, 5)), f32. */
 int pstm_mlp_forward(const float* params, const float* x, int n, float* y, void* stream);
+
+/* y (n, 2) = network(x (n, 5)) through the device code the migration's
+ * learned traveltime uses (BF16 MFMA, f32 accumulation; the numbers of
+ * pstm_mlp_forward with pstm_mlp_set_precision(1) up to summation order).
+ * Uses the migration library's workspace: do not overlap with a migration
+ * call on another stream. */
+int pstm_mlp_forward_mfma(const float* params, const float* x, int n, float* y, void* stream);
 
 /* Mean squared error of network(x) against target (n, 2) into *loss (device
  * f32) and its gradient w.r.t. every parameter into grads. */

@@ -70,3 +70,31 @@ def adam(p, g, m, v, lr, b1, b2, eps, step):
     mh = m / (1 - b1 ** step)
     vh = v / (1 - b2 ** step)
     return p - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def bf16_round(a):
+    """Nearest bfloat16 (ties to even) of float32 values, returned as float64."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def bf16_pair(a):
+    """a (rounded to float32) as the sum of two bfloat16 values hi + lo."""
+    a32 = np.asarray(a, dtype=np.float32)
+    hi = bf16_round(a32)
+    lo = bf16_round((a32.astype(np.float64) - hi).astype(np.float32))
+    return hi + lo
+
+
+def forward_bf16(p, x):
+    """Forward pass with the numerics of the fused inference: weights rounded to
+    bfloat16, the input and each hidden activation carried as a bfloat16 pair,
+    products and sums exact (f64 here; f32 accumulation on the device),
+    biases in f32."""
+    ls = unpack(np.asarray(p, np.float32).astype(np.float64))
+    a = bf16_pair(x)
+    for i, (W, b) in enumerate(ls):
+        z = a @ bf16_round(W).T + b
+        a = bf16_pair(silu(z)) if i < len(ls) - 1 else z
+    return a

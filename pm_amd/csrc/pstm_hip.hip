@@ -116,6 +116,14 @@ struct KParams {
     float lo_eps;                // eps_u where u >= 0 can be decided differently from f64, else -inf
     float eps_u, eps_x;          // f32 bands around the sample bounds and the aperture edge
     double tan_dip, min_ap64, max_ap64, t_start_s, dt_s, cv0, ck, eta_const;
+    // learned VTI traveltime: network operand images and the t0 lattice
+    const void* mlpA;            // bf16 weight operands (k_prep_mlp)
+    const void* mlpB;            // f32 biases in accumulator layout
+    int lat_off;                 // byte offset of the lattice in LDS
+    int lat_nodes;               // lattice samples per (pair, window): 256 MW / K + 1
+    int lat_shift;               // log2 K
+    float lat_ifrac;             // 1 / K
+    float m_tau_sc, m_v_lo, m_v_sc, m_eta_sc, m_inv_out;
 };
 
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -364,6 +372,156 @@ __device__ __forceinline__ bool exact_decision(const KParams& P, int pil, int ro
 // the gather
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// fused traveltime surrogate: BF16 MFMA, f32 accumulation
+// ---------------------------------------------------------------------------
+// One wavefront evaluates the network for 32 rows with
+// v_mfma_f32_32x32x8_bf16 in the transposed form D = W . X^T: A = weights
+// (32 outputs x 8 inputs), B = activations (8 inputs x 32 rows), D = 32
+// outputs x 32 rows.  Lane l holds row l % 32; its accumulator register r
+// holds output 8 (r / 4) + 4 (l / 32) + r % 4 of the M tile, which is exactly
+// the B operand layout (inputs 8 ks + 4 (l / 32) + 0..3) of K step
+// ks = 4 mt + r / 4 of the next layer: activations never leave registers.
+// The input and every hidden activation enter as a bf16 pair hi + lo (two
+// products per K step), the weights as bf16.
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16v4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMlpLayers = 5;
+__host__ __device__ constexpr int mlp_in(int l) { return l == 0 ? PSTM_MLP_IN : PSTM_MLP_HIDDEN; }
+__host__ __device__ constexpr int mlp_out(int l) { return l == kMlpLayers - 1 ? PSTM_MLP_OUT : PSTM_MLP_HIDDEN; }
+__host__ __device__ constexpr int mlp_w_off(int l) {
+    int off = 0;
+    for (int i = 0; i < l; ++i) off += mlp_in(i) * mlp_out(i) + mlp_out(i);
+    return off;
+}
+static_assert(mlp_w_off(kMlpLayers) == PSTM_MLP_PARAMS, "network layout");
+// weight operands, in bf16x4 units: layer 0 [mt 2][lane 64]; layers 1-3
+// [mt 2][ks 8][lane 64]; layer 4 (outputs padded to 32) [ks 8][lane 64]
+constexpr int kA1 = 2 * 64, kA4 = kA1 + 3 * 16 * 64, kANum = kA4 + 8 * 64;
+// biases in accumulator layout, in f32x16 units: layers 0-3 [mt 2][lane 64],
+// layer 4 [lane 64]
+constexpr int kB4 = 4 * 2 * 64, kBNum = kB4 + 64;
+
+__host__ __device__ inline unsigned short bf16_bits(float f) {
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // nearest, ties to even
+    return (unsigned short)(u >> 16);
+}
+
+__global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restrict__ A, float* __restrict__ B) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < kANum * 4) {
+        const int v = e / 4, el = e % 4;
+        int l, mt, ks, lane;
+        if (v < kA1) {
+            l = 0; mt = v / 64; ks = 0; lane = v % 64;
+        } else if (v < kA4) {
+            const int r = v - kA1;
+            l = 1 + r / (16 * 64); mt = (r / (8 * 64)) % 2; ks = (r / 64) % 8; lane = r % 64;
+        } else {
+            l = 4; mt = 0; ks = (v - kA4) / 64; lane = (v - kA4) % 64;
+        }
+        const int o = mt * 32 + lane % 32, i = 8 * ks + 4 * (lane / 32) + el;
+        const float w = (o < mlp_out(l) && i < mlp_in(l)) ? P[mlp_w_off(l) + o * mlp_in(l) + i] : 0.0f;
+        A[e] = bf16_bits(w);
+    } else if (e < kANum * 4 + kBNum * 16) {
+        const int f = e - kANum * 4, v = f / 16, r = f % 16;
+        int l, mt, lane;
+        if (v < kB4) {
+            l = v / 128; mt = (v / 64) % 2; lane = v % 64;
+        } else {
+            l = 4; mt = 0; lane = v - kB4;
+        }
+        const int o = mt * 32 + 8 * (r / 4) + 4 * (lane / 32) + r % 4;
+        B[f] = o < mlp_out(l) ? P[mlp_w_off(l) + mlp_in(l) * mlp_out(l) + o] : 0.0f;
+    }
+}
+
+__device__ __forceinline__ void split_bf16(float a0, float a1, float a2, float a3, bf16x4_t& hi, bf16x4_t& lo) {
+    bf16v4_t h;
+    h.x = (__bf16)a0; h.y = (__bf16)a1; h.z = (__bf16)a2; h.w = (__bf16)a3;
+    bf16v4_t r;
+    r.x = (__bf16)(a0 - (float)h.x); r.y = (__bf16)(a1 - (float)h.y);
+    r.z = (__bf16)(a2 - (float)h.z); r.w = (__bf16)(a3 - (float)h.w);
+    hi = __builtin_bit_cast(bf16x4_t, h);
+    lo = __builtin_bit_cast(bf16x4_t, r);
+}
+
+__device__ __forceinline__ float silu(float z) { return z * frcp(1.0f + __expf(-z)); }
+
+// SiLU of the two M tiles, re-packed as the next layer's B operands
+__device__ __forceinline__ void act_pack(const f32x16_t (&acc)[2], bf16x4_t (&h)[8], bf16x4_t (&l)[8]) {
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            split_bf16(silu(acc[mt][4 * g]), silu(acc[mt][4 * g + 1]), silu(acc[mt][4 * g + 2]),
+                       silu(acc[mt][4 * g + 3]), h[4 * mt + g], l[4 * mt + g]);
+}
+
+__device__ __forceinline__ f32x16_t mfma2(bf16x4_t a, bf16x4_t bh, bf16x4_t bl, f32x16_t c) {
+    c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bh, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, c, 0, 0, 0);
+}
+
+// Network outputs (y0, y1) of the row held by this lane (valid in lanes
+// 0-31; lanes l and l + 32 carry the same row).  Wave-uniform call, all
+// lanes active.
+__device__ __forceinline__ void mlp_eval32(const bf16x4_t* __restrict__ WA, const f32x16_t* __restrict__ WB,
+                                           const float (&x)[5], float& y0, float& y1) {
+    const int lane = threadIdx.x & 63;
+    const bool upper = lane >= 32;
+    bf16x4_t h[8], l[8];
+    {
+        bf16x4_t xh, xl;
+        split_bf16(upper ? x[4] : x[0], upper ? 0.0f : x[1], upper ? 0.0f : x[2], upper ? 0.0f : x[3], xh, xl);
+        f32x16_t acc[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma2(WA[mt * 64 + lane], xh, xl, WB[mt * 64 + lane]);
+        act_pack(acc, h, l);
+    }
+#pragma unroll 1
+    for (int layer = 1; layer < 4; ++layer) {
+        const bf16x4_t* wa = WA + kA1 + (layer - 1) * 16 * 64;
+        f32x16_t acc[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            acc[mt] = WB[(layer * 2 + mt) * 64 + lane];
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks) acc[mt] = mfma2(wa[(mt * 8 + ks) * 64 + lane], h[ks], l[ks], acc[mt]);
+        }
+        act_pack(acc, h, l);
+    }
+    f32x16_t o = WB[kB4 + lane];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) o = mfma2(WA[kA4 + ks * 64 + lane], h[ks], l[ks], o);
+    y0 = o[0];
+    y1 = o[1];
+}
+
+// Batched inference through the fused path's device code: one wavefront per
+// 32 rows of x (n, 5), outputs y (n, 2).
+__global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x4_t* __restrict__ WA,
+                                                          const f32x16_t* __restrict__ WB,
+                                                          const float* __restrict__ x, int n, float* __restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const long wv = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long r = wv * 32 + (lane & 31);
+    float xi[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (r < n) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) xi[i] = x[r * 5 + i];
+    }
+    float y0, y1;
+    mlp_eval32(WA, WB, xi, y0, y1);
+    if (lane < 32 && r < n) {
+        y[2 * r] = y0;
+        y[2 * r + 1] = y1;
+    }
+}
+
 // Traveltime of one contribution from the pair terms e/f (LDS) and the
 // (pillar, sample) terms.  y1/y2 return 1/t_src, 1/t_rec for anti-aliasing.
 template <int TT, bool AA>
@@ -391,6 +549,13 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         // k > 0: v(z) >= v0, so the exact time is >= 0 (and exactly 0 at t0 = 0,
         // where the f32 ratio only rounds to 1 +- ulp)
         t = fmaxf(t, P.t_floor);
+    } else if (TT == PSTM_TT_VTI_LEARNED) {
+        // learned legs: t^2 = tau^2 + h^2 s (1 + c), c from the lattice
+        // (e2v, e1v here); 1 + c >= 0 keeps t >= t0 as the exact VTI time is
+        const float a1 = fmaf(e.x * cs, fmaxf(1.0f + e2v, 0.0f), q);
+        const float a2 = fmaf(e.y * cs, fmaxf(1.0f + e1v, 0.0f), q);
+        t = fsqrt(a1) + fsqrt(a2);
+        if (!(t0v > 0.f)) t = t0v;
     } else {
         // VTI one-way legs (anisotropy_vti.py:181-220) with s = 1/v^2:
         // t^2 = tau^2 + h^2 s - 2 eta (h^2 s)^2 / (tau^2 + (1 + 2 eta) h^2 s)
@@ -523,6 +688,15 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         const int pt_p = tid / P.batch, pt_j = tid % P.batch;
         const bool pt_thread = tid < kNPil * P.batch;
 
+        // learned traveltime: lattice sample below each slot and the lerp fraction
+        int nidx[MW];
+        float nfr = 0.0f;
+        if (TT == PSTM_TT_VTI_LEARNED) {
+#pragma unroll
+            for (int m = 0; m < MW; ++m) nidx[m] = (64 * (tw + kTW * m) + lane) >> P.lat_shift;
+            nfr = (float)(lane & ((1 << P.lat_shift) - 1)) * P.lat_ifrac;
+        }
+
         for (int j0 = 0; j0 < P.ntr; j0 += P.batch) {
             const int nb = min(P.batch, P.ntr - j0);
             // (pillar, trace) pair table; the max-aperture test is the
@@ -566,6 +740,49 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
             if (!__syncthreads_or(inap)) continue;
 
+            if (TT == PSTM_TT_VTI_LEARNED) {
+                // network on the t0 lattice of every in-aperture pair of the
+                // batch, 32 (pair, lattice sample) rows per wavefront step
+                const int nn = P.lat_nodes;
+                const int R = kNPil * P.batch * nn;
+                float2* lat = reinterpret_cast<float2*>(smem + P.lat_off);
+                const float dt_last = P.nt > 1 ? P.t0k[P.nt - 1] - P.t0k[P.nt - 2] : 0.0f;
+                for (int c0 = wave * 32; c0 < R; c0 += kWaves * 32) {
+                    const int r = c0 + (lane & 31);
+                    bool valid = false;
+                    float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                    if (r < R) {
+                        const int pp = r / nn, node = r - pp * nn;
+                        const float4 e = pairs[pp * kPairVec];
+                        if (e.z != INFINITY) {
+                            valid = true;
+                            const int pil = pil0 + pp / P.batch;
+                            const int k = kbase + (node << P.lat_shift);
+                            const int kk = min(k, P.nt - 1);
+                            const float t0 = P.t0k[kk] + (float)(k - kk) * dt_last;
+                            const float tau = 0.5f * t0;
+                            const float q = tau * tau;
+                            const float sl = P.kc[(size_t)pil * P.kc_pstride + kk].x;  // 1/v^2
+                            const float eta = 0.5f * P.ek[(size_t)pil * P.ek_pstride + kk].x;
+                            const float hs = e.x * sl, hr = e.y * sl;
+                            const float ts2 = q + hs, tr2 = q + hr;
+                            const float sn_s = ts2 > 0.0f ? fsqrt(hs / ts2) : 0.0f;
+                            const float sn_r = tr2 > 0.0f ? fsqrt(hr / tr2) : 0.0f;
+                            const float v = frsq(sl);
+                            x[0] = fmaf(tau, P.m_tau_sc, -1.0f);
+                            x[1] = fmaf(2.0f, sn_s, -1.0f);
+                            x[2] = fmaf(2.0f, sn_r, -1.0f);
+                            x[3] = fmaf(v - P.m_v_lo, P.m_v_sc, -1.0f);
+                            x[4] = fmaf(eta, P.m_eta_sc, -1.0f);
+                        }
+                    }
+                    if (!__any(valid)) continue;  // wave-uniform
+                    float y0, y1;
+                    mlp_eval32(static_cast<const bf16x4_t*>(P.mlpA), static_cast<const f32x16_t*>(P.mlpB), x, y0, y1);
+                    if (lane < 32 && valid) lat[r] = make_float2(y0 * P.m_inv_out, y1 * P.m_inv_out);
+                }
+            }
+
             // stage the batch of trace rows in LDS (coalesced)
             if (AA) {
                 const double* src = P.prow + (size_t)j0 * P.rowlen;
@@ -606,19 +823,30 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     if (!any_in) continue;  // time-dependent aperture excludes every slot
                     float t[MW], u[MW], y1[MW], y2[MW];
                     bool ok[MW];
+                    if (TT == PSTM_TT_VTI_LEARNED) {
+                        const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * P.lat_nodes;
+#pragma unroll
+                        for (int m = 0; m < MW; ++m) {
+                            const float2 c0 = lp[nidx[m]], c1 = lp[nidx[m] + 1];
+                            e2[p][m] = fmaf(nfr, c1.x - c0.x, c0.x);
+                            e1[p][m] = fmaf(nfr, c1.y - c0.y, c0.y);
+                        }
+                    }
 #pragma unroll
                     for (int m = 0; m < MW; ++m) {
                         t[m] = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m], y1[m],
                                                     y2[m]);
                         u[m] = fmaf(t[m], P.idt, P.uoff);
                         // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
-                        const bool in_ap = xt[m] <= (COUNT ? 1.0f - P.eps_x : 1.0f);
+                        const bool in_ap = xt[m] <= ((COUNT && TT != PSTM_TT_VTI_LEARNED) ? 1.0f - P.eps_x : 1.0f);
                         const bool below = u[m] < P.umax_lo;
                         bool near = in_ap && !below && u[m] < P.umax_hi;
                         if (LOB) near = near || (in_ap && u[m] < P.lo_eps && u[m] > -P.lo_eps);
                         if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
+                        // the learned time has no f64 counterpart to decide against
+                        if (TT == PSTM_TT_VTI_LEARNED) near = false;
                         any_near |= near;
-                        ok[m] = in_ap && below && !near;
+                        ok[m] = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u[m] < P.umax : below) && !near;
                         if (LOB) ok[m] = ok[m] && u[m] >= 0.0f;
                     }
                     float tw[MW];
@@ -653,7 +881,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 }
                 // slots near a threshold: recompute which, decide in f64 as the
                 // oracle does (rare; the slots were left out above)
-                if (any_near) {
+                if (TT != PSTM_TT_VTI_LEARNED && any_near) {
 #pragma unroll
                     for (int p = 0; p < kPW; ++p) {
                         const int pi = (pg * kPW + p) * P.batch + jj;
@@ -820,15 +1048,23 @@ int validate(const PstmMigrateArgs* a) {
                             !a->receiver_y || !a->midpoint_x || !a->midpoint_y))
         return fail(1, "trace pointers must be set");
     if (a->vel_mode != PSTM_VEL_1D && a->vel_mode != PSTM_VEL_3D) return fail(1, "invalid vel_mode");
-    if (a->traveltime < PSTM_TT_STRAIGHT || a->traveltime > PSTM_TT_VTI) return fail(1, "invalid traveltime");
+    if (a->traveltime < PSTM_TT_STRAIGHT || a->traveltime > PSTM_TT_VTI_LEARNED) return fail(1, "invalid traveltime");
     if (a->aa_mode != PSTM_AA_NONE && a->aa_mode != PSTM_AA_TRIANGLE) return fail(1, "invalid aa_mode");
     if (a->aa_mode == PSTM_AA_TRIANGLE && a->traveltime != PSTM_TT_STRAIGHT)
         return fail(1, "triangle anti-aliasing is implemented for the straight-ray traveltime only");
     if (a->aa_mode == PSTM_AA_TRIANGLE && (a->aa_max_half_width < 1 || a->aa_max_half_width > 64))
         return fail(1, "aa_max_half_width must be in [1, 64]");
-    if (a->traveltime == PSTM_TT_VTI) {
+    if (a->traveltime == PSTM_TT_VTI || a->traveltime == PSTM_TT_VTI_LEARNED) {
         if (a->eta_mode < PSTM_ETA_CONST || a->eta_mode > PSTM_ETA_3D) return fail(1, "invalid eta_mode");
         if (a->eta_mode != PSTM_ETA_CONST && !a->eta) return fail(1, "eta pointer must be set");
+    }
+    if (a->traveltime == PSTM_TT_VTI_LEARNED) {
+        const int K = a->mlp_node_spacing;
+        if (!a->mlp_params) return fail(1, "mlp_params must be set for the learned traveltime");
+        if (K != 8 && K != 16 && K != 32 && K != 64) return fail(1, "mlp_node_spacing must be 8, 16, 32 or 64");
+        if (!(a->mlp_tau_max > 0.0) || !(a->mlp_v_hi > a->mlp_v_lo) || !(a->mlp_eta_max > 0.0) ||
+            !(a->mlp_out_scale > 0.0))
+            return fail(1, "learned traveltime encoding: need tau_max > 0, v_hi > v_lo, eta_max > 0, out_scale > 0");
     }
     if (a->traveltime == PSTM_TT_CURVED && !(a->curved_v0 > 0.0))
         return fail(1, "curved_v0 must be positive");
@@ -911,6 +1147,8 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     const int rowlen = aa ? ns + 2 * lmax + 2 : ns;
     const size_t elem = aa ? sizeof(double) : sizeof(float);
     const int pair_vec = (aa || tt == PSTM_TT_CURVED) ? 2 : 1;
+    const bool learned = tt == PSTM_TT_VTI_LEARNED;
+    const bool vti_like = tt == PSTM_TT_VTI || learned;
 
     // batch size from the LDS budget
     // time windows
@@ -922,8 +1160,10 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     int mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
     // batch size from the LDS budget: trace rows + pair table + f64 totals
     const size_t tot_bytes = (size_t)kPW * mw * kThreads * sizeof(double);
+    const int lat_nodes = learned ? 256 * mw / a.mlp_node_spacing + 1 : 0;
     auto lds_for = [&](int b) {
-        return align_up(align_up((size_t)b * rowlen * elem, 16) + (size_t)kNPil * b * pair_vec * 16, 16) + tot_bytes;
+        return align_up(align_up((size_t)b * rowlen * elem, 16) + (size_t)kNPil * b * pair_vec * 16, 16) + tot_bytes +
+               align_up((size_t)kNPil * b * lat_nodes * 8, 16);
     };
     int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
     int batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
@@ -949,7 +1189,9 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     const size_t o_kc = off; off = align_up(off + (size_t)npv * nt * 16, 256);
     const size_t o_qk = off; off = align_up(off + (size_t)nt * 4, 256);
     const size_t o_t0 = off; off = align_up(off + (size_t)nt * 4, 256);
-    const size_t o_ek = off; off = align_up(off + (tt == PSTM_TT_VTI ? (size_t)nev * nt * 8 : 0), 256);
+    const size_t o_ek = off; off = align_up(off + (vti_like ? (size_t)nev * nt * 8 : 0), 256);
+    const size_t o_mlpA = off; off = align_up(off + (learned ? (size_t)kANum * 8 : 0), 256);
+    const size_t o_mlpB = off; off = align_up(off + (learned ? (size_t)kBNum * 64 : 0), 256);
     const size_t o_cnt = off; off = align_up(off + (size_t)grid * 8, 256);
     const size_t o_prow = off; off = align_up(off + (aa ? (size_t)ntr * rowlen * 8 : 0), 256);
     rc = ensure_workspace(off);
@@ -970,7 +1212,13 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     hipLaunchKernelGGL(k_prep_time_terms, dim3((unsigned)((nkc + 255) / 256)), dim3(256), 0, st, a, tt, npv, tan_dip,
                        kc, qk, t0k);
     HIP_TRY(hipGetLastError());
-    if (tt == PSTM_TT_VTI) {
+    if (learned) {
+        const int n = kANum * 4 + kBNum * 16;
+        hipLaunchKernelGGL(k_prep_mlp, dim3((n + 255) / 256), dim3(256), 0, st, a.mlp_params,
+                           reinterpret_cast<unsigned short*>(ws + o_mlpA), reinterpret_cast<float*>(ws + o_mlpB));
+        HIP_TRY(hipGetLastError());
+    }
+    if (vti_like) {
         const long nek = (long)nev * nt;
         hipLaunchKernelGGL(k_prep_eta, dim3((unsigned)((nek + 255) / 256)), dim3(256), 0, st, a, nev, ek);
         HIP_TRY(hipGetLastError());
@@ -1033,14 +1281,13 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.my64 = a.midpoint_y;
     kp.vrms64 = a.vrms;
     kp.tax64 = a.t_axis_ms;
-    kp.eta64 = (a.traveltime == PSTM_TT_VTI && a.eta_mode != PSTM_ETA_CONST) ? a.eta : nullptr;
+    kp.eta64 = (vti_like && a.eta_mode != PSTM_ETA_CONST) ? a.eta : nullptr;
     kp.eta_pstride = a.eta_mode == PSTM_ETA_3D ? nt : 0;
     kp.tt_req = a.traveltime;
     // u >= 0 can only fail or be decided differently from f64 when t can be
     // at or below t_start: t_start > 0, a curved ray with k < 0, or VTI
     // VTI returns t = t0 for t0 <= 0 (negative output times give u < 0)
-    const bool lo_check = a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0) ||
-                          tt == PSTM_TT_VTI;
+    const bool lo_check = a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0) || vti_like;
     kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
     kp.lo_eps = lo_check ? kp.eps_u : -INFINITY;
     kp.umax_lo = kp.umax - kp.eps_u;
@@ -1054,12 +1301,24 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     kp.cv0 = a.curved_v0;
     kp.ck = a.curved_k;
     kp.eta_const = a.eta_constant;
+    kp.mlpA = learned ? ws + o_mlpA : nullptr;
+    kp.mlpB = learned ? ws + o_mlpB : nullptr;
+    kp.lat_off = (int)(kp.tot_off + tot_bytes);
+    kp.lat_nodes = lat_nodes;
+    kp.lat_shift = learned ? __builtin_ctz((unsigned)a.mlp_node_spacing) : 0;
+    kp.lat_ifrac = learned ? 1.0f / (float)a.mlp_node_spacing : 0.0f;
+    kp.m_tau_sc = learned ? (float)(2.0 / a.mlp_tau_max) : 0.0f;
+    kp.m_v_lo = (float)a.mlp_v_lo;
+    kp.m_v_sc = learned ? (float)(2.0 / (a.mlp_v_hi - a.mlp_v_lo)) : 0.0f;
+    kp.m_eta_sc = learned ? (float)(2.0 / a.mlp_eta_max) : 0.0f;
+    kp.m_inv_out = learned ? (float)(1.0 / a.mlp_out_scale) : 0.0f;
 
     const bool count = a.n_effective != nullptr;
     if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, count, lo_check, kp, grid, lds, st);
     else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, count, lo_check, kp, grid, lds, st);
     else if (tt == PSTM_TT_CURVED) rc = dispatch_mw<PSTM_TT_CURVED, false>(mw, count, lo_check, kp, grid, lds, st);
-    else rc = dispatch_mw<PSTM_TT_VTI, false>(mw, count, lo_check, kp, grid, lds, st);
+    else if (tt == PSTM_TT_VTI) rc = dispatch_mw<PSTM_TT_VTI, false>(mw, count, lo_check, kp, grid, lds, st);
+    else rc = dispatch_mw<PSTM_TT_VTI_LEARNED, false>(mw, count, lo_check, kp, grid, lds, st);
     if (rc) return rc;
     if (a.n_effective) {
         hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(256), 0, st, (const unsigned long long*)cnt, grid,
@@ -1168,6 +1427,28 @@ int pstm_hip_fold_tile(const PstmMigrateArgs* args, void* stream) {
     if (rc) return rc;
     if (args->n_traces == 0) return 0;
     hipLaunchKernelGGL(k_fold, dim3(args->nx * args->ny), dim3(256), 0, static_cast<hipStream_t>(stream), *args);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int pstm_mlp_forward_mfma(const float* params, const float* x, int n, float* y, void* stream) {
+    g_last_error.clear();
+    if (!params || !x || !y || n < 0) return fail(1, "pstm_mlp_forward_mfma: bad arguments");
+    if (n == 0) return 0;
+    int rc = ensure_init();
+    if (rc) return rc;
+    rc = ensure_workspace((size_t)kANum * 8 + (size_t)kBNum * 64);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(g_state.ws);
+    const int np = kANum * 4 + kBNum * 16;
+    hipLaunchKernelGGL(k_prep_mlp, dim3((np + 255) / 256), dim3(256), 0, st, params,
+                       reinterpret_cast<unsigned short*>(ws), reinterpret_cast<float*>(ws + (size_t)kANum * 8));
+    HIP_TRY(hipGetLastError());
+    const long waves = ((long)n + 31) / 32;
+    hipLaunchKernelGGL(k_mlp_forward_mfma, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st,
+                       reinterpret_cast<const bf16x4_t*>(ws),
+                       reinterpret_cast<const f32x16_t*>(ws + (size_t)kANum * 8), x, n, y);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -1,9 +1,10 @@
 // pstm_mlp.hip — traveltime surrogate: a 5-64-64-64-64-2 SiLU MLP trained and
 // evaluated by hand-written gfx950 kernels.
 //
-// The network maps normalised (tau, d_s, d_r, v, eta) to the anisotropic
-// correction of each one-way leg, rho = t_leg / sqrt(tau^2 + d^2/v^2) - 1, so
-// that t_leg = t_hyp (1 + rho).  The exact targets are the one-way VTI times of
+// The network maps the encoded (tau, s_s, s_r, v, eta) of a (pillar, trace,
+// t0) triple (s = sine of a leg's straight-ray angle) to out_scale times the
+// anisotropic factor c of each one-way leg, t_leg^2 = tau^2 + (d/v)^2 (1 + c).
+// The exact targets come from the one-way VTI times of
 // pstm/algorithm/anisotropy_vti.py:181-220 evaluated in f64 (k_vti_targets).
 //
 // Parameters live in one f32 buffer (layout below).  Training is f32:
@@ -52,8 +53,28 @@ int fail(int code, const std::string& m) {
 
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
+// f32 -> nearest bf16 (ties to even), returned as f32
+__device__ __forceinline__ float bf16r(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+// a carried as a pair of bf16 values, hi + lo (about 16 significant bits)
+__device__ __forceinline__ float bf16pair(float a) {
+    const float hi = bf16r(a);
+    return hi + bf16r(a - hi);
+}
+
+// Precision of the forward pass.  Q = false: f32.  Q = true: the numerics of
+// the migration kernel's fused inference, i.e. weights rounded to bf16, the
+// network input and every hidden activation carried as a bf16 pair (hi + lo,
+// two MFMA products per layer), f32 accumulation and biases.  Training with
+// Q = true is quantisation-aware (straight-through gradients).
+bool g_q = false;
+
 // Z[s, o] = b[o] + sum_i X[s, i] W[o, i];  A = silu(Z) (or Z for the output layer)
-template <bool ACT>
+template <bool ACT, bool Q>
 __global__ void k_linear_fwd(const float* __restrict__ P, int l, const float* __restrict__ X, int n,
                              float* __restrict__ Z, float* __restrict__ A) {
     const int in = layer_in(l), out = layer_out(l);
@@ -63,9 +84,21 @@ __global__ void k_linear_fwd(const float* __restrict__ P, int l, const float* __
     const float* W = P + layer_w_off(l) + o * in;
     const float* x = X + (size_t)s * in;
     float z = P[layer_b_off(l) + o];
-    for (int i = 0; i < in; ++i) z = fmaf(x[i], W[i], z);
+    for (int i = 0; i < in; ++i) {
+        const float xv = (Q && l == 0) ? bf16pair(x[i]) : x[i];
+        z = fmaf(xv, Q ? bf16r(W[i]) : W[i], z);
+    }
     if (Z) Z[e] = z;
-    A[e] = ACT ? z * sigmoidf_(z) : z;
+    const float a = ACT ? z * sigmoidf_(z) : z;
+    A[e] = (Q && ACT) ? bf16pair(a) : a;
+}
+
+template <bool ACT>
+void launch_fwd(unsigned nb, hipStream_t st, const float* P, int l, const float* X, int n, float* Z, float* A) {
+    if (g_q)
+        hipLaunchKernelGGL((k_linear_fwd<ACT, true>), dim3(nb), dim3(256), 0, st, P, l, X, n, Z, A);
+    else
+        hipLaunchKernelGGL((k_linear_fwd<ACT, false>), dim3(nb), dim3(256), 0, st, P, l, X, n, Z, A);
 }
 
 // dZ = dA * silu'(Z) (in place in dA), output layer: dZ = dA
@@ -77,7 +110,8 @@ __global__ void k_silu_bwd(const float* __restrict__ Z, float* __restrict__ dA, 
     dA[e] *= sg * (1.0f + z * (1.0f - sg));
 }
 
-// dX[s, i] = sum_o dZ[s, o] W[o, i]
+// dX[s, i] = sum_o dZ[s, o] W[o, i]  (W rounded to bf16 when Q)
+template <bool Q>
 __global__ void k_linear_bwd_x(const float* __restrict__ P, int l, const float* __restrict__ dZ, int n,
                                float* __restrict__ dX) {
     const int in = layer_in(l), out = layer_out(l);
@@ -87,7 +121,7 @@ __global__ void k_linear_bwd_x(const float* __restrict__ P, int l, const float* 
     const float* W = P + layer_w_off(l);
     const float* dz = dZ + (size_t)s * out;
     float acc = 0.0f;
-    for (int o = 0; o < out; ++o) acc = fmaf(dz[o], W[o * in + i], acc);
+    for (int o = 0; o < out; ++o) acc = fmaf(dz[o], Q ? bf16r(W[o * in + i]) : W[o * in + i], acc);
     dX[e] = acc;
 }
 
@@ -211,9 +245,9 @@ int forward_saved(const float* P, const float* X, int n, float* const* Zs, float
     for (int l = 0; l < kLayers; ++l) {
         const long cnt = (long)n * layer_out(l);
         if (l < kLayers - 1)
-            hipLaunchKernelGGL(k_linear_fwd<true>, dim3(nblk(cnt)), dim3(256), 0, st, P, l, in, n, Zs[l], As[l]);
+            launch_fwd<true>(nblk(cnt), st, P, l, in, n, Zs[l], As[l]);
         else
-            hipLaunchKernelGGL(k_linear_fwd<false>, dim3(nblk(cnt)), dim3(256), 0, st, P, l, in, n, Zs[l], As[l]);
+            launch_fwd<false>(nblk(cnt), st, P, l, in, n, Zs[l], As[l]);
         HIP_TRY(hipGetLastError());
         in = As[l];
     }
@@ -257,8 +291,12 @@ int loss_grad(const float* P, const float* X, const float* T, int n, float* G, f
                            n, G);
         HIP_TRY(hipGetLastError());
         if (l > 0) {
-            hipLaunchKernelGGL(k_linear_bwd_x, dim3(nblk((long)n * layer_in(l))), dim3(256), 0, st, P, l, dA, n,
-                               dX);
+            if (g_q)
+                hipLaunchKernelGGL(k_linear_bwd_x<true>, dim3(nblk((long)n * layer_in(l))), dim3(256), 0, st, P, l,
+                                   dA, n, dX);
+            else
+                hipLaunchKernelGGL(k_linear_bwd_x<false>, dim3(nblk((long)n * layer_in(l))), dim3(256), 0, st, P, l,
+                                   dA, n, dX);
             HIP_TRY(hipGetLastError());
             float* tmp = dA;
             dA = dX;
@@ -273,6 +311,13 @@ int loss_grad(const float* P, const float* X, const float* T, int n, float* G, f
 extern "C" {
 
 int pstm_mlp_param_count(void) { return kParams; }
+
+int pstm_mlp_set_precision(int bf16) {
+    g_err.clear();
+    if (bf16 != 0 && bf16 != 1) return fail(1, "pstm_mlp_set_precision: expected 0 (f32) or 1 (bf16)");
+    g_q = bf16 == 1;
+    return 0;
+}
 
 const char* pstm_mlp_last_error(void) { return g_err.c_str(); }
 
@@ -291,11 +336,9 @@ int pstm_mlp_forward(const float* params, const float* x, int n, float* y, void*
         float* out = l == kLayers - 1 ? y : (l % 2 == 0 ? a0 : a1);
         const long cnt = (long)n * layer_out(l);
         if (l < kLayers - 1)
-            hipLaunchKernelGGL(k_linear_fwd<true>, dim3(nblk(cnt)), dim3(256), 0, st, params, l, in, n,
-                               (float*)nullptr, out);
+            launch_fwd<true>(nblk(cnt), st, params, l, in, n, nullptr, out);
         else
-            hipLaunchKernelGGL(k_linear_fwd<false>, dim3(nblk(cnt)), dim3(256), 0, st, params, l, in, n,
-                               (float*)nullptr, out);
+            launch_fwd<false>(nblk(cnt), st, params, l, in, n, nullptr, out);
         HIP_TRY(hipGetLastError());
         in = out;
     }

@@ -13,9 +13,9 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
-TT_STRAIGHT, TT_CURVED, TT_VTI = 0, 1, 2
+TT_STRAIGHT, TT_CURVED, TT_VTI, TT_VTI_LEARNED = 0, 1, 2, 3
 AA_NONE, AA_TRIANGLE = 0, 1
 VEL_1D, VEL_3D = 0, 1
 ETA_CONST, ETA_1D, ETA_3D = 0, 1, 2
@@ -48,6 +48,9 @@ class PstmMigrateArgs(ctypes.Structure):
         ("grid_dx", ctypes.c_double), ("grid_dy", ctypes.c_double),
         ("curved_v0", ctypes.c_double), ("curved_k", ctypes.c_double),
         ("eta_constant", ctypes.c_double),
+        ("mlp_params", _vp), ("mlp_node_spacing", ctypes.c_int32), ("mlp_reserved", ctypes.c_int32),
+        ("mlp_tau_max", ctypes.c_double), ("mlp_v_lo", ctypes.c_double), ("mlp_v_hi", ctypes.c_double),
+        ("mlp_eta_max", ctypes.c_double), ("mlp_out_scale", ctypes.c_double),
     ]
 
 
@@ -58,7 +61,8 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
     "pstm_hip_set_tuning", "pstm_hip_last_launch",
-    "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_forward", "pstm_mlp_loss_grad",
+    "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_set_precision", "pstm_mlp_forward",
+    "pstm_mlp_forward_mfma", "pstm_mlp_loss_grad",
     "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_vti_leg_times",
 )
 
@@ -108,8 +112,12 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_mlp_param_count.argtypes = []
     lib.pstm_mlp_last_error.restype = ctypes.c_char_p
     lib.pstm_mlp_last_error.argtypes = []
+    lib.pstm_mlp_set_precision.restype = ctypes.c_int
+    lib.pstm_mlp_set_precision.argtypes = [ctypes.c_int]
     lib.pstm_mlp_forward.restype = ctypes.c_int
     lib.pstm_mlp_forward.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
+    lib.pstm_mlp_forward_mfma.restype = ctypes.c_int
+    lib.pstm_mlp_forward_mfma.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
     lib.pstm_mlp_loss_grad.restype = ctypes.c_int
     lib.pstm_mlp_loss_grad.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
     lib.pstm_mlp_adam.restype = ctypes.c_int

@@ -163,12 +163,16 @@ class HipKernel:
     def migrate_device(self, dt: DeviceTraces, image, fold2d, x_axis, y_axis, t_axis_ms,
                        vrms, config: KernelConfig, *, traveltime: int | None = None,
                        trace_index=None, eta=None, count_effective: bool = False,
-                       group_start: np.ndarray | None = None) -> int | None:
+                       group_start: np.ndarray | None = None, surrogate=None,
+                       node_spacing: int = 16) -> int | None:
         """Accumulate one tile into device tensors ``image`` (nx, ny, nt) f32
         and ``fold2d`` (nx, ny) i32.  All array arguments are device tensors
         (vrms (nt,) or (nx, ny, nt) f64).  Asynchronous on the current stream
         unless ``count_effective`` is set, in which case the effective
-        contribution count E is returned."""
+        contribution count E is returned.  ``traveltime=TT_VTI_LEARNED`` needs
+        ``surrogate`` (a trained pm_amd.surrogate.TraveltimeSurrogate) and
+        evaluates it on a t0 lattice of ``node_spacing`` output samples inside
+        the gather."""
         self._require_init()
         torch = _torch()
         lib = self._lib
@@ -222,6 +226,15 @@ class HipKernel:
         a.grid_dx, a.grid_dy = float(config.grid_dx), float(config.grid_dy)
         a.curved_v0, a.curved_k = float(config.curved_ray_v0), float(config.curved_ray_k)
         a.eta_constant = float(config.vti_eta_constant)
+        if tt == _abi.TT_VTI_LEARNED:
+            if surrogate is None:
+                raise ValueError("the learned traveltime needs a trained surrogate")
+            n = surrogate.norm
+            keep.append(surrogate.params)
+            a.mlp_params = _ptr(surrogate.params)
+            a.mlp_node_spacing = int(node_spacing)
+            a.mlp_tau_max, a.mlp_v_lo, a.mlp_v_hi = float(n.tau_max), float(n.v_lo), float(n.v_hi)
+            a.mlp_eta_max, a.mlp_out_scale = float(n.eta_max), float(n.out_scale)
         stream = ctypes.c_void_p(torch.cuda.current_stream(self._device_index).cuda_stream)
         if group_start is None:
             rc = lib.pstm_hip_migrate_tile(ctypes.byref(a), stream)

@@ -38,6 +38,9 @@ class DeviceTile:
     rows: Any            # (n,) int32 trace rows in the aperture-padded rectangle
     image: Any           # (tnx, tny, nt) f32 (slab of the tile-major image)
     fold: Any            # (tnx, tny) i32
+    eta: Any = None      # VTI: (tnx, tny, nt) f64, or None (constant eta)
+    learned_ok: bool = False   # surrogate passed this tile's held-out check
+    heldout_error: float | None = None   # max held-out traveltime error (samples)
 
     @property
     def n_pillars(self) -> int:
@@ -80,6 +83,8 @@ class TileExecutor:
         else:
             vfull = None
             v1 = torch.as_tensor(job.vrms_1d, dtype=f64, device=dev)
+        ecube = job.meta.get("eta_cube") if self.config.vti_enabled else None
+        efull = ecube.resample(job.x_axis, job.y_axis, job.t_axis_ms, device=dev) if ecube is not None else None
         tiles = job.tiles()
         slab = sum((x1 - x0) * (y1 - y0) for x0, x1, y0, y1 in tiles) * self.nt
         self.image_storage = torch.zeros(slab, dtype=torch.float32, device=dev)
@@ -97,24 +102,85 @@ class TileExecutor:
             self.tiles.append(DeviceTile(
                 x0, x1, y0, y1, torch.as_tensor(job.x_axis[x0:x1], dtype=f64, device=dev),
                 torch.as_tensor(job.y_axis[y0:y1], dtype=f64, device=dev), v, rows, img,
-                torch.zeros((tnx, tny), dtype=torch.int32, device=dev)))
-        del vfull
+                torch.zeros((tnx, tny), dtype=torch.int32, device=dev),
+                efull[x0:x1, y0:y1].contiguous() if efull is not None else None))
+        del vfull, efull
 
     def reset(self) -> None:
         self.image_storage.zero_()
         for t in self.tiles:
             t.fold.zero_()
 
-    def run(self, count_effective: bool = False) -> int | None:
-        """One pass over every tile (asynchronous unless counting)."""
+    def run(self, count_effective: bool = False, surrogate=None, node_spacing: int = 16,
+            tiles: list[int] | None = None) -> int | None:
+        """One pass over every tile (or the listed ones), asynchronous unless
+        counting.  With a surrogate (VTI jobs), tiles whose held-out check
+        passed (``check_surrogate``) use the learned traveltime and the others
+        the exact one."""
+        from pm_amd.kernels import _abi
+
         total = 0
-        for t in self.tiles:
+        for i in (range(len(self.tiles)) if tiles is None else tiles):
+            t = self.tiles[i]
+            tt = None
+            if surrogate is not None and t.learned_ok:
+                tt = _abi.TT_VTI_LEARNED
             e = self.kernel.migrate_device(self.traces, t.image, t.fold, t.x_axis, t.y_axis,
                                            self.t_axis, t.vrms, self.config, trace_index=t.rows,
-                                           count_effective=count_effective)
+                                           count_effective=count_effective, eta=t.eta, traveltime=tt,
+                                           surrogate=surrogate, node_spacing=node_spacing)
             if count_effective:
                 total += int(e)
         return total if count_effective else None
+
+    def check_surrogate(self, surrogate, node_spacing: int = 16, n: int = 8192, bar: float = 0.25,
+                        seed: int = 0, tiles: list[int] | None = None) -> list[float]:
+        """Per-tile held-out check of the learned traveltime: n random
+        (pillar, in-aperture trace, output sample) triples of the tile, the
+        learned time as the gather forms it (pm_amd.surrogate.lattice_times)
+        against the exact VTI time, in input samples.  A tile passes when the
+        maximum error is at most ``bar``; the others fall back to the exact
+        traveltime.  Returns the per-tile maxima."""
+        import torch
+
+        from pm_amd.surrogate import lattice_times, vti_leg_times
+
+        g = torch.Generator(device=self.device)
+        g.manual_seed(seed)
+        tr = self.traces
+        tau_axis = self.t_axis / 2000.0
+        dt_in = tr.sample_rate_ms / 1000.0
+        ns = tr.n_samples
+        out = []
+        for i in (range(len(self.tiles)) if tiles is None else tiles):
+            t = self.tiles[i]
+            tnx, tny = t.x1 - t.x0, t.y1 - t.y0
+            m = 4 * n
+            pil = torch.randint(0, tnx * tny, (m,), device=self.device, generator=g)
+            row = t.rows[torch.randint(0, int(t.rows.shape[0]), (m,), device=self.device, generator=g)].long()
+            k = torch.randint(0, self.nt, (m,), device=self.device, generator=g)
+            ox, oy = t.x_axis[pil // tny], t.y_axis[pil % tny]
+            keep = ((ox - tr.midpoint_x[row]) ** 2 + (oy - tr.midpoint_y[row]) ** 2).sqrt() <= self.config.max_aperture_m
+            pil, row, k, ox, oy = pil[keep][:n], row[keep][:n], k[keep][:n], ox[keep][:n], oy[keep][:n]
+            ds = ((ox - tr.source_x[row]) ** 2 + (oy - tr.source_y[row]) ** 2).sqrt()
+            dr = ((ox - tr.receiver_x[row]) ** 2 + (oy - tr.receiver_y[row]) ** 2).sqrt()
+            vflat = t.vrms.reshape(-1, self.nt) if t.vrms.dim() == 3 else None
+            eflat = t.eta.reshape(-1, self.nt) if t.eta is not None else None
+            v_at = (lambda kk: vflat[pil, kk]) if vflat is not None else (lambda kk: t.vrms[kk])
+            if eflat is not None:
+                e_at = lambda kk: eflat[pil, kk]  # noqa: E731
+            else:
+                e_at = lambda kk: torch.full(kk.shape, float(self.config.vti_eta_constant),  # noqa: E731
+                                             dtype=torch.float64, device=self.device)
+            learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing)
+            tau = tau_axis[k]
+            exact = vti_leg_times(tau, ds, v_at(k), e_at(k)) + vti_leg_times(tau, dr, v_at(k), e_at(k))
+            inside = exact / dt_in < ns - 1
+            err = float(((learned - exact).abs() / dt_in)[inside].max()) if bool(inside.any()) else 0.0
+            t.heldout_error = err
+            t.learned_ok = err <= bar
+            out.append(err)
+        return out
 
     def contributions(self) -> int:
         """C = sum of the 3-D fold = sum over pillars of fold2d * nt."""

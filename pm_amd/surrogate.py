@@ -1,17 +1,21 @@
 """Learned VTI traveltime surrogate: data, training and evaluation.
 
-The network (5 -> 64 -> 64 -> 64 -> 64 -> 2, SiLU) maps the normalised
-(tau, d_s, d_r, v_rms, eta) of a (pillar, trace, t0) triple to the anisotropic
-corrections (rho_s, rho_r) of the two one-way legs,
+The network (5 -> 64 -> 64 -> 64 -> 64 -> 2, SiLU) maps the encoded
+(tau, s_s, s_r, v_rms, eta) of a (pillar, trace, t0) triple (tau = t0 / 2,
+s = sine of a leg's straight-ray angle) to out_scale * (c_s, c_r), the
+anisotropic factor of each one-way leg:
 
-    t_leg = sqrt(tau^2 + d^2 / v^2) * (1 + rho),
+    t_leg = sqrt(tau^2 + (d / v)^2 (1 + c)),
 
 where the exact leg time is the VTI one-way time of
-pstm/algorithm/anisotropy_vti.py:181-220.  Training samples are drawn on the
-job's own v_rms and eta fields (random x, y, t0 in the model, random source and
-receiver distances up to the aperture plus half the maximum offset) and their
-targets come from an f64 HIP kernel; forward, backward and Adam are HIP
-kernels (pm_amd/csrc/pstm_mlp.hip).  Tensors are device memory only.
+pstm/algorithm/anisotropy_vti.py:181-220 (for which c = -2 eta s^2 /
+(1 + 2 eta s^2); the network learns it from the exact times).  Training
+samples are drawn on the job's own v_rms and eta fields (random x, y, t0 in
+the model, random source and receiver distances) and their targets come from
+an f64 HIP kernel; forward, backward and Adam are HIP kernels
+(pm_amd/csrc/pstm_mlp.hip), and the migration evaluates the same network
+inside its gather with BF16 MFMA (pm_amd/csrc/pstm_hip.hip).  Tensors are
+device memory only.
 """
 
 from __future__ import annotations
@@ -42,29 +46,39 @@ def _check(lib, rc: int, what: str) -> None:
 
 @dataclass
 class Normalizer:
-    """Affine maps of the five inputs to about [-1, 1]."""
+    """Input encoding of the network.
+
+    The five inputs are (tau, d_s, d_r, v, eta).  The distances enter through
+    the sine of the straight-ray angle of each leg, s = d / (v t_hyp) with
+    t_hyp = sqrt(tau^2 + d^2 / v^2) (s = 0 when t_hyp = 0): for the VTI time
+    the leg factor is a function of (s, eta) alone,
+    c = -2 eta s^2 / (1 + 2 eta s^2), while d / (v tau) itself is unbounded at
+    shallow times.  Every input is then mapped affinely to
+    about [-1, 1]."""
 
     tau_max: float
     d_max: float
     v_lo: float
     v_hi: float
     eta_max: float
+    out_scale: float = 10.0   # network outputs are out_scale * (c_s, c_r)
 
-    def scales(self) -> tuple[np.ndarray, np.ndarray]:
-        """x_norm = x * scale + offset, per input (tau, d_s, d_r, v, eta)."""
-        sc = np.array([2.0 / self.tau_max, 2.0 / self.d_max, 2.0 / self.d_max,
-                       2.0 / (self.v_hi - self.v_lo), 2.0 / max(self.eta_max, 1e-6)])
-        off = np.array([-1.0, -1.0, -1.0, -1.0 - 2.0 * self.v_lo / (self.v_hi - self.v_lo), -1.0])
-        return sc, off
-
-    def apply(self, x):
-        """x (n, 5) torch f64/f32 -> normalised f32."""
+    def encode(self, tau, d_s, d_r, v, eta):
+        """Raw inputs (torch, any float dtype) -> (n, 5) f32 network inputs."""
         import torch
 
-        sc, off = self.scales()
-        s = torch.as_tensor(sc, dtype=x.dtype, device=x.device)
-        o = torch.as_tensor(off, dtype=x.dtype, device=x.device)
-        return (x * s + o).float().contiguous()
+        def sine(d):
+            th = hyperbolic(tau, d, v)
+            return torch.where(th > 0, d / (v * th.clamp_min(1e-300)), torch.zeros_like(th))
+
+        cols = [2.0 * tau / self.tau_max - 1.0, 2.0 * sine(d_s) - 1.0, 2.0 * sine(d_r) - 1.0,
+                2.0 * (v - self.v_lo) / (self.v_hi - self.v_lo) - 1.0,
+                2.0 * eta / max(self.eta_max, 1e-6) - 1.0]
+        return torch.stack(cols, 1).float().contiguous()
+
+    def apply(self, x):
+        """x (n, 5) raw (tau, d_s, d_r, v, eta) -> encoded f32."""
+        return self.encode(x[:, 0], x[:, 1], x[:, 2], x[:, 3], x[:, 4])
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -80,6 +94,13 @@ def init_params(seed: int = 0) -> np.ndarray:
     p = np.concatenate(out).astype(np.float32)
     assert p.size == _abi.MLP_PARAMS
     return p
+
+
+def set_precision(bf16: bool) -> None:
+    """f32 (False) or the fused inference's bf16 numerics (True) for the
+    library's forward / gradient / training calls."""
+    lib = _abi.load_library()
+    _check(lib, lib.pstm_mlp_set_precision(1 if bf16 else 0), "pstm_mlp_set_precision")
 
 
 def vti_leg_times(tau, h, v, eta):
@@ -102,7 +123,7 @@ def hyperbolic(tau, d, v):
 class TrainingSet:
     x_raw: object      # (n, 5) f64: tau, d_s, d_r, v, eta
     x: object          # (n, 5) f32 normalised
-    y: object          # (n, 2) f32: rho_s, rho_r
+    y: object          # (n, 2) f32: out_scale * (c_s, c_r)
     t_exact: object    # (n, 2) f64 exact leg times
 
 
@@ -127,13 +148,28 @@ def make_training_set(vcube, ecube, n: int, d_max: float, norm: Normalizer, seed
     drt = torch.as_tensor(dr, device=device)
     t_s = vti_leg_times(tau, dst, v, e)
     t_r = vti_leg_times(tau, drt, v, e)
-    hs, hr = hyperbolic(tau, dst, v), hyperbolic(tau, drt, v)
-    ok = (hs > 0) & (hr > 0)
-    rho_s = torch.where(ok, t_s / hs.clamp_min(1e-30) - 1.0, torch.zeros_like(hs))
-    rho_r = torch.where(ok, t_r / hr.clamp_min(1e-30) - 1.0, torch.zeros_like(hr))
     xr = torch.stack([tau, dst, drt, v, e], 1)
-    return TrainingSet(xr, norm.apply(xr), torch.stack([rho_s, rho_r], 1).float().contiguous(),
-                       torch.stack([t_s, t_r], 1))
+    c = torch.stack([anisotropic_factor(t_s, tau, dst, v, e), anisotropic_factor(t_r, tau, drt, v, e)], 1)
+    y = (c * norm.out_scale).float().contiguous()
+    return TrainingSet(xr, norm.apply(xr), y, torch.stack([t_s, t_r], 1))
+
+
+def anisotropic_factor(t, tau, d, v, eta):
+    """c = (t^2 - tau^2) / (d / v)^2 - 1 from exact leg times t (f64).  At
+    tau = 0 the exact time is 0 by convention, and c takes its limit there,
+    -2 eta / (1 + 2 eta); where (d / v)^2 is below 1e-10 s^2 the time does not
+    depend on c and c = 0."""
+    import torch
+
+    hs = (d / v) ** 2
+    c = (t * t - tau * tau) / hs.clamp_min(1e-300) - 1.0
+    c = torch.where(tau > 0, c, -2.0 * eta / (1.0 + 2.0 * eta))
+    return torch.where(hs > 1e-10, c, torch.zeros_like(c))
+
+
+def learned_leg_time(tau, d, v, c):
+    """sqrt(tau^2 + (d / v)^2 max(1 + c, 0)), as the gather evaluates it."""
+    return (tau * tau + (d / v) ** 2 * (1.0 + c).clamp_min(0.0)).sqrt()
 
 
 def _trilinear(cube, x, y, t):
@@ -179,15 +215,20 @@ class TraveltimeSurrogate:
         p = torch.as_tensor(init_params(seed), device=device)
         return cls(norm, p, torch.zeros_like(p), torch.zeros_like(p), torch.zeros_like(p))
 
-    def forward(self, x):
-        """Network output (n, 2) for normalised inputs x (n, 5) f32."""
+    def forward(self, x, bf16: bool = False):
+        """Network output (n, 2) for normalised inputs x (n, 5) f32, in f32 or
+        with the fused inference's bf16 numerics."""
         import torch
 
         lib = _abi.load_library()
         x = x.float().contiguous()
         y = torch.empty((x.shape[0], 2), dtype=torch.float32, device=x.device)
-        _check(lib, lib.pstm_mlp_forward(_p(self.params), _p(x), int(x.shape[0]), _p(y), _stream()),
-               "pstm_mlp_forward")
+        set_precision(bf16)
+        try:
+            _check(lib, lib.pstm_mlp_forward(_p(self.params), _p(x), int(x.shape[0]), _p(y), _stream()),
+                   "pstm_mlp_forward")
+        finally:
+            set_precision(False)
         return y
 
     def loss_grad(self, x, y):
@@ -200,11 +241,14 @@ class TraveltimeSurrogate:
         return loss, self.grads
 
     def train(self, ts: TrainingSet, steps: int, batch: int = 8192, lr: float = 2e-3,
-              lr_final: float = 5e-5, seed: int = 0, log_every: int = 0):
-        """Adam with a cosine-decayed learning rate over `steps` minibatches."""
+              lr_final: float = 5e-5, seed: int = 0, log_every: int = 0, qat_fraction: float = 0.0):
+        """Adam with a cosine-decayed learning rate over `steps` minibatches;
+        the last qat_fraction of the steps train through the bf16 numerics of
+        the fused inference (quantisation-aware)."""
         import torch
 
         lib = _abi.load_library()
+        q_from = steps - int(round(qat_fraction * steps))
         n = int(ts.x.shape[0])
         g = torch.Generator(device=ts.x.device)
         g.manual_seed(seed)
@@ -215,25 +259,28 @@ class TraveltimeSurrogate:
             yb = ts.y.index_select(0, idx).contiguous()
             self.step += 1
             a = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * i / max(steps - 1, 1)))
-            _check(lib, lib.pstm_mlp_train_step(_p(self.params), _p(self.m), _p(self.v), _p(self.grads),
-                                                _p(xb), _p(yb), batch, float(a), 0.9, 0.999, 1e-8,
-                                                self.step, _p(loss), _stream()), "pstm_mlp_train_step")
+            set_precision(i >= q_from)
+            try:
+                _check(lib, lib.pstm_mlp_train_step(_p(self.params), _p(self.m), _p(self.v), _p(self.grads),
+                                                    _p(xb), _p(yb), batch, float(a), 0.9, 0.999, 1e-8,
+                                                    self.step, _p(loss), _stream()), "pstm_mlp_train_step")
+            finally:
+                set_precision(False)
             if log_every and (i % log_every == 0 or i == steps - 1):
                 self.history.append((i, float(loss.item())))
         return self
 
-    def leg_times(self, ts: TrainingSet):
+    def leg_times(self, ts: TrainingSet, bf16: bool = False):
         """Predicted (t_s, t_r) f64 for a sample set."""
-        rho = self.forward(ts.x).double()
-        tau, ds, dr, v = ts.x_raw[:, 0], ts.x_raw[:, 1], ts.x_raw[:, 2], ts.x_raw[:, 3]
         import torch
 
-        return torch.stack([hyperbolic(tau, ds, v) * (1 + rho[:, 0]),
-                            hyperbolic(tau, dr, v) * (1 + rho[:, 1])], 1)
+        c = self.forward(ts.x, bf16).double() / self.norm.out_scale
+        tau, ds, dr, v = ts.x_raw[:, 0], ts.x_raw[:, 1], ts.x_raw[:, 2], ts.x_raw[:, 3]
+        return torch.stack([learned_leg_time(tau, ds, v, c[:, 0]), learned_leg_time(tau, dr, v, c[:, 1])], 1)
 
-    def error_samples(self, ts: TrainingSet, dt_ms: float) -> dict:
+    def error_samples(self, ts: TrainingSet, dt_ms: float, bf16: bool = False) -> dict:
         """Error of t_s + t_r against the exact sum, in input samples."""
-        pred = self.leg_times(ts).sum(1)
+        pred = self.leg_times(ts, bf16).sum(1)
         exact = ts.t_exact.sum(1)
         err = (pred - exact).abs() / (dt_ms / 1000.0)
         return {"max": float(err.max()), "p99": float(err.quantile(0.99)) if err.numel() <= 16_000_000 else None,
@@ -244,3 +291,35 @@ class TraveltimeSurrogate:
         import torch
 
         return self.params.to(torch.bfloat16)
+
+
+def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, node_spacing: int):
+    """Learned two-leg time of output samples k as the gather forms it: the
+    network (bf16 numerics) on the lattice samples k0 = K floor(k / K) and
+    k0 + K (times past the axis extrapolated from its last step, v and eta
+    taken at the last sample), c interpolated linearly, then
+    sum of sqrt(tau^2 + (d / v)^2 max(1 + c, 0)); t = t0 where t0 <= 0.
+
+    tau_axis: (nt,) one-way times t0 / 2 (f64); k: (n,) int64 sample indices;
+    ds, dr: (n,) leg distances; v_at(kk), eta_at(kk): (n,) f64 values at the
+    sample's own pillar and samples kk."""
+    import torch
+
+    nt = int(tau_axis.shape[0])
+    K = int(node_spacing)
+    dtau = (tau_axis[-1] - tau_axis[-2]) if nt > 1 else tau_axis.new_tensor(0.0)
+    k0 = torch.div(k, K, rounding_mode="floor") * K
+    frac = ((k - k0).double() / K)[:, None]
+
+    def at_node(kn):
+        kk = kn.clamp(max=nt - 1)
+        tau = tau_axis[kk] + (kn - kk).double() * dtau
+        x = sur.norm.encode(tau, ds, dr, v_at(kk), eta_at(kk))
+        return sur.forward(x, bf16=True).double() / sur.norm.out_scale
+
+    c0, c1 = at_node(k0), at_node(k0 + K)
+    c = c0 + frac * (c1 - c0)
+    tau = tau_axis[k]
+    v = v_at(k)
+    t = learned_leg_time(tau, ds, v, c[:, 0]) + learned_leg_time(tau, dr, v, c[:, 1])
+    return torch.where(tau > 0, t, 2.0 * tau)

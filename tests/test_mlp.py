@@ -1,7 +1,9 @@
 """Traveltime surrogate: the FP64 NumPy network (oracle/mlp.py) pinned by
-finite differences and the closed-form Adam update; the HIP forward, gradient
-and Adam kernels checked against it on the GPU; exact VTI targets against the
-oracle's golden-pinned VTI time; training reaches the 0.25-sample bar."""
+finite differences and the closed-form Adam update, its bf16 rounding pinned
+against torch's; the HIP forward, gradient and Adam kernels and the fused
+path's BF16 MFMA inference checked against it on the GPU; exact VTI targets
+against the oracle's golden-pinned VTI time; training reaches the
+0.25-sample bar under the fused inference's numerics."""
 
 from __future__ import annotations
 
@@ -22,6 +24,36 @@ def _rand_params(seed=0):
 
 def test_layout_matches_header():
     assert sum(fi * fo + fo for fi, fo in omlp.SIZES) == _abi.MLP_PARAMS == 12994
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.standard_normal(20000) * 10.0 ** rng.integers(-6, 6, 20000),
+                        [0.0, -0.0, 1.0, 1.00390625, 1.005859375, -3.0]]).astype(np.float32)
+    ref = torch.as_tensor(x).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(omlp.bf16_round(x), ref)
+
+
+def test_bf16_pair_carries_sixteen_bits():
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-8, 8, 10000).astype(np.float32)
+    err = np.abs(omlp.bf16_pair(x) - x.astype(np.float64))
+    assert np.all(err <= 2.0 ** -16 * np.abs(x.astype(np.float64)))
+
+
+def test_bf16_forward_close_to_f64():
+    """The bf16 numerics stay near the f64 network (weights rounded: a few
+    1e-3 relative), and equal it for weights that are bf16 already."""
+    rng = np.random.default_rng(13)
+    p = _rand_params(13)
+    x = rng.uniform(-1, 1, (256, 5))
+    y64, yb = omlp.forward(p, x), omlp.forward_bf16(p, x)
+    assert np.max(np.abs(yb - y64)) < 0.05 * np.max(np.abs(y64))
+    pb = omlp.bf16_round(p.astype(np.float32))
+    xb = omlp.bf16_round(x.astype(np.float32))
+    np.testing.assert_allclose(omlp.forward_bf16(pb, xb), omlp.forward(pb, xb), rtol=2e-4, atol=1e-5)
 
 
 def test_numpy_gradient_matches_finite_differences():
@@ -109,18 +141,50 @@ class TestHipSurrogate:
         ref = np.array([orc.vti_leg(*a) for a in zip(tau, h, v, e)])
         np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-15)
 
+    def test_bf16_mode_and_mfma_match_oracle(self):
+        """pstm_mlp_forward in bf16 mode and the fused path's MFMA inference
+        (pstm_mlp_forward_mfma) against the bf16 NumPy forward; n = 1000 is
+        not a multiple of the 32-row MFMA tile."""
+        import torch
+
+        from pm_amd.surrogate import _stream
+
+        lib = _abi.load_library()
+        rng = np.random.default_rng(14)
+        p = _rand_params(14).astype(np.float32)
+        x = rng.uniform(-1, 1, (1000, 5)).astype(np.float32)
+        ref = omlp.forward_bf16(p, x)
+        pd, xd = torch.as_tensor(p, device="cuda"), torch.as_tensor(x, device="cuda")
+        scale = np.max(np.abs(ref))
+        for mode in ("bf16", "mfma"):
+            y = torch.full((1000, 2), float("nan"), device="cuda")
+            if mode == "bf16":
+                assert lib.pstm_mlp_set_precision(1) == 0
+                try:
+                    assert lib.pstm_mlp_forward(pd.data_ptr(), xd.data_ptr(), 1000, y.data_ptr(), _stream()) == 0
+                finally:
+                    lib.pstm_mlp_set_precision(0)
+            else:
+                assert lib.pstm_mlp_forward_mfma(pd.data_ptr(), xd.data_ptr(), 1000, y.data_ptr(), _stream()) == 0
+            got = y.cpu().numpy()
+            err = np.max(np.abs(got - ref)) / scale
+            print(mode, "max abs error / max |y|", err)
+            assert err < 2e-5, mode
+
     def test_training_reaches_quarter_sample(self):
-        """A small run on the VTI model converges below 0.25 sample on held-out
-        samples (dt 2 ms)."""
+        """Training on the VTI model (the last quarter quantisation-aware)
+        reaches held-out errors below 0.25 sample (dt 2 ms) under the fused
+        inference's bf16 numerics."""
         from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set
         from pm_amd.synthetic import vti_model
 
         vc, ec = vti_model(n_xy=16, n_t=51, x_hi=2000.0, t_hi_ms=2000.0)
         norm = Normalizer(tau_max=1.0, d_max=2000.0, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
-        tr = make_training_set(vc, ec, 400_000, 2000.0, norm, seed=1)
-        ho = make_training_set(vc, ec, 20_000, 2000.0, norm, seed=2)
-        s = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=3000, batch=4096, log_every=500)
-        err = s.error_samples(ho, 2.0)
-        print("loss history", s.history, "held-out error (samples)", err)
+        tr = make_training_set(vc, ec, 1_000_000, 2000.0, norm, seed=1)
+        ho = make_training_set(vc, ec, 50_000, 2000.0, norm, seed=2)
+        s = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=20000, batch=8192, log_every=2000,
+                                                           qat_fraction=0.25)
+        err = s.error_samples(ho, 2.0, bf16=True)
+        print("loss history", s.history, "held-out error (samples, bf16)", err)
         assert s.history[-1][1] < s.history[0][1] * 1e-2
         assert err["max"] < 0.25
