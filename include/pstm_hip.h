@@ -181,6 +181,48 @@ int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_star
                           int32_t n_groups, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Plans: many output tiles (and common-offset bins: a tile's trace list may
+ * hold the rows of every bin) in one persistent, hipGraph-captured gather.
+ * Replaces the executor's host loop over tiles and bins
+ * (pstm/pipeline/executor.py:977-1358 and 1397-1858, dispatch at 1691-1745).
+ * ------------------------------------------------------------------------- */
+typedef struct PstmTileDesc {
+    const double* x_axis;         /* (nx,)                                 */
+    const double* y_axis;         /* (ny,)                                 */
+    const double* vrms;           /* (nt,) or (nx, ny, nt) per vel_mode    */
+    const double* eta;            /* NULL, (nt,) or (nx, ny, nt)           */
+    float* image;                 /* (nx, ny, nt) f32, +=                  */
+    int32_t* fold;                /* (nx, ny) pillar fold, +=              */
+    const int32_t* trace_index;   /* (n_traces,) rows of the block         */
+    int32_t n_traces;
+    int32_t nx, ny;
+    int32_t reserved;             /* 0                                     */
+} PstmTileDesc;
+
+typedef struct PstmPlan PstmPlan;
+
+/* Build a plan: `common` holds the trace block (n_traces = its row count),
+ * the output time axis and every KernelConfig field; its x_axis, y_axis,
+ * vrms, eta, image, fold and trace_index are ignored (per-tile values come
+ * from `tiles`).  Geometry, velocity and fold terms are computed here, once;
+ * amplitudes and (for the learned traveltime) mlp_params are read at every
+ * run.  If common->n_effective is set, every run adds E to it. */
+int pstm_hip_plan_create(const PstmMigrateArgs* common, const PstmTileDesc* tiles, int32_t n_tiles,
+                         PstmPlan** plan);
+
+/* One migration of every tile of the plan on `stream` (image and fold +=).
+ * use_graph = 1: the first call captures the run's launch sequence into a
+ * hipGraph, later calls replay it; 0: launch directly.  Runs of one plan on
+ * one device must not overlap. */
+int pstm_hip_plan_run(PstmPlan* plan, int32_t use_graph, void* stream);
+
+/* Work items (tile x pillar group x time window), persistent workgroups and
+ * the launch shape of a plan. */
+int pstm_hip_plan_info(const PstmPlan* plan, long long* n_items, int* grid, int* mw, int* batch);
+
+int pstm_hip_plan_destroy(PstmPlan* plan);
+
+/* ---------------------------------------------------------------------------
  * Traveltime surrogate (no counterpart in the reference: the learned part of
  * this backend).  A 5-64-64-64-64-2 SiLU MLP; inputs are the encoded
  * (tau, s_s, s_r, v, eta) of PstmMigrateArgs.mlp_*, outputs out_scale * c per

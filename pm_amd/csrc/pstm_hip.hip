@@ -72,27 +72,46 @@ constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS use
 //   y : weight constant  (spreading ? 1/v : 1) * (obliquity ? t0 : 1)
 //   z : taper slope  1/(ap - ts)              (ts = ap (1 - taper))
 //   w : taper offset -ts/(ap - ts)  => x_t = d*z + w, in-aperture x_t <= 1
-struct KParams {
-    const float* amp;            // (rows, ns)
-    const int32_t* tidx;         // (ntr) or null
-    const float4* geo;           // (ntr, 2): {sx, sy, rx, ry}, {mx, my, w, 0}
-    const double* prow;          // AA: (ntr, rowlen) double-integrated rows
-    const float4* kc;            // (npv, nt)
-    const float* qk;             // (nt) tau^2 (curved: v(z)^2)
+// Per-tile view of the gather's inputs and outputs, in a device table: one
+// entry for a single-tile call, one per output tile for a plan
+// (pstm_hip_plan_*); the gather looks each work item's tile up.
+struct TileView {
+    const double* xax;           // (nx) f64 pillar x
+    const double* yax;           // (ny) f64 pillar y
+    const float4* kc;            // (npv, nt) per-(pillar, sample) terms
     const float2* ek;            // VTI (nev, nt): {2 eta, 1 + 2 eta}
+    const double* vrms64;        // (npv, nt) f64 velocity (exact decisions)
+    const double* eta64;         // VTI: null (constant), (nt) or (npil, nt)
+    float* image;                // (nx, ny, nt) f32, +=
+    const float4* geo;           // (ntr, 2): {sx, sy, rx, ry}, {mx, my, w, 0} tile-relative f32
+    const int32_t* tidx;         // (ntr) rows of the trace block, or null
+    int32_t* fold;               // (nx, ny) i32 (plans: fold += fold_cnt every run)
+    const int32_t* fold_cnt;     // plans: (nx, ny) fold counts taken when the plan was built
+    int ntr, nx, ny, kc_pstride;
+    int ek_pstride, eta_pstride, n_pgroups, item0;  // item0: first work item of the tile
+};
+
+// Parameters of one gather launch.
+//   kc.x : traveltime slowness^2 s = 1/v^2       (curved ray: v(z) at depth)
+//   kc.y : weight constant  (spreading ? 1/v : 1) * (obliquity ? t0 : 1)
+//   kc.z : taper slope  1/(ap tf)                => x_t = d * slope + taper_off
+struct KParams {
+    const TileView* tiles;       // (n_tiles) device table, ascending item0
+    int n_tiles;
+    unsigned* queue;             // work counter (zeroed before each launch)
+    const float* amp;            // (rows, ns)
+    const double* prow;          // AA: double-integrated rows
+    int prow_global;             // AA rows indexed by block row (plans) or by list position
+    const float* qk;             // (nt) tau^2 (curved: v(z)^2)
     const float* t0k;            // (nt) t0 seconds
-    const double* xax;
-    const double* yax;
-    float* image;                // (nx, ny, nt)
     unsigned long long* cnt_part;  // (gridDim.x) or null
-    int ns, ntr, nx, ny, nt;
-    int kc_pstride, ek_pstride;  // nt or 0
+    int ns, nt;
     int rowlen, lo;              // AA row length / index offset
     int batch;                   // traces per LDS batch
     int pair_off;                // byte offset of the pair table in LDS
     int tot_off;                 // byte offset of the f64 running totals in LDS
     float taper_off;             // x_t = d * slope + taper_off (taper_off = -(1-tf)/tf)
-    int n_items, n_pgroups;      // work items = pillar groups x time windows
+    int n_items;                 // work items over all tiles (pillar groups x time windows)
     int wpow;                    // 0, 1 or 2: power of 1/t in the weight
     float idt, uoff, umax;       // u = t*idt + uoff, valid u in [0, umax)
     float umax_lo, umax_hi;      // umax -+ eps_u: band decided in f64
@@ -108,10 +127,7 @@ struct KParams {
     const double* ry64;
     const double* mx64;
     const double* my64;
-    const double* vrms64;        // (npv, nt)
     const double* tax64;         // (nt) ms
-    const double* eta64;         // VTI: null (constant), (nt) or (npil, nt)
-    int eta_pstride;             // 0 or nt (eta64 per pillar)
     int tt_req;                  // traveltime asked for (PSTM_TT_*)
     float lo_eps;                // eps_u where u >= 0 can be decided differently from f64, else -inf
     float eps_u, eps_x;          // f32 bands around the sample bounds and the aperture edge
@@ -203,7 +219,7 @@ __global__ void k_prep_time_terms(PstmMigrateArgs a, int tt_eff, int npv, double
     const double tf = a.taper_fraction;
     const double slope = tf > 0.0 ? 1.0 / (apv * tf) : 1e30 / apv;
     kc[e] = make_float4((float)s, (float)wc, (float)slope, 0.0f);
-    if (e < nt) {
+    if (qk && e < nt) {  // shared by every tile: written by one tile's preparation
         qk[k] = (float)q;
         t0k[k] = (float)t_s;
     }
@@ -311,12 +327,13 @@ __global__ __launch_bounds__(256) void k_integrate(PstmMigrateArgs a, int rowlen
 // operation order, for the aperture test as well (dist_mid > aperture,
 // :185-187).  Returns whether the contribution counts and its f64 position.
 template <int TT>
-__device__ __forceinline__ bool exact_decision(const KParams& P, int pil, int row, int k, double* u_out) {
-    const int ix = pil / P.ny, iy = pil % P.ny;
-    const double ox = P.xax[ix], oy = P.yax[iy];
+__device__ __forceinline__ bool exact_decision(const KParams& P, const TileView& T, int pil, int row, int k,
+                                               double* u_out) {
+    const int ix = pil / T.ny, iy = pil % T.ny;
+    const double ox = T.xax[ix], oy = T.yax[iy];
     const double dxm = __dsub_rn(ox, P.mx64[row]), dym = __dsub_rn(oy, P.my64[row]);
     const double dist_mid = __dsqrt_rn(__dadd_rn(__dmul_rn(dxm, dxm), __dmul_rn(dym, dym)));
-    const double v = P.vrms64[(size_t)pil * P.kc_pstride + k];
+    const double v = T.vrms64[(size_t)pil * T.kc_pstride + k];
     const double t_s = P.tax64[k] / 1000.0;
     double ap = v * t_s * 0.5 * P.tan_dip;
     if (ap < P.min_ap64) ap = P.min_ap64;
@@ -344,7 +361,7 @@ __device__ __forceinline__ bool exact_decision(const KParams& P, int pil, int ro
             (1.0 / kg) * log((vz + sqrt(vz * vz + k2 * k2)) / (v0 + sqrt(v0 * v0 + k2 * k2)));
     } else {
         // VTI one-way legs (anisotropy_vti.py:181-220)
-        const double eta = P.eta64 ? P.eta64[(size_t)pil * P.eta_pstride + k] : P.eta_const;
+        const double eta = T.eta64 ? T.eta64[(size_t)pil * T.eta_pstride + k] : P.eta_const;
         const double vv = 1.0 / sqrt(1.0 / (v * v));
         const double tau = t_s / 2.0;
         t = 0.0;
@@ -628,13 +645,29 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     const int wave = tid >> 6;
     const int tw = wave % kTW;
     const int pg = wave / kTW;
-    const int npil = P.nx * P.ny;
     const int win = 64 * kTW * MW;
     unsigned long long cnt = 0;
+    __shared__ int s_item;
 
-    for (int item = blockIdx.x; item < P.n_items; item += gridDim.x) {
-        const int pgroup = item % P.n_pgroups;
-        const int kwin = item / P.n_pgroups;
+    // persistent workgroups drain the work queue; every workgroup leaves when
+    // the counter passes n_items
+    for (;;) {
+        if (tid == 0) s_item = (int)atomicAdd(P.queue, 1u);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);  // uniform: keeps the tile view scalar
+        if (item >= P.n_items) break;
+        int lo_t = 0, hi_t = P.n_tiles - 1;  // last tile with item0 <= item
+        while (lo_t < hi_t) {
+            const int mid = (lo_t + hi_t + 1) >> 1;
+            if (P.tiles[mid].item0 <= item) lo_t = mid;
+            else hi_t = mid - 1;
+        }
+        // read in place (scalar loads where used) rather than copied to registers
+        const TileView& T = P.tiles[lo_t];
+        const int npil = T.nx * T.ny;
+        const int li = item - T.item0;
+        const int pgroup = li % T.n_pgroups;
+        const int kwin = li / T.n_pgroups;
         const int pil0 = pgroup * kNPil;
         const int kbase = kwin * win;
 
@@ -657,12 +690,12 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 e2[p][m] = 0.0f;
                 e1[p][m] = 1.0f;
                 if (kv && pil < npil) {
-                    const float4 c = P.kc[(size_t)pil * P.kc_pstride + k];
+                    const float4 c = T.kc[(size_t)pil * T.kc_pstride + k];
                     cs[p][m] = c.x;
                     cw[p][m] = c.y;
                     ci[p][m] = c.z;
                     if (TT == PSTM_TT_VTI) {
-                        const float2 ee = P.ek[(size_t)pil * P.ek_pstride + k];
+                        const float2 ee = T.ek[(size_t)pil * T.ek_pstride + k];
                         e2[p][m] = ee.x;
                         e1[p][m] = ee.y;
                     }
@@ -697,8 +730,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             nfr = (float)(lane & ((1 << P.lat_shift) - 1)) * P.lat_ifrac;
         }
 
-        for (int j0 = 0; j0 < P.ntr; j0 += P.batch) {
-            const int nb = min(P.batch, P.ntr - j0);
+        for (int j0 = 0; j0 < T.ntr; j0 += P.batch) {
+            const int nb = min(P.batch, T.ntr - j0);
             // (pillar, trace) pair table; the max-aperture test is the
             // oracle's f64 one (numba_cpu_optimized.py:167-173)
             int inap = 0;
@@ -707,16 +740,16 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
                 float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (pt_j < nb && pil < npil) {
-                    const double pox64 = P.xax[pil / P.ny], poy64 = P.yax[pil % P.ny];
-                    const float pox = (float)(pox64 - P.xax[0]), poy = (float)(poy64 - P.yax[0]);
-                    const int trow = P.tidx ? P.tidx[j0 + pt_j] : j0 + pt_j;
+                    const double pox64 = T.xax[pil / T.ny], poy64 = T.yax[pil % T.ny];
+                    const float pox = (float)(pox64 - T.xax[0]), poy = (float)(poy64 - T.yax[0]);
+                    const int trow = T.tidx ? T.tidx[j0 + pt_j] : j0 + pt_j;
                     const double dx64 = __dsub_rn(pox64, P.mx64[trow]);
                     const double dy64 = __dsub_rn(poy64, P.my64[trow]);
                     const double dm64 = __dsqrt_rn(__dadd_rn(__dmul_rn(dx64, dx64), __dmul_rn(dy64, dy64)));
                     if (!(dm64 > P.max_ap64)) {
                         inap = 1;
-                        const float4 g0 = P.geo[2 * (j0 + pt_j)];
-                        const float4 g1 = P.geo[2 * (j0 + pt_j) + 1];
+                        const float4 g0 = T.geo[2 * (j0 + pt_j)];
+                        const float4 g1 = T.geo[2 * (j0 + pt_j) + 1];
                         const float dxs = pox - g0.x, dys = poy - g0.y;
                         const float dxr = pox - g0.z, dyr = poy - g0.w;
                         const float dxm = pox - g1.x, dym = poy - g1.y;
@@ -762,8 +795,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             const float t0 = P.t0k[kk] + (float)(k - kk) * dt_last;
                             const float tau = 0.5f * t0;
                             const float q = tau * tau;
-                            const float sl = P.kc[(size_t)pil * P.kc_pstride + kk].x;  // 1/v^2
-                            const float eta = 0.5f * P.ek[(size_t)pil * P.ek_pstride + kk].x;
+                            const float sl = T.kc[(size_t)pil * T.kc_pstride + kk].x;  // 1/v^2
+                            const float eta = 0.5f * T.ek[(size_t)pil * T.ek_pstride + kk].x;
                             const float hs = e.x * sl, hr = e.y * sl;
                             const float ts2 = q + hs, tr2 = q + hr;
                             const float sn_s = ts2 > 0.0f ? fsqrt(hs / ts2) : 0.0f;
@@ -784,15 +817,22 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
 
             // stage the batch of trace rows in LDS (coalesced)
-            if (AA) {
+            if (AA && !P.prow_global) {
                 const double* src = P.prow + (size_t)j0 * P.rowlen;
                 double* dst = reinterpret_cast<double*>(rows);
                 const int n = nb * P.rowlen;
                 for (int e = tid; e < n; e += kThreads) dst[e] = src[e];
+            } else if (AA) {
+                double* dst = reinterpret_cast<double*>(rows);
+                for (int r = 0; r < nb; ++r) {
+                    const int row = T.tidx ? T.tidx[j0 + r] : j0 + r;
+                    const double* src = P.prow + (size_t)row * P.rowlen;
+                    for (int e = tid; e < P.rowlen; e += kThreads) dst[r * P.rowlen + e] = src[e];
+                }
             } else {
                 float* dst = reinterpret_cast<float*>(rows);
                 for (int r = 0; r < nb; ++r) {
-                    const int row = P.tidx ? P.tidx[j0 + r] : j0 + r;
+                    const int row = T.tidx ? T.tidx[j0 + r] : j0 + r;
                     const float* src = P.amp + (size_t)row * P.ns;
                     for (int e = tid; e < P.ns; e += kThreads) dst[r * P.rowlen + e] = src[e];
                 }
@@ -903,9 +943,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             if (!near) continue;
                             const int pil = pil0 + pg * kPW + p;
                             const int k = kbase + 64 * (tw + kTW * m) + lane;
-                            const int trow = P.tidx ? P.tidx[j0 + jj] : j0 + jj;
+                            const int trow = T.tidx ? T.tidx[j0 + jj] : j0 + jj;
                             double u64;
-                            if (exact_decision<TT>(P, pil, trow, k, &u64)) {
+                            if (exact_decision<TT>(P, T, pil, trow, k, &u64)) {
                                 const int i0 = (int)u64;
                                 const float f0 = (float)(u64 - (double)i0);
                                 acc[p][m] = fmaf(contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2),
@@ -935,7 +975,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             for (int p = 0; p < kPW; ++p) {
                 const int pil = pil0 + pg * kPW + p;
                 if (pil < npil) {
-                    float* dst = P.image + (size_t)pil * P.nt + k;
+                    float* dst = T.image + (size_t)pil * P.nt + k;
                     *dst = *dst + (float)tot[(p * MW + m) * kThreads + tid];
                 }
             }
@@ -1074,11 +1114,20 @@ int validate(const PstmMigrateArgs* a) {
     return 0;
 }
 
+// set by plan creation: configure the kernel's dynamic LDS limit only, so that
+// nothing but stream work happens while a plan's launch sequence is captured
+bool g_attr_only = false;
+
 template <int MW, int TT, bool AA, bool COUNT, bool LOB>
 int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
     auto fn = k_migrate<MW, TT, AA, COUNT, LOB>;
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
+    static size_t lds_set = 0;  // per instantiation
+    if (lds > lds_set) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        lds_set = lds;
+    }
+    if (g_attr_only) return 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (g_prof.on) {
         HIP_TRY(hipEventCreate(&ev0));
@@ -1116,6 +1165,229 @@ int dispatch_mw(int mw, bool count, bool lob, const KParams& kp, int grid, size_
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+__global__ void k_reset_queue(unsigned* q) {
+    if (threadIdx.x == 0) *q = 0u;
+}
+
+// the one-entry tile table of a single-tile call (kernel argument -> device)
+__global__ void k_set_view(TileView v, TileView* dst) {
+    if (threadIdx.x == 0) *dst = v;
+}
+
+// plans: fold += counts taken when the plan was built, one workgroup per tile
+__global__ __launch_bounds__(256) void k_fold_add(const TileView* tiles) {
+    const TileView t = tiles[blockIdx.x];
+    for (int p = threadIdx.x; p < t.nx * t.ny; p += blockDim.x) t.fold[p] += t.fold_cnt[p];
+}
+
+struct Shape;
+bool lo_check(const PstmMigrateArgs& a, const Shape& sh);
+
+// Launch shape and LDS layout shared by every tile of a call or plan.
+struct Shape {
+    int tt = 0;                  // traveltime evaluated (curved with |k| < 1e-10 -> straight)
+    bool aa = false, learned = false, vti_like = false;
+    int lmax = 0, lo = 0, rowlen = 0, pair_vec = 1;
+    size_t elem = 4;
+    int mw = 1, batch = 1, lat_nodes = 0, win = 256, nwin = 1;
+    size_t tot_bytes = 0, lds = 0;
+    int occupancy = 1;
+};
+
+Shape launch_shape(const PstmMigrateArgs& a) {
+    Shape sh;
+    sh.tt = a.traveltime;
+    if (sh.tt == PSTM_TT_CURVED && fabs(a.curved_k) < 1e-10) sh.tt = PSTM_TT_STRAIGHT;
+    sh.aa = a.aa_mode == PSTM_AA_TRIANGLE;
+    sh.learned = sh.tt == PSTM_TT_VTI_LEARNED;
+    sh.vti_like = sh.tt == PSTM_TT_VTI || sh.learned;
+    sh.lmax = sh.aa ? a.aa_max_half_width : 0;
+    sh.lo = sh.aa ? sh.lmax + 1 : 0;
+    const int ns = a.n_samples, nt = a.nt;
+    sh.rowlen = sh.aa ? ns + 2 * sh.lmax + 2 : ns;
+    sh.elem = sh.aa ? sizeof(double) : sizeof(float);
+    sh.pair_vec = (sh.aa || sh.tt == PSTM_TT_CURVED) ? 2 : 1;
+    // time windows of at most 1024 samples, split evenly; MW blocks of 64 per wave
+    const int nwin = (nt + 1023) / 1024;
+    const int per_win = (nt + nwin - 1) / nwin;
+    int mw_cap = g_tune.mw_cap > 0 ? g_tune.mw_cap : env_int("PSTM_HIP_MW");
+    if (mw_cap <= 0) mw_cap = 4;
+    mw_cap = std::min(4, mw_cap);
+    sh.mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
+    // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
+    sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * sizeof(double);
+    sh.lat_nodes = sh.learned ? 256 * sh.mw / a.mlp_node_spacing + 1 : 0;
+    auto lds_for = [&](int b) {
+        return align_up(align_up((size_t)b * sh.rowlen * sh.elem, 16) + (size_t)kNPil * b * sh.pair_vec * 16, 16) +
+               sh.tot_bytes + align_up((size_t)kNPil * b * sh.lat_nodes * 8, 16);
+    };
+    int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
+    sh.batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
+    while (sh.batch > 1 && lds_for(sh.batch) > kLdsBudget) --sh.batch;
+    sh.lds = lds_for(sh.batch);
+    sh.win = 256 * sh.mw;
+    sh.nwin = (nt + sh.win - 1) / sh.win;
+    sh.occupancy = sh.lds <= kLdsBudget / 2 ? 2 : 1;  // workgroups per CU the LDS allows
+    return sh;
+}
+
+bool lo_check(const PstmMigrateArgs& a, const Shape& sh) {
+    return a.t_start_in_ms > 0.0 || (sh.tt == PSTM_TT_CURVED && a.curved_k < 0.0) || sh.vti_like;
+}
+
+// Kernel parameters common to every tile (everything but the tile table).
+void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double tan_dip) {
+    const int ns = a.n_samples, nt = a.nt;
+    kp.tiles = nullptr;
+    kp.n_tiles = 0;
+    kp.amp = a.amplitudes;
+    kp.prow_global = 0;
+    kp.ns = ns;
+    kp.nt = nt;
+    kp.rowlen = sh.rowlen;
+    kp.lo = sh.lo;
+    kp.batch = sh.batch;
+    kp.pair_off = (int)align_up((size_t)sh.batch * sh.rowlen * sh.elem, 16);
+    kp.tot_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * sh.batch * sh.pair_vec * 16, 16);
+    kp.taper_off = a.taper_fraction > 0.0 ? (float)(-(1.0 - a.taper_fraction) / a.taper_fraction) : -1e30f;
+    kp.wpow = (a.apply_spreading ? 1 : 0) + (a.apply_obliquity ? 1 : 0);
+    const double dt_s = a.dt_in_ms / 1000.0;
+    kp.idt = (float)(1.0 / dt_s);
+    kp.uoff = (float)(-(a.t_start_in_ms / 1000.0) / dt_s);
+    kp.umax = (float)(ns - 1);
+    kp.max_ap = (float)a.max_aperture_m;
+    kp.gdx_idt = (float)(a.grid_dx / dt_s);
+    kp.gdy_idt = (float)(a.grid_dy / dt_s);
+    kp.lmax = (float)sh.lmax;
+    kp.ck2 = (float)(a.curved_k * a.curved_k);
+    kp.ik_ln2 = (float)(sh.tt == PSTM_TT_CURVED ? M_LN2 / a.curved_k : 0.0);
+    kp.v0 = (float)a.curved_v0;
+    kp.t_floor = a.curved_k > 0.0 ? 0.0f : -INFINITY;
+    kp.sx64 = a.source_x;
+    kp.sy64 = a.source_y;
+    kp.rx64 = a.receiver_x;
+    kp.ry64 = a.receiver_y;
+    kp.mx64 = a.midpoint_x;
+    kp.my64 = a.midpoint_y;
+    kp.tax64 = a.t_axis_ms;
+    kp.tt_req = a.traveltime;
+    // u >= 0 can only fail or be decided differently from f64 when t can be
+    // at or below t_start: t_start > 0, a curved ray with k < 0, or VTI
+    // (t = t0 for t0 <= 0, and negative output times give u < 0)
+    kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
+    kp.lo_eps = lo_check(a, sh) ? kp.eps_u : -INFINITY;
+    kp.umax_lo = kp.umax - kp.eps_u;
+    kp.umax_hi = kp.umax + kp.eps_u;
+    kp.eps_x = a.n_effective ? 1e-4f : 0.0f;
+    kp.tan_dip = tan_dip;
+    kp.min_ap64 = a.min_aperture_m;
+    kp.max_ap64 = a.max_aperture_m;
+    kp.t_start_s = a.t_start_in_ms / 1000.0;
+    kp.dt_s = dt_s;
+    kp.cv0 = a.curved_v0;
+    kp.ck = a.curved_k;
+    kp.eta_const = a.eta_constant;
+    kp.lat_off = (int)(kp.tot_off + sh.tot_bytes);
+    kp.lat_nodes = sh.lat_nodes;
+    kp.lat_shift = sh.learned ? __builtin_ctz((unsigned)a.mlp_node_spacing) : 0;
+    kp.lat_ifrac = sh.learned ? 1.0f / (float)a.mlp_node_spacing : 0.0f;
+    kp.m_tau_sc = sh.learned ? (float)(2.0 / a.mlp_tau_max) : 0.0f;
+    kp.m_v_lo = (float)a.mlp_v_lo;
+    kp.m_v_sc = sh.learned ? (float)(2.0 / (a.mlp_v_hi - a.mlp_v_lo)) : 0.0f;
+    kp.m_eta_sc = sh.learned ? (float)(2.0 / a.mlp_eta_max) : 0.0f;
+    kp.m_inv_out = sh.learned ? (float)(1.0 / a.mlp_out_scale) : 0.0f;
+    kp.mlpA = nullptr;
+    kp.mlpB = nullptr;
+}
+
+// Per-tile buffers: f32 tile-relative geometry (ntr x 32 B), per-(pillar,
+// sample) terms (npv x nt x 16 B) and VTI eta terms (nev x nt x 8 B).
+struct TileBufs {
+    float4* geo;
+    float4* kc;
+    float2* ek;
+};
+
+size_t tile_bytes(const PstmMigrateArgs& a, const Shape& sh, int ntr) {
+    const int npil = a.nx * a.ny;
+    const int npv = a.vel_mode == PSTM_VEL_3D ? npil : 1;
+    const int nev = a.eta_mode == PSTM_ETA_3D ? npil : 1;
+    return align_up((size_t)ntr * 32, 256) + align_up((size_t)npv * a.nt * 16, 256) +
+           align_up(sh.vti_like ? (size_t)nev * a.nt * 8 : 0, 256);
+}
+
+TileBufs carve_tile(char* base, const PstmMigrateArgs& a, const Shape& sh, int ntr) {
+    const int npil = a.nx * a.ny;
+    const int npv = a.vel_mode == PSTM_VEL_3D ? npil : 1;
+    TileBufs b;
+    b.geo = reinterpret_cast<float4*>(base);
+    base += align_up((size_t)ntr * 32, 256);
+    b.kc = reinterpret_cast<float4*>(base);
+    base += align_up((size_t)npv * a.nt * 16, 256);
+    b.ek = reinterpret_cast<float2*>(base);
+    return b;
+}
+
+// Launch the per-tile preparation kernels of `a` (one tile: axes, velocity,
+// eta, trace list) into b; qk/t0k (shared by all tiles) when non-null.
+int prep_tile(const PstmMigrateArgs& a, const Shape& sh, double tan_dip, const TileBufs& b, float* qk,
+              float* t0k, hipStream_t st) {
+    const int npil = a.nx * a.ny;
+    const int npv = a.vel_mode == PSTM_VEL_3D ? npil : 1;
+    const int nev = a.eta_mode == PSTM_ETA_3D ? npil : 1;
+    if (a.n_traces > 0) {
+        hipLaunchKernelGGL(k_prep_geometry, dim3((a.n_traces + 255) / 256), dim3(256), 0, st, a, b.geo);
+        HIP_TRY(hipGetLastError());
+    }
+    const long nkc = (long)npv * a.nt;
+    hipLaunchKernelGGL(k_prep_time_terms, dim3((unsigned)((nkc + 255) / 256)), dim3(256), 0, st, a, sh.tt, npv,
+                       tan_dip, b.kc, qk, t0k);
+    HIP_TRY(hipGetLastError());
+    if (sh.vti_like) {
+        const long nek = (long)nev * a.nt;
+        hipLaunchKernelGGL(k_prep_eta, dim3((unsigned)((nek + 255) / 256)), dim3(256), 0, st, a, nev, b.ek);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+TileView make_view(const PstmMigrateArgs& a, const Shape& sh, const TileBufs& b, int item0) {
+    const int npil = a.nx * a.ny;
+    TileView t;
+    t.xax = a.x_axis;
+    t.yax = a.y_axis;
+    t.kc = b.kc;
+    t.ek = b.ek;
+    t.vrms64 = a.vrms;
+    t.eta64 = (sh.vti_like && a.eta_mode != PSTM_ETA_CONST) ? a.eta : nullptr;
+    t.image = a.image;
+    t.geo = b.geo;
+    t.tidx = a.trace_index;
+    t.fold = a.fold;
+    t.fold_cnt = nullptr;
+    t.ntr = a.n_traces;
+    t.nx = a.nx;
+    t.ny = a.ny;
+    t.kc_pstride = a.vel_mode == PSTM_VEL_3D ? a.nt : 0;
+    t.ek_pstride = (sh.vti_like && a.eta_mode == PSTM_ETA_3D) ? a.nt : 0;
+    t.eta_pstride = a.eta_mode == PSTM_ETA_3D ? a.nt : 0;
+    t.n_pgroups = (npil + kNPil - 1) / kNPil;
+    t.item0 = item0;
+    return t;
+}
+
+long tile_items(const PstmMigrateArgs& a, const Shape& sh) {
+    return (long)((a.nx * a.ny + kNPil - 1) / kNPil) * sh.nwin;
+}
+
+int dispatch(const Shape& sh, bool count, bool lob, const KParams& kp, int grid, hipStream_t st) {
+    if (sh.tt == PSTM_TT_STRAIGHT && sh.aa) return dispatch_mw<PSTM_TT_STRAIGHT, true>(sh.mw, count, lob, kp, grid, sh.lds, st);
+    if (sh.tt == PSTM_TT_STRAIGHT) return dispatch_mw<PSTM_TT_STRAIGHT, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
+    if (sh.tt == PSTM_TT_CURVED) return dispatch_mw<PSTM_TT_CURVED, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
+    if (sh.tt == PSTM_TT_VTI) return dispatch_mw<PSTM_TT_VTI, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
+    return dispatch_mw<PSTM_TT_VTI_LEARNED, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
+}
+
 // shared by migrate_tile and migrate_bins: `ntr` rows starting at row
 // offset `row0` of args->trace_index (or of the block).
 int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hipStream_t st) {
@@ -1135,190 +1407,77 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
         if (a.trace_weights) a.trace_weights += row0;
     }
     if (ntr == 0) return 0;
-    const int nt = a.nt, ns = a.n_samples;
-    const int npil = a.nx * a.ny;
-    const int npv = a.vel_mode == PSTM_VEL_3D ? npil : 1;
-    const int nev = a.eta_mode == PSTM_ETA_3D ? npil : 1;
-    int tt = a.traveltime;
-    if (tt == PSTM_TT_CURVED && fabs(a.curved_k) < 1e-10) tt = PSTM_TT_STRAIGHT;
-    const bool aa = a.aa_mode == PSTM_AA_TRIANGLE;
-    const int lmax = aa ? a.aa_max_half_width : 0;
-    const int lo = aa ? lmax + 1 : 0;
-    const int rowlen = aa ? ns + 2 * lmax + 2 : ns;
-    const size_t elem = aa ? sizeof(double) : sizeof(float);
-    const int pair_vec = (aa || tt == PSTM_TT_CURVED) ? 2 : 1;
-    const bool learned = tt == PSTM_TT_VTI_LEARNED;
-    const bool vti_like = tt == PSTM_TT_VTI || learned;
-
-    // batch size from the LDS budget
-    // time windows
-    const int nwin = (nt + 1023) / 1024;
-    const int per_win = (nt + nwin - 1) / nwin;
-    int mw_cap = g_tune.mw_cap > 0 ? g_tune.mw_cap : env_int("PSTM_HIP_MW");
-    if (mw_cap <= 0) mw_cap = 4;
-    mw_cap = std::min(4, mw_cap);
-    int mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
-    // batch size from the LDS budget: trace rows + pair table + f64 totals
-    const size_t tot_bytes = (size_t)kPW * mw * kThreads * sizeof(double);
-    const int lat_nodes = learned ? 256 * mw / a.mlp_node_spacing + 1 : 0;
-    auto lds_for = [&](int b) {
-        return align_up(align_up((size_t)b * rowlen * elem, 16) + (size_t)kNPil * b * pair_vec * 16, 16) + tot_bytes +
-               align_up((size_t)kNPil * b * lat_nodes * 8, 16);
-    };
-    int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
-    int batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
-    while (batch > 1 && lds_for(batch) > kLdsBudget) --batch;
-    const size_t lds = lds_for(batch);
-    if (lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
-    const int win = 256 * mw;
-    const int nwin2 = (nt + win - 1) / win;
-    const int n_pgroups = (npil + kNPil - 1) / kNPil;
-    const long n_items = (long)n_pgroups * nwin2;
+    const int nt = a.nt;
+    const Shape sh = launch_shape(a);
+    if (sh.lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
+    const long n_items = tile_items(a, sh);
     if (n_items > (1L << 30)) return fail(1, "tile too large");
-    const int occupancy = lds <= kLdsBudget / 2 ? 2 : 1;  // workgroups per CU the LDS allows
-    const int grid = (int)std::min<long>(n_items, (long)g_state.n_cu * occupancy);
-
-    g_last.mw = mw;
-    g_last.batch = batch;
+    const int grid = (int)std::min<long>(n_items, (long)g_state.n_cu * sh.occupancy);
+    g_last.mw = sh.mw;
+    g_last.batch = sh.batch;
     g_last.grid = grid;
     g_last.items = (int)n_items;
 
     // workspace layout
     size_t off = 0;
-    const size_t o_geo = off; off = align_up(off + (size_t)ntr * 32, 256);
-    const size_t o_kc = off; off = align_up(off + (size_t)npv * nt * 16, 256);
+    const size_t o_tile = off; off = align_up(off + tile_bytes(a, sh, ntr), 256);
     const size_t o_qk = off; off = align_up(off + (size_t)nt * 4, 256);
     const size_t o_t0 = off; off = align_up(off + (size_t)nt * 4, 256);
-    const size_t o_ek = off; off = align_up(off + (vti_like ? (size_t)nev * nt * 8 : 0), 256);
-    const size_t o_mlpA = off; off = align_up(off + (learned ? (size_t)kANum * 8 : 0), 256);
-    const size_t o_mlpB = off; off = align_up(off + (learned ? (size_t)kBNum * 64 : 0), 256);
+    const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kANum * 8 : 0), 256);
+    const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBNum * 64 : 0), 256);
     const size_t o_cnt = off; off = align_up(off + (size_t)grid * 8, 256);
-    const size_t o_prow = off; off = align_up(off + (aa ? (size_t)ntr * rowlen * 8 : 0), 256);
+    const size_t o_q = off; off = align_up(off + 8, 256);
+    const size_t o_view = off; off = align_up(off + sizeof(TileView), 256);
+    const size_t o_prow = off; off = align_up(off + (sh.aa ? (size_t)ntr * sh.rowlen * 8 : 0), 256);
     rc = ensure_workspace(off);
     if (rc) return rc;
     char* ws = static_cast<char*>(g_state.ws);
-    float4* geo = reinterpret_cast<float4*>(ws + o_geo);
-    float4* kc = reinterpret_cast<float4*>(ws + o_kc);
+    const TileBufs tb = carve_tile(ws + o_tile, a, sh, ntr);
     float* qk = reinterpret_cast<float*>(ws + o_qk);
     float* t0k = reinterpret_cast<float*>(ws + o_t0);
-    float2* ek = reinterpret_cast<float2*>(ws + o_ek);
     unsigned long long* cnt = reinterpret_cast<unsigned long long*>(ws + o_cnt);
+    unsigned* queue = reinterpret_cast<unsigned*>(ws + o_q);
     double* prow = reinterpret_cast<double*>(ws + o_prow);
 
-    hipLaunchKernelGGL(k_prep_geometry, dim3((ntr + 255) / 256), dim3(256), 0, st, a, geo);
-    HIP_TRY(hipGetLastError());
     const double tan_dip = tan(a.max_dip_deg * M_PI / 180.0);
-    const long nkc = (long)npv * nt;
-    hipLaunchKernelGGL(k_prep_time_terms, dim3((unsigned)((nkc + 255) / 256)), dim3(256), 0, st, a, tt, npv, tan_dip,
-                       kc, qk, t0k);
-    HIP_TRY(hipGetLastError());
-    if (learned) {
+    rc = prep_tile(a, sh, tan_dip, tb, qk, t0k, st);
+    if (rc) return rc;
+    if (sh.learned) {
         const int n = kANum * 4 + kBNum * 16;
         hipLaunchKernelGGL(k_prep_mlp, dim3((n + 255) / 256), dim3(256), 0, st, a.mlp_params,
                            reinterpret_cast<unsigned short*>(ws + o_mlpA), reinterpret_cast<float*>(ws + o_mlpB));
         HIP_TRY(hipGetLastError());
     }
-    if (vti_like) {
-        const long nek = (long)nev * nt;
-        hipLaunchKernelGGL(k_prep_eta, dim3((unsigned)((nek + 255) / 256)), dim3(256), 0, st, a, nev, ek);
-        HIP_TRY(hipGetLastError());
-    }
-    if (aa) {
-        hipLaunchKernelGGL(k_integrate, dim3(ntr), dim3(256), 0, st, a, rowlen, lo, prow);
+    if (sh.aa) {
+        hipLaunchKernelGGL(k_integrate, dim3(ntr), dim3(256), 0, st, a, sh.rowlen, sh.lo, prow);
         HIP_TRY(hipGetLastError());
     }
     if (do_fold) {
-        hipLaunchKernelGGL(k_fold, dim3(npil), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_fold, dim3(a.nx * a.ny), dim3(256), 0, st, a);
         HIP_TRY(hipGetLastError());
     }
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, st, queue);
+    HIP_TRY(hipGetLastError());
+    TileView* view = reinterpret_cast<TileView*>(ws + o_view);
+    hipLaunchKernelGGL(k_set_view, dim3(1), dim3(64), 0, st, make_view(a, sh, tb, 0), view);
+    HIP_TRY(hipGetLastError());
 
     KParams kp;
-    kp.amp = a.amplitudes;
-    kp.tidx = a.trace_index;
-    kp.geo = geo;
+    fill_common(kp, a, sh, tan_dip);
+    kp.tiles = view;
+    kp.n_tiles = 1;
+    kp.queue = queue;
     kp.prow = prow;
-    kp.kc = kc;
     kp.qk = qk;
-    kp.ek = ek;
     kp.t0k = t0k;
-    kp.xax = a.x_axis;
-    kp.yax = a.y_axis;
-    kp.image = a.image;
     kp.cnt_part = a.n_effective ? cnt : nullptr;
-    kp.ns = ns;
-    kp.ntr = ntr;
-    kp.nx = a.nx;
-    kp.ny = a.ny;
-    kp.nt = nt;
-    kp.kc_pstride = npv > 1 ? nt : 0;
-    kp.ek_pstride = nev > 1 ? nt : 0;
-    kp.rowlen = rowlen;
-    kp.lo = lo;
-    kp.batch = batch;
-    kp.pair_off = (int)align_up((size_t)batch * rowlen * elem, 16);
-    kp.tot_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * batch * pair_vec * 16, 16);
-    kp.taper_off = a.taper_fraction > 0.0 ? (float)(-(1.0 - a.taper_fraction) / a.taper_fraction) : -1e30f;
     kp.n_items = (int)n_items;
-    kp.n_pgroups = n_pgroups;
-    kp.wpow = (a.apply_spreading ? 1 : 0) + (a.apply_obliquity ? 1 : 0);
-    const double dt_s = a.dt_in_ms / 1000.0;
-    kp.idt = (float)(1.0 / dt_s);
-    kp.uoff = (float)(-(a.t_start_in_ms / 1000.0) / dt_s);
-    kp.umax = (float)(ns - 1);
-    kp.max_ap = (float)a.max_aperture_m;
-    kp.gdx_idt = (float)(a.grid_dx / dt_s);
-    kp.gdy_idt = (float)(a.grid_dy / dt_s);
-    kp.lmax = (float)lmax;
-    kp.ck2 = (float)(a.curved_k * a.curved_k);
-    kp.ik_ln2 = (float)(tt == PSTM_TT_CURVED ? M_LN2 / a.curved_k : 0.0);
-    kp.v0 = (float)a.curved_v0;
-    kp.t_floor = a.curved_k > 0.0 ? 0.0f : -INFINITY;
-    kp.sx64 = a.source_x;
-    kp.sy64 = a.source_y;
-    kp.rx64 = a.receiver_x;
-    kp.ry64 = a.receiver_y;
-    kp.mx64 = a.midpoint_x;
-    kp.my64 = a.midpoint_y;
-    kp.vrms64 = a.vrms;
-    kp.tax64 = a.t_axis_ms;
-    kp.eta64 = (vti_like && a.eta_mode != PSTM_ETA_CONST) ? a.eta : nullptr;
-    kp.eta_pstride = a.eta_mode == PSTM_ETA_3D ? nt : 0;
-    kp.tt_req = a.traveltime;
-    // u >= 0 can only fail or be decided differently from f64 when t can be
-    // at or below t_start: t_start > 0, a curved ray with k < 0, or VTI
-    // VTI returns t = t0 for t0 <= 0 (negative output times give u < 0)
-    const bool lo_check = a.t_start_in_ms > 0.0 || (tt == PSTM_TT_CURVED && a.curved_k < 0.0) || vti_like;
-    kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
-    kp.lo_eps = lo_check ? kp.eps_u : -INFINITY;
-    kp.umax_lo = kp.umax - kp.eps_u;
-    kp.umax_hi = kp.umax + kp.eps_u;
-    kp.eps_x = a.n_effective ? 1e-4f : 0.0f;
-    kp.tan_dip = tan_dip;
-    kp.min_ap64 = a.min_aperture_m;
-    kp.max_ap64 = a.max_aperture_m;
-    kp.t_start_s = a.t_start_in_ms / 1000.0;
-    kp.dt_s = a.dt_in_ms / 1000.0;
-    kp.cv0 = a.curved_v0;
-    kp.ck = a.curved_k;
-    kp.eta_const = a.eta_constant;
-    kp.mlpA = learned ? ws + o_mlpA : nullptr;
-    kp.mlpB = learned ? ws + o_mlpB : nullptr;
-    kp.lat_off = (int)(kp.tot_off + tot_bytes);
-    kp.lat_nodes = lat_nodes;
-    kp.lat_shift = learned ? __builtin_ctz((unsigned)a.mlp_node_spacing) : 0;
-    kp.lat_ifrac = learned ? 1.0f / (float)a.mlp_node_spacing : 0.0f;
-    kp.m_tau_sc = learned ? (float)(2.0 / a.mlp_tau_max) : 0.0f;
-    kp.m_v_lo = (float)a.mlp_v_lo;
-    kp.m_v_sc = learned ? (float)(2.0 / (a.mlp_v_hi - a.mlp_v_lo)) : 0.0f;
-    kp.m_eta_sc = learned ? (float)(2.0 / a.mlp_eta_max) : 0.0f;
-    kp.m_inv_out = learned ? (float)(1.0 / a.mlp_out_scale) : 0.0f;
-
+    if (sh.learned) {
+        kp.mlpA = ws + o_mlpA;
+        kp.mlpB = ws + o_mlpB;
+    }
     const bool count = a.n_effective != nullptr;
-    if (tt == PSTM_TT_STRAIGHT && aa) rc = dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, count, lo_check, kp, grid, lds, st);
-    else if (tt == PSTM_TT_STRAIGHT) rc = dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, count, lo_check, kp, grid, lds, st);
-    else if (tt == PSTM_TT_CURVED) rc = dispatch_mw<PSTM_TT_CURVED, false>(mw, count, lo_check, kp, grid, lds, st);
-    else if (tt == PSTM_TT_VTI) rc = dispatch_mw<PSTM_TT_VTI, false>(mw, count, lo_check, kp, grid, lds, st);
-    else rc = dispatch_mw<PSTM_TT_VTI_LEARNED, false>(mw, count, lo_check, kp, grid, lds, st);
+    rc = dispatch(sh, count, kp.lo_eps != -INFINITY, kp, grid, st);
     if (rc) return rc;
     if (a.n_effective) {
         hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(256), 0, st, (const unsigned long long*)cnt, grid,
@@ -1330,7 +1489,279 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
 
 }  // namespace
 
+// A device-resident work queue over many output tiles (common-offset bins
+// included: a tile's trace list may hold rows of every bin): per-tile
+// geometry, velocity and fold terms are built once; a run is one persistent
+// gather launch whose workgroups take (tile, pillar group, time window)
+// items from a device counter, heaviest tiles first, plus a few small
+// kernels, replayed from a captured hipGraph.
+struct PstmPlan {
+    PstmMigrateArgs common;
+    Shape sh;
+    int n_tiles = 0;
+    long n_items = 0;
+    int grid = 0;
+    int block_rows = 0;
+    double tan_dip = 0.0;
+    void* mem = nullptr;
+    TileView* d_views = nullptr;
+    unsigned* queue = nullptr;
+    unsigned long long* cnt = nullptr;
+    double* prow = nullptr;
+    void* mlpA = nullptr;
+    void* mlpB = nullptr;
+    KParams kp;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// the launch sequence of one plan run on `st`
+int plan_enqueue(PstmPlan* pl, hipStream_t st) {
+    const PstmMigrateArgs& a = pl->common;
+    const Shape& sh = pl->sh;
+    if (sh.learned) {
+        const int n = kANum * 4 + kBNum * 16;
+        hipLaunchKernelGGL(k_prep_mlp, dim3((n + 255) / 256), dim3(256), 0, st, a.mlp_params,
+                           static_cast<unsigned short*>(pl->mlpA), static_cast<float*>(pl->mlpB));
+        HIP_TRY(hipGetLastError());
+    }
+    if (sh.aa) {
+        PstmMigrateArgs all = a;
+        all.trace_index = nullptr;
+        all.n_traces = pl->block_rows;
+        hipLaunchKernelGGL(k_integrate, dim3(pl->block_rows), dim3(256), 0, st, all, sh.rowlen, sh.lo, pl->prow);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_fold_add, dim3(pl->n_tiles), dim3(256), 0, st, (const TileView*)pl->d_views);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, st, pl->queue);
+    HIP_TRY(hipGetLastError());
+    const bool count = a.n_effective != nullptr;
+    int rc = dispatch(sh, count, pl->kp.lo_eps != -INFINITY, pl->kp, pl->grid, st);
+    if (rc) return rc;
+    if (count) {
+        hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(256), 0, st, (const unsigned long long*)pl->cnt, pl->grid,
+                           a.n_effective);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+void plan_free(PstmPlan* pl) {
+    if (!pl) return;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->cap) (void)hipStreamDestroy(pl->cap);
+    if (pl->mem) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(pl->mem);
+    }
+    delete pl;
+}
+
+}  // namespace
+
 extern "C" {
+
+int pstm_hip_plan_create(const PstmMigrateArgs* common, const PstmTileDesc* tiles, int32_t n_tiles,
+                         PstmPlan** out) {
+    g_last_error.clear();
+    if (!common || !tiles || !out || n_tiles <= 0) return fail(1, "pstm_hip_plan_create: bad arguments");
+    *out = nullptr;
+    int rc = ensure_init();
+    if (rc) return rc;
+    // per-tile argument sets, validated like single-tile calls
+    std::vector<PstmMigrateArgs> targs(n_tiles);
+    for (int i = 0; i < n_tiles; ++i) {
+        PstmMigrateArgs t = *common;
+        const PstmTileDesc& d = tiles[i];
+        t.x_axis = d.x_axis;
+        t.y_axis = d.y_axis;
+        t.vrms = d.vrms;
+        t.eta = d.eta;
+        t.image = d.image;
+        t.fold = d.fold;
+        t.trace_index = d.trace_index;
+        t.n_traces = d.n_traces;
+        t.nx = d.nx;
+        t.ny = d.ny;
+        rc = validate(&t);
+        if (rc) return fail(rc, "tile " + std::to_string(i) + ": " + g_last_error);
+        if (d.n_traces > 0 && !d.trace_index) return fail(1, "tile " + std::to_string(i) + ": trace_index must be set");
+        targs[i] = t;
+    }
+    if (common->n_traces <= 0) return fail(1, "pstm_hip_plan_create: common.n_traces must be the rows of the block");
+    const Shape sh = launch_shape(targs[0]);
+    if (sh.lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
+    // heaviest tiles first
+    std::vector<int> order(n_tiles);
+    for (int i = 0; i < n_tiles; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return (long)targs[x].n_traces * tile_items(targs[x], sh) >
+                                                (long)targs[y].n_traces * tile_items(targs[y], sh); });
+    auto* pl = new PstmPlan();
+    pl->common = *common;
+    pl->sh = sh;
+    pl->n_tiles = n_tiles;
+    pl->block_rows = common->n_traces;
+    pl->tan_dip = tan(common->max_dip_deg * M_PI / 180.0);
+    // device memory: tiles, fold counts, shared terms, views, counters, rows
+    std::vector<size_t> o_tile(n_tiles), o_fc(n_tiles);
+    size_t off = 0;
+    for (int i = 0; i < n_tiles; ++i) {
+        o_tile[i] = off;
+        off = align_up(off + tile_bytes(targs[i], sh, targs[i].n_traces), 256);
+        o_fc[i] = off;
+        off = align_up(off + (size_t)targs[i].nx * targs[i].ny * 4, 256);
+    }
+    const int nt = common->nt;
+    const size_t o_qk = off; off = align_up(off + (size_t)nt * 4, 256);
+    const size_t o_t0 = off; off = align_up(off + (size_t)nt * 4, 256);
+    const size_t o_views = off; off = align_up(off + (size_t)n_tiles * sizeof(TileView), 256);
+    long n_items = 0;
+    for (int i = 0; i < n_tiles; ++i) n_items += tile_items(targs[i], sh);
+    if (n_items > (1L << 30)) {
+        plan_free(pl);
+        return fail(1, "plan too large");
+    }
+    pl->n_items = n_items;
+    pl->grid = (int)std::min<long>(n_items, (long)g_state.n_cu * sh.occupancy);
+    const size_t o_cnt = off; off = align_up(off + (size_t)pl->grid * 8, 256);
+    const size_t o_q = off; off = align_up(off + 8, 256);
+    const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kANum * 8 : 0), 256);
+    const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBNum * 64 : 0), 256);
+    const size_t o_prow = off; off = align_up(off + (sh.aa ? (size_t)pl->block_rows * sh.rowlen * 8 : 0), 256);
+    hipError_t e = hipMalloc(&pl->mem, off);
+    if (e != hipSuccess) {
+        pl->mem = nullptr;
+        plan_free(pl);
+        return fail(3, std::string("pstm_hip_plan_create: hipMalloc: ") + hipGetErrorString(e));
+    }
+    char* base = static_cast<char*>(pl->mem);
+    float* qk = reinterpret_cast<float*>(base + o_qk);
+    float* t0k = reinterpret_cast<float*>(base + o_t0);
+    pl->d_views = reinterpret_cast<TileView*>(base + o_views);
+    pl->cnt = reinterpret_cast<unsigned long long*>(base + o_cnt);
+    pl->queue = reinterpret_cast<unsigned*>(base + o_q);
+    pl->mlpA = sh.learned ? base + o_mlpA : nullptr;
+    pl->mlpB = sh.learned ? base + o_mlpB : nullptr;
+    pl->prow = reinterpret_cast<double*>(base + o_prow);
+    e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        pl->cap = nullptr;
+        plan_free(pl);
+        return fail(3, std::string("pstm_hip_plan_create: hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    hipStream_t st = pl->cap;
+    // per-tile preparation, fold counts and views (in item order)
+    std::vector<TileView> views(n_tiles);
+    long item0 = 0;
+    for (int r = 0; r < n_tiles; ++r) {
+        const int i = order[r];
+        const PstmMigrateArgs& t = targs[i];
+        const TileBufs tb = carve_tile(base + o_tile[i], t, sh, t.n_traces);
+        rc = prep_tile(t, sh, pl->tan_dip, tb, r == 0 ? qk : nullptr, r == 0 ? t0k : nullptr, st);
+        if (rc) { plan_free(pl); return rc; }
+        int32_t* fc = reinterpret_cast<int32_t*>(base + o_fc[i]);
+        e = hipMemsetAsync(fc, 0, (size_t)t.nx * t.ny * 4, st);
+        if (e != hipSuccess) { plan_free(pl); return fail(3, hipGetErrorString(e)); }
+        if (t.n_traces > 0) {
+            PstmMigrateArgs fa = t;
+            fa.fold = fc;
+            hipLaunchKernelGGL(k_fold, dim3(t.nx * t.ny), dim3(256), 0, st, fa);
+            e = hipGetLastError();
+            if (e != hipSuccess) { plan_free(pl); return fail(3, hipGetErrorString(e)); }
+        }
+        views[r] = make_view(t, sh, tb, (int)item0);
+        views[r].fold_cnt = fc;
+        item0 += tile_items(t, sh);
+    }
+    e = hipMemcpyAsync(pl->d_views, views.data(), views.size() * sizeof(TileView), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { plan_free(pl); return fail(3, std::string("pstm_hip_plan_create: ") + hipGetErrorString(e)); }
+
+    KParams& kp = pl->kp;
+    fill_common(kp, *common, sh, pl->tan_dip);
+    kp.tiles = pl->d_views;
+    kp.n_tiles = n_tiles;
+    kp.queue = pl->queue;
+    kp.prow = pl->prow;
+    kp.prow_global = 1;
+    kp.qk = qk;
+    kp.t0k = t0k;
+    kp.cnt_part = common->n_effective ? pl->cnt : nullptr;
+    kp.n_items = (int)n_items;
+    kp.mlpA = pl->mlpA;
+    kp.mlpB = pl->mlpB;
+    // kernel attributes now, so that a captured run holds stream work only
+    g_attr_only = true;
+    rc = dispatch(sh, common->n_effective != nullptr, kp.lo_eps != -INFINITY, kp, pl->grid, st);
+    g_attr_only = false;
+    if (rc) { plan_free(pl); return rc; }
+    g_last.mw = sh.mw;
+    g_last.batch = sh.batch;
+    g_last.grid = pl->grid;
+    g_last.items = (int)n_items;
+    *out = pl;
+    return 0;
+}
+
+int pstm_hip_plan_run(PstmPlan* pl, int32_t use_graph, void* stream) {
+    g_last_error.clear();
+    if (!pl) return fail(1, "pstm_hip_plan_run: plan is NULL");
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (!use_graph) return plan_enqueue(pl, st);
+    if (!pl->exec) {
+        const bool prof = g_prof.on;
+        g_prof.on = false;  // no event records inside the captured sequence
+        HIP_TRY(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
+        rc = plan_enqueue(pl, pl->cap);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(pl->cap, &g);
+        g_prof.on = prof;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIP_TRY(e);
+        pl->graph = g;
+        HIP_TRY(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+    }
+    if (g_prof.on) {
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+        HIP_TRY(hipEventRecord(ev0, st));
+    }
+    HIP_TRY(hipGraphLaunch(pl->exec, st));
+    if (g_prof.on) {
+        HIP_TRY(hipEventRecord(ev1, st));
+        g_prof.events.emplace_back(ev0, ev1);
+    }
+    return 0;
+}
+
+int pstm_hip_plan_info(const PstmPlan* pl, long long* n_items, int* grid, int* mw, int* batch) {
+    g_last_error.clear();
+    if (!pl || !n_items || !grid || !mw || !batch) return fail(1, "pstm_hip_plan_info: bad arguments");
+    *n_items = pl->n_items;
+    *grid = pl->grid;
+    *mw = pl->sh.mw;
+    *batch = pl->sh.batch;
+    return 0;
+}
+
+int pstm_hip_plan_destroy(PstmPlan* pl) {
+    g_last_error.clear();
+    plan_free(pl);
+    return 0;
+}
+
 
 int pstm_hip_abi_version(void) { return PSTM_HIP_ABI_VERSION; }
 

@@ -54,6 +54,17 @@ class PstmMigrateArgs(ctypes.Structure):
     ]
 
 
+class PstmTileDesc(ctypes.Structure):
+    """Field-for-field mirror of ``struct PstmTileDesc``."""
+
+    _fields_ = [
+        ("x_axis", _vp), ("y_axis", _vp), ("vrms", _vp), ("eta", _vp),
+        ("image", _vp), ("fold", _vp), ("trace_index", _vp),
+        ("n_traces", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 #: every symbol include/pstm_hip.h declares
 EXPORTED_SYMBOLS = (
     "pstm_hip_abi_version", "pstm_hip_args_size", "pstm_hip_last_error",
@@ -61,6 +72,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
     "pstm_hip_set_tuning", "pstm_hip_last_launch",
+    "pstm_hip_plan_create", "pstm_hip_plan_run", "pstm_hip_plan_info", "pstm_hip_plan_destroy",
     "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_set_precision", "pstm_mlp_forward",
     "pstm_mlp_forward_mfma", "pstm_mlp_loss_grad",
     "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_vti_leg_times",
@@ -107,6 +119,14 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     ip = ctypes.POINTER(ctypes.c_int)
     lib.pstm_hip_last_launch.restype = ctypes.c_int
     lib.pstm_hip_last_launch.argtypes = [ip, ip, ip, ip]
+    lib.pstm_hip_plan_create.restype = ctypes.c_int
+    lib.pstm_hip_plan_create.argtypes = [_vp, _vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    lib.pstm_hip_plan_run.restype = ctypes.c_int
+    lib.pstm_hip_plan_run.argtypes = [_vp, ctypes.c_int32, _vp]
+    lib.pstm_hip_plan_info.restype = ctypes.c_int
+    lib.pstm_hip_plan_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong), ip, ip, ip]
+    lib.pstm_hip_plan_destroy.restype = ctypes.c_int
+    lib.pstm_hip_plan_destroy.argtypes = [_vp]
     f = ctypes.c_float
     lib.pstm_mlp_param_count.restype = ctypes.c_int
     lib.pstm_mlp_param_count.argtypes = []

@@ -176,10 +176,76 @@ class HipKernel:
         self._require_init()
         torch = _torch()
         lib = self._lib
+        counter = torch.zeros(1, dtype=torch.int64, device=self.device) if count_effective else None
+        a, keep = self._args(dt, image, fold2d, x_axis, y_axis, t_axis_ms, vrms, config, traveltime,
+                             trace_index, eta, counter, surrogate, node_spacing)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self._device_index).cuda_stream)
+        if group_start is None:
+            rc = lib.pstm_hip_migrate_tile(ctypes.byref(a), stream)
+            self._check(lib, rc, "pstm_hip_migrate_tile")
+        else:
+            gs = np.ascontiguousarray(group_start, dtype=np.int32)
+            rc = lib.pstm_hip_migrate_bins(ctypes.byref(a), gs.ctypes.data_as(ctypes.c_void_p),
+                                           int(gs.shape[0] - 1), stream)
+            self._check(lib, rc, "pstm_hip_migrate_bins")
+        del keep
+        if counter is not None:
+            self.synchronize()
+            return int(counter.item())
+        return None
+
+    def create_plan(self, dt: DeviceTraces, tiles: list[dict], t_axis_ms, config: KernelConfig, *,
+                    traveltime: int | None = None, count_effective: bool = False, surrogate=None,
+                    node_spacing: int = 16) -> "HipPlan":
+        """A device-resident work queue over many output tiles: one persistent,
+        hipGraph-captured gather per run (``HipPlan.run``).  Each tile is a
+        dict with image, fold (device tensors, accumulated), x_axis, y_axis,
+        vrms, trace_index and optionally eta; ``dt`` is the whole trace block
+        (every tile's trace_index indexes its rows)."""
+        self._require_init()
+        torch = _torch()
+        lib = self._lib
+        if not tiles:
+            raise ValueError("a plan needs at least one tile")
+        counter = torch.zeros(1, dtype=torch.int64, device=self.device) if count_effective else None
+        t0 = tiles[0]
+        a, keep = self._args(dt, t0["image"], t0["fold"], t0["x_axis"], t0["y_axis"], t_axis_ms, t0["vrms"],
+                             config, traveltime, t0["trace_index"], t0.get("eta"), counter, surrogate,
+                             node_spacing)
+        a.n_traces = dt.n_rows
+        descs = (_abi.PstmTileDesc * len(tiles))()
+        nt = int(t_axis_ms.shape[0])
+        for d, t in zip(descs, tiles):
+            nx, ny = int(t["image"].shape[0]), int(t["image"].shape[1])
+            if tuple(t["image"].shape) != (nx, ny, nt) or t["image"].dtype != torch.float32:
+                raise ValueError("tile image must be (nx, ny, nt) float32")
+            v = t["vrms"]
+            if (v.dim() == 3) != (a.vel_mode == _abi.VEL_3D):
+                raise ValueError("every tile of a plan needs the same velocity layout")
+            if v.dim() == 3 and tuple(v.shape) != (nx, ny, nt):
+                raise ValueError(f"3-D velocity shape {tuple(v.shape)} != tile {(nx, ny, nt)}")
+            eta = t.get("eta")
+            if (eta is None) != (a.eta_mode == _abi.ETA_CONST) or (
+                    eta is not None and (eta.dim() == 1) != (a.eta_mode == _abi.ETA_1D)):
+                raise ValueError("every tile of a plan needs the same eta layout")
+            d.x_axis, d.y_axis, d.vrms = _ptr(t["x_axis"]), _ptr(t["y_axis"]), _ptr(v)
+            d.eta = _ptr(eta) if eta is not None else None
+            d.image, d.fold = _ptr(t["image"]), _ptr(t["fold"])
+            d.trace_index = _ptr(t["trace_index"])
+            d.n_traces = int(t["trace_index"].shape[0])
+            d.nx, d.ny = nx, ny
+            keep.extend([t["x_axis"], t["y_axis"], v, eta, t["image"], t["fold"], t["trace_index"]])
+        handle = ctypes.c_void_p()
+        rc = lib.pstm_hip_plan_create(ctypes.byref(a), descs, len(tiles), ctypes.byref(handle))
+        self._check(lib, rc, "pstm_hip_plan_create")
+        return HipPlan(self, handle, keep, counter)
+
+    def _args(self, dt, image, fold2d, x_axis, y_axis, t_axis_ms, vrms, config, traveltime, trace_index, eta,
+              counter, surrogate, node_spacing):
         nx, ny, nt = (int(s) for s in image.shape)
         tt = self._traveltime(config) if traveltime is None else traveltime
         a = _abi.PstmMigrateArgs()
-        keep: list[Any] = []
+        keep: list[Any] = [dt, image, fold2d, x_axis, y_axis, t_axis_ms, vrms, counter]
         a.amplitudes = _ptr(dt.amplitudes)
         a.source_x, a.source_y = _ptr(dt.source_x), _ptr(dt.source_y)
         a.receiver_x, a.receiver_y = _ptr(dt.receiver_x), _ptr(dt.receiver_y)
@@ -199,14 +265,13 @@ class HipKernel:
         if a.vel_mode == _abi.VEL_1D and int(vrms.shape[0]) != nt:
             raise ValueError(f"1-D velocity length {int(vrms.shape[0])} != nt {nt}")
         if eta is not None:
+            keep.append(eta)
             a.eta = _ptr(eta)
             a.eta_mode = _abi.ETA_1D if eta.dim() == 1 else _abi.ETA_3D
         else:
             a.eta_mode = _abi.ETA_CONST
         a.image, a.fold = _ptr(image), _ptr(fold2d)
-        counter = None
-        if count_effective:
-            counter = torch.zeros(1, dtype=torch.int64, device=self.device)
+        if counter is not None:
             a.n_effective = _ptr(counter)
         a.n_samples = dt.n_samples
         a.nx, a.ny, a.nt = nx, ny, nt
@@ -235,19 +300,7 @@ class HipKernel:
             a.mlp_node_spacing = int(node_spacing)
             a.mlp_tau_max, a.mlp_v_lo, a.mlp_v_hi = float(n.tau_max), float(n.v_lo), float(n.v_hi)
             a.mlp_eta_max, a.mlp_out_scale = float(n.eta_max), float(n.out_scale)
-        stream = ctypes.c_void_p(torch.cuda.current_stream(self._device_index).cuda_stream)
-        if group_start is None:
-            rc = lib.pstm_hip_migrate_tile(ctypes.byref(a), stream)
-            self._check(lib, rc, "pstm_hip_migrate_tile")
-        else:
-            gs = np.ascontiguousarray(group_start, dtype=np.int32)
-            rc = lib.pstm_hip_migrate_bins(ctypes.byref(a), gs.ctypes.data_as(ctypes.c_void_p),
-                                           int(gs.shape[0] - 1), stream)
-            self._check(lib, rc, "pstm_hip_migrate_bins")
-        if counter is not None:
-            self.synchronize()
-            return int(counter.item())
-        return None
+        return a, keep
 
     # -- internals -----------------------------------------------------------
     def _require_init(self) -> None:
@@ -327,6 +380,45 @@ class HipKernel:
         return KernelMetrics(n_traces_processed=traces.n_traces,
                              n_samples_output=nx * ny * nt,
                              compute_time_s=time.perf_counter() - start)
+
+
+class HipPlan:
+    """A plan built by HipKernel.create_plan: run() migrates every tile once
+    (image and fold +=), asynchronously on the current stream."""
+
+    def __init__(self, kernel: HipKernel, handle, keep: list, counter):
+        self.kernel = kernel
+        self._h = handle
+        self._keep = keep          # device tensors the plan's pointers refer to
+        self.counter = counter     # E accumulates here when the plan counts
+
+    def run(self, use_graph: bool = True) -> None:
+        torch = _torch()
+        if self._h is None:
+            raise RuntimeError("plan destroyed")
+        lib = self.kernel._lib
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.kernel._device_index).cuda_stream)
+        HipKernel._check(lib, lib.pstm_hip_plan_run(self._h, 1 if use_graph else 0, stream), "pstm_hip_plan_run")
+
+    def info(self) -> dict:
+        lib = self.kernel._lib
+        n = ctypes.c_longlong()
+        g, mw, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        HipKernel._check(lib, lib.pstm_hip_plan_info(self._h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(mw),
+                                                     ctypes.byref(b)), "pstm_hip_plan_info")
+        return {"items": int(n.value), "grid": g.value, "mw": mw.value, "batch": b.value}
+
+    def destroy(self) -> None:
+        if self._h is not None:
+            self.kernel._lib.pstm_hip_plan_destroy(self._h)
+            self._h = None
+            self._keep = []
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def create_hip_kernel(device: int = 0) -> HipKernel:

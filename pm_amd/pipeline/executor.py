@@ -133,6 +133,19 @@ class TileExecutor:
                 total += int(e)
         return total if count_effective else None
 
+    def plan(self, tiles: list[int] | None = None, traveltime: int | None = None, surrogate=None,
+             node_spacing: int = 16, count_effective: bool = False):
+        """A device-resident work queue over the listed tiles (all by default):
+        HipPlan.run() migrates them in one persistent, hipGraph-captured gather
+        launch, heaviest tiles first.  Trace lists may hold the rows of any
+        number of common-offset bins; all of them land in the shared image."""
+        sel = range(len(self.tiles)) if tiles is None else tiles
+        descs = [dict(image=t.image, fold=t.fold, x_axis=t.x_axis, y_axis=t.y_axis, vrms=t.vrms,
+                      trace_index=t.rows, eta=t.eta) for t in (self.tiles[i] for i in sel)]
+        return self.kernel.create_plan(self.traces, descs, self.t_axis, self.config, traveltime=traveltime,
+                                       count_effective=count_effective, surrogate=surrogate,
+                                       node_spacing=node_spacing)
+
     def check_surrogate(self, surrogate, node_spacing: int = 16, n: int = 8192, bar: float = 0.25,
                         seed: int = 0, tiles: list[int] | None = None) -> list[float]:
         """Per-tile held-out check of the learned traveltime: n random

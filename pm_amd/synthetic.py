@@ -150,20 +150,31 @@ def reflector_job(seed: int = 2, n_traces: int = 200_000, n_samples: int = 1501,
                   n_out: int = 256, nt: int = 1000, d_out: float = 12.5, tile: int = 64,
                   n_planes: int = 40, n_diffractors: int = 200, antialias: str = "triangle",
                   device: str = "cpu") -> SyntheticJob:
-    import torch
-
     rng = np.random.default_rng(seed)
     x_hi = d_out * (n_out - 1)
     lo, hi = -500.0, x_hi + 500.0
     sx, sy, rx, ry = _geometry(rng, n_traces, lo, hi, 3000.0)
     cube = lateral_cube(lo, hi)
+    tr, meta = _reflector_traces(rng, (sx, sy, rx, ry), n_samples, cube, x_hi, n_planes, n_diffractors, device)
+    x = d_out * np.arange(n_out)
+    cfg = KernelConfig(max_aperture_m=2500.0, min_aperture_m=500.0, max_dip_degrees=60.0,
+                       taper_fraction=0.1, apply_spreading=True, apply_obliquity=True,
+                       grid_dx=d_out, grid_dy=d_out, antialias=antialias)
+    return SyntheticJob("reflectors", tr, sx, sy, rx, ry, 2.0, 0.0, x,
+                        x.copy(), 2.0 * np.arange(nt), cfg, cube=cube, tile=tile, meta=meta)
+
+
+def _reflector_traces(rng, geom, n_samples: int, cube, x_hi: float, n_planes: int, n_diffractors: int,
+                      device: str):
+    """Dipping planes (mirror-image time in the plane-centre velocity) and
+    point diffractors (DSR time in the local velocity), 30 Hz Ricker, f32."""
+    import torch
+
     dt_s = 0.002
     f = 30.0
-    dev = device
-    tr = torch.zeros((n_traces, n_samples), dtype=torch.float64, device=dev)
-    S = [torch.as_tensor(a, device=dev) for a in (sx, sy, rx, ry)]
+    tr = torch.zeros((len(geom[0]), n_samples), dtype=torch.float64, device=device)
+    S = [torch.as_tensor(a, device=device) for a in geom]
     xc = yc = 0.5 * x_hi
-    # dipping planes: mirror-image time in the plane-centre velocity
     dips = rng.uniform(0.0, np.deg2rad(30.0), n_planes)
     strikes = rng.uniform(0.0, 2.0 * np.pi, n_planes)
     t0c = rng.uniform(0.3, 2.8, n_planes)
@@ -179,7 +190,6 @@ def reflector_job(seed: int = 2, n_traces: int = 200_000, n_samples: int = 1501,
         mz_ = -2 * d_s * n_vec[2]
         t = torch.sqrt((mx_ - S[2]) ** 2 + (my_ - S[3]) ** 2 + mz_ ** 2) / vc[i]
         add_events(tr, t, torch.full_like(t, amps[i]), f, dt_s)
-    # point diffractors: DSR time with the local velocity
     xd = rng.uniform(0.0, x_hi, n_diffractors)
     yd = rng.uniform(0.0, x_hi, n_diffractors)
     t0d = rng.uniform(0.2, 2.8, n_diffractors)
@@ -190,13 +200,47 @@ def reflector_job(seed: int = 2, n_traces: int = 200_000, n_samples: int = 1501,
         t = (torch.sqrt(q + ((S[0] - xd[i]) ** 2 + (S[1] - yd[i]) ** 2) / vd[i] ** 2)
              + torch.sqrt(q + ((S[2] - xd[i]) ** 2 + (S[3] - yd[i]) ** 2) / vd[i] ** 2))
         add_events(tr, t, torch.full_like(t, ad[i]), f, dt_s)
+    return tr.float().contiguous(), {"diffractors": np.stack([xd, yd, t0d], 1)}
+
+
+def bin_counts(rng, n_bins: int = 40, total: int = 1_600_000, median: int = 30_000, lo: int = 5_000,
+               hi: int = 120_000) -> np.ndarray:
+    """Lognormal trace counts per common-offset bin, clip(median e^{0.6 z},
+    lo, hi), rescaled to sum to about `total`."""
+    c = np.clip(median * np.exp(0.6 * rng.standard_normal(n_bins)), lo, hi)
+    return np.maximum(1, np.round(c * total / c.sum())).astype(np.int64)
+
+
+def bins_job(seed: int = 3, n_bins: int = 40, total: int = 1_600_000, n_samples: int = 1501,
+             n_out: int = 256, nt: int = 1000, d_out: float = 12.5, tile: int = 64, bin_width: float = 50.0,
+             n_planes: int = 40, n_diffractors: int = 200, device: str = "cpu") -> SyntheticJob:
+    """Common-offset bins b = 0 .. n_bins-1 (offsets in [50 b, 50 b + 50) m,
+    uniform midpoints and azimuths) with lognormal counts, the reflector
+    job's reflectivity and velocity, plain migration (no anti-aliasing) into
+    one shared image.  Traces are stored bin by bin; meta["bin_start"] holds
+    the first row of each bin."""
+    rng = np.random.default_rng(seed)
+    counts = bin_counts(rng, n_bins, total, median=max(1, int(30_000 * total / 1_600_000)),
+                        lo=max(1, int(5_000 * total / 1_600_000)), hi=max(1, int(120_000 * total / 1_600_000)))
+    x_hi = d_out * (n_out - 1)
+    lo, hi = -500.0, x_hi + 500.0
+    parts = []
+    for b, n in enumerate(counts):
+        mx, my = rng.uniform(lo, hi, n), rng.uniform(lo, hi, n)
+        h = rng.uniform(bin_width * b, bin_width * (b + 1), n)
+        az = rng.uniform(0.0, 2.0 * np.pi, n)
+        parts.append((mx - 0.5 * h * np.cos(az), my - 0.5 * h * np.sin(az),
+                      mx + 0.5 * h * np.cos(az), my + 0.5 * h * np.sin(az)))
+    sx, sy, rx, ry = (np.concatenate([p[i] for p in parts]) for i in range(4))
+    cube = lateral_cube(lo, hi)
+    tr, meta = _reflector_traces(rng, (sx, sy, rx, ry), n_samples, cube, x_hi, n_planes, n_diffractors, device)
+    meta["bin_start"] = np.concatenate([[0], np.cumsum(counts)])
     x = d_out * np.arange(n_out)
     cfg = KernelConfig(max_aperture_m=2500.0, min_aperture_m=500.0, max_dip_degrees=60.0,
                        taper_fraction=0.1, apply_spreading=True, apply_obliquity=True,
-                       grid_dx=d_out, grid_dy=d_out, antialias=antialias)
-    return SyntheticJob("reflectors", tr.float().contiguous(), sx, sy, rx, ry, 2.0, 0.0, x,
-                        x.copy(), 2.0 * np.arange(nt), cfg, cube=cube, tile=tile,
-                        meta={"diffractors": np.stack([xd, yd, t0d], 1)})
+                       grid_dx=d_out, grid_dy=d_out, antialias="none")
+    return SyntheticJob("bins", tr, sx, sy, rx, ry, 2.0, 0.0, x, x.copy(), 2.0 * np.arange(nt), cfg,
+                        cube=cube, tile=tile, meta=meta)
 
 
 def vti_model(n_xy: int = 128, n_t: int = 201, x_lo: float = -500.0, x_hi: float = 6887.5,
