@@ -3,15 +3,18 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload reflectors]
 
 A step is one migration pass over the whole synthetic job with every input
-already resident in HBM: for the default workload, 16 tile launches (64 x 64
+already resident in HBM: for the default workload, 16 output tiles (64 x 64
 pillars x 1000 samples each) over 200,000 traces of 1501 samples with 3-D
-v_rms and triangle anti-aliasing.  A contribution is one (trace, output
-sample) pair inside the maximum aperture, C = sum of the 3-D fold; the rate
-is C / step time.  The roofline entry uses E (contributions that pass the
-time-dependent aperture and the sample bounds) x the frozen FLOPs per
-contribution over the gather kernel's own launch time, measured with HIP
-events on the launch stream.  The CPU baseline is the C++ oracle (built with
--O3 -ffast-math, OpenMP) on a bounded pillar sample of the same job.
+v_rms and triangle anti-aliasing, run as one plan (one persistent gather
+launch fed by a device work queue, replayed from a captured hipGraph).  A
+contribution is one (trace, output sample) pair inside the maximum aperture,
+C = sum of the 3-D fold; the rate is C / step time.  The roofline entry uses E
+(contributions that pass the time-dependent aperture and the sample bounds) x
+the frozen FLOPs per contribution over the gather launch's own duration,
+measured with HIP events on its stream.  The CPU baseline is the C++ oracle
+(built with -O3 -ffast-math, OpenMP) on a bounded pillar sample of the same
+job.  The learned_vs_exact entry times VTI migration with the learned and the
+exact traveltime on a reduced VTI diffractor job.
 
 Prints one JSON line.
 """
@@ -45,6 +48,13 @@ def make_job(name: str, device: str):
             "reflectors: 200000 traces x 1501 samples (2 ms), 40 dipping planes + 200 "
             "diffractors, 3-D v_rms cube 64x64x151, output 256x256 bins at 12.5 m x 1000 "
             "samples in 16 tiles of 64x64, aperture 2500 m, dip 60, triangle AA, f32")
+    if name == "bins":
+        from pm_amd.synthetic import bins_job
+
+        return bins_job(device=device), (
+            "bins: 40 common-offset bins (50 m spacing), lognormal counts (median 30k, 5k-120k), "
+            "1.6M traces x 1501 samples, reflector model, output 256x256 x 1000 samples in 16 tiles "
+            "of 64x64, one shared image, no AA, f32")
     if name == "point_diffractor":
         return point_diffractor_job(device=device), (
             "point diffractor: 4096 traces x 1001 samples, output 64x64 x 500, aperture "
@@ -86,9 +96,10 @@ def cpu_baseline(ex, job, budget_s: float) -> dict:
         res = orc.migrate(inp, threads=threads, fast=True)
         return time.perf_counter() - t0, int(res.fold2d.sum()) * len(job.t_axis_ms), res
 
-    dt1, _, _ = run(1)
+    run(1)                      # warm-up: library load, OpenMP thread start
+    dt2, _, _ = run(2)
     ny = tile.y1 - tile.y0
-    n_p = int(max(1, min(ny, round(budget_s / max(dt1, 1e-3)))))
+    n_p = int(max(1, min(ny, round(2 * budget_s / max(dt2, 1e-3)))))
     dt, c, _ = run(n_p)
     log(f"cpu baseline: {n_p} pillars x {len(rows)} traces in {dt:.1f} s on {threads} threads")
     return {"value": c / dt / 1e9, "unit": "Gcontrib/s", "cores": threads, "kind": "port",
@@ -97,21 +108,129 @@ def cpu_baseline(ex, job, budget_s: float) -> dict:
                        f"C_sample={c}, {dt:.2f} s; oracle C++ FP64 -O3 -ffast-math OpenMP")}
 
 
+# HBM bytes per gather launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per
+# dispatch scaled by true bytes / counter of the calibration kernel), per workload
+# (profiles/k_migrate_reflectors_aa.md: FETCH x 2.000 + WRITE x 1.000, one plan launch)
+TRAFFIC: dict = {"reflectors": (12_882_893_089 * 2.000 + 256_520 * 1.000) * 1024}
+
+
+def learned_vs_exact(kernel, args) -> dict:
+    """Config-4 family, reduced to fit a bench run: VTI diffractor job, the
+    surrogate trained on the job's own v_rms / eta model by the library's
+    kernels, per-tile held-out check, then one pass of every tile with the
+    exact VTI traveltime and one with the learned traveltime (each a plan:
+    one persistent graph launch), timed with HIP events; image error of the
+    learned image against the exact one, and of the isotropic image (the
+    method without the network) against it."""
+    import torch
+    from dataclasses import replace
+
+    from pm_amd.kernels import _abi
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set
+    from pm_amd.synthetic import vti_diffractor_job
+
+    t0 = time.perf_counter()
+    job = vti_diffractor_job(n_traces=args.vti_traces, n_samples=2001, n_out=args.vti_out, nt=1500, tile=64,
+                             device="cuda")
+    ex = TileExecutor(kernel, job)
+    vc, ec = job.cube, job.meta["eta_cube"]
+    d_max = job.config.max_aperture_m + 1500.0
+    norm = Normalizer(tau_max=1.5, d_max=d_max, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
+    torch.cuda.synchronize()
+    log(f"vti job ready in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    tr = make_training_set(vc, ec, args.train_samples, d_max, norm, seed=1, t_max_ms=3000.0)
+    sur = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=args.train_steps, batch=8192,
+                                                          qat_fraction=0.25, log_every=args.train_steps // 4)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del tr
+    log(f"surrogate trained in {train_s:.1f} s: {sur.history}")
+    K = args.node_spacing
+    errs = ex.check_surrogate(sur, node_spacing=K)
+    learned_tiles = [i for i, t in enumerate(ex.tiles) if t.learned_ok]
+    exact_tiles = [i for i, t in enumerate(ex.tiles) if not t.learned_ok]
+    lib = _abi.load_library()
+
+    def timed(plan_list):
+        ex.reset()
+        for p in plan_list:
+            p.run(use_graph=True)       # capture + warm
+        torch.cuda.synchronize()
+        ex.reset()
+        torch.cuda.synchronize()
+        lib.pstm_hip_profile_reset(1)
+        for p in plan_list:
+            p.run(use_graph=True)
+        kms, nl = ctypes.c_double(), ctypes.c_longlong()
+        lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
+        lib.pstm_hip_profile_reset(0)
+        img, _ = ex.assemble()
+        torch.cuda.synchronize()
+        return kms.value, img.double()
+
+    exact_plan = ex.plan()
+    ms_exact, img_exact = timed([exact_plan])
+    plans = []
+    if learned_tiles:
+        plans.append(ex.plan(tiles=learned_tiles, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=K))
+    if exact_tiles:
+        plans.append(ex.plan(tiles=exact_tiles))
+    ms_learned, img_learned = timed(plans)
+    c_total = ex.contributions()
+    saved = ex.config
+    ex.config = replace(saved, kernel_type="straight_ray", vti_enabled=False)
+    iso_plan = ex.plan()
+    _, img_iso = timed([iso_plan])
+    ex.config = saved
+    nrm = float(torch.linalg.norm(img_exact))
+    rel = float(torch.linalg.norm(img_learned - img_exact)) / nrm
+    rel_iso = float(torch.linalg.norm(img_iso - img_exact)) / nrm
+    for p in [exact_plan, iso_plan, *plans]:
+        p.destroy()
+    out = {
+        "workload": (f"VTI diffractors: {job.n_traces} traces x 2001 samples, 3-D v_rms 1500-4500 and eta "
+                     f"0-0.2 on 128x128x201 grids, output {args.vti_out}x{args.vti_out} x 1500 samples in "
+                     f"{len(ex.tiles)} tiles, aperture 2500 m"),
+        "surrogate": {"train_samples": args.train_samples, "steps": args.train_steps, "batch": 8192,
+                      "train_s": round(train_s, 1), "node_spacing": K,
+                      "heldout_max_samples_per_tile": [round(e, 4) for e in errs],
+                      "tiles_learned": len(learned_tiles), "tiles_exact_fallback": len(exact_tiles)},
+        "contributions": c_total,
+        "exact_gcontrib_s": round(c_total / (ms_exact / 1e3) / 1e9, 3),
+        "learned_gcontrib_s": round(c_total / (ms_learned / 1e3) / 1e9, 3),
+        "exact_ms": round(ms_exact, 2), "learned_ms": round(ms_learned, 2),
+        "learned_speedup": round(ms_exact / ms_learned, 3),
+        "image_rel_l2_learned_vs_exact": rel,
+        "image_rel_l2_without_network_vs_exact": rel_iso,
+    }
+    log(f"learned vs exact: {out}")
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="reflectors", choices=["reflectors", "point_diffractor"])
-    ap.add_argument("--autotune", action="store_true",
-                    help="time candidate launch shapes on the busiest tile first")
-    ap.add_argument("--candidates", default="",
-                    help="autotune candidates as MWxBATCH,MWxBATCH,... (default: a 4x3 grid)")
+    ap.add_argument("--workload", default="reflectors", choices=["reflectors", "point_diffractor", "bins"])
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip timing candidate launch shapes on the busiest tile")
+    ap.add_argument("--candidates", default="1x16,2x8,2x16,3x8,4x8",
+                    help="autotune candidates as MWxBATCH,MWxBATCH,...")
     ap.add_argument("--profile", action="store_true",
-                    help="profiler run: one untimed copy of 256 MiB (traffic calibration) "
-                         "and the timed steps only, no counting pass")
+                    help="profiler run: one elementwise kernel over 256 MiB (traffic calibration), "
+                         "then the steps as eager launches; no counting pass, no CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample (0 = skip)")
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="skip the learned-vs-exact VTI traveltime comparison")
+    ap.add_argument("--vti-traces", type=int, default=100_000)
+    ap.add_argument("--vti-out", type=int, default=128)
+    ap.add_argument("--train-samples", type=int, default=10_000_000)
+    ap.add_argument("--train-steps", type=int, default=40_000)
+    ap.add_argument("--node-spacing", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the migration shards "
@@ -139,7 +258,7 @@ def main() -> None:
     torch.cuda.synchronize()
     log(f"executor ready: {len(ex.tiles)} tiles, rows/tile {[int(t.rows.shape[0]) for t in ex.tiles][:4]}...")
     tune = None
-    if args.autotune:
+    if args.autotune and not args.profile:
         from pm_amd.autotune import autotune
 
         cands = None
@@ -147,66 +266,80 @@ def main() -> None:
             cands = [tuple(int(v) for v in c.split("x")) for c in args.candidates.split(",")]
         tune = autotune(kernel, ex, candidates=cands)
         log(f"autotune: mw={tune.mw} batch={tune.batch} ({tune.ms:.1f} ms on the busiest tile)")
+    lib = _abi.load_library()
 
     if args.profile:
-        # plain copy of known size: reference point for the HBM byte counters
-        a = torch.empty(64 * 2**20, dtype=torch.float32, device=dev).fill_(1.0)
+        # elementwise kernel of known size (a name nothing else here launches):
+        # reference point for the HBM byte counters
+        a = torch.full((64 * 2**20,), 7, dtype=torch.int32, device=dev)
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        b.copy_(a)
+        torch.bitwise_not(a, out=b)
         torch.cuda.synchronize()
-        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
+        log("calibration bitwise_not: 268435456 bytes read, 268435456 bytes written")
         del a, b
-        e_total = 0
-        c_total = 0
-    else:
-        # E (counting pass, not timed) and C
-        ex.reset()
-        e_total = ex.run(count_effective=True)
-        c_total = ex.contributions()
-        img, _ = ex.assemble()
-        assert bool(torch.isfinite(img).all()) and float(img.abs().max()) > 0, "empty or non-finite image"
-        log(f"C = {c_total:.4e} contributions, E = {e_total:.4e}")
+        plan = ex.plan()
+        for _ in range(args.steps):
+            ex.reset()
+            plan.run(use_graph=False)
+            torch.cuda.synchronize()
+            log("profile step done")
+        plan.destroy()
+        kernel.cleanup()
+        return
 
+    # E (counting plan, not timed) and C
+    cplan = ex.plan(count_effective=True)
+    ex.reset()
+    cplan.run(use_graph=False)
+    torch.cuda.synchronize()
+    e_total = int(cplan.counter.item())
+    cplan.destroy()
+    c_total = ex.contributions()
+    img, _ = ex.assemble()
+    assert bool(torch.isfinite(img).all()) and float(img.abs().max()) > 0, "empty or non-finite image"
+    log(f"C = {c_total:.4e} contributions, E = {e_total:.4e}")
+
+    plan = ex.plan()
     for _ in range(args.warmup):
         ex.reset()
-        ex.run()
+        plan.run()           # the first run captures the graph
     torch.cuda.synchronize()
-    lib = _abi.load_library()
+    # HIP events on the launch stream around every graph launch (the gather
+    # plus a few small kernels: fold add, queue reset, AA row integration)
     lib.pstm_hip_profile_reset(1)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ex.reset()
-        ex.run()
+        plan.run()
+        torch.cuda.synchronize()
         log("step done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kms = ctypes.c_double()
-    nl = ctypes.c_longlong()
-    lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
-    lib.pstm_hip_profile_reset(0)
     ms_step = elapsed * 1000.0 / args.steps
     value = c_total / (elapsed / args.steps) / 1e9
-    if args.profile:
-        log(f"profile run: {ms_step:.1f} ms per step")
-        kernel.cleanup()
-        return
+    kms, nl = ctypes.c_double(), ctypes.c_longlong()
+    lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
+    lib.pstm_hip_profile_reset(0)
+    launch = plan.info()
+    plan.destroy()
 
     cfg = job.config
     f = roofline.flops_per_contribution(cfg.kernel_type, cfg.antialias,
                                         bool(cfg.apply_spreading or cfg.apply_obliquity))
     launches = int(nl.value)
     launch_ms = kms.value / max(launches, 1)
-    e_per_launch = e_total / len(ex.tiles)
+    e_per_launch = e_total          # one gather launch covers every tile
     achieved = f * e_per_launch / (launch_ms / 1000.0) / 1e12
     peak = roofline.MI355X_FP32_VALU_TFLOPS
     rl = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-          "frac": round(achieved / peak, 4), "traffic": None,
-          "kernel": "k_migrate (gather)", "flops_per_contribution": f,
+          "frac": round(achieved / peak, 4), "traffic": TRAFFIC.get(args.workload),
+          "kernel": "k_migrate (gather, all tiles in one persistent launch; timed as its graph)",
+          "flops_per_contribution": f,
           "effective_contributions_per_launch": int(e_per_launch),
           "avg_launch_ms": round(launch_ms, 3), "launches_timed": launches,
           "frac_of_unpacked_fp32": round(achieved / roofline.MI355X_FP32_VALU_UNPACKED_TFLOPS, 4)}
+    learned = learned_vs_exact(kernel, args) if args.learned else None
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(ex, job, args.cpu_seconds)
@@ -218,10 +351,13 @@ def main() -> None:
         "config": {"workload": desc, "traces": job.n_traces, "samples": job.n_samples,
                    "output": list(ex.shape), "tiles": len(ex.tiles),
                    "contributions": c_total, "effective_contributions": e_total,
-                   "traveltime": "exact straight-ray DSR", "parallelism": "1 GPU"},
+                   "traveltime": "exact straight-ray DSR", "parallelism": "1 GPU",
+                   "launch": "one persistent hipGraph-captured gather per step over every tile"},
         "roofline": rl, "cpu_baseline": cpu,
-        "launch": _abi.last_launch(lib),
+        "launch": launch,
     }
+    if learned is not None:
+        out["learned_vs_exact"] = learned
     if tune is not None:
         out["autotune"] = tune.as_dict()
     if cpu:

@@ -648,6 +648,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     const int win = 64 * kTW * MW;
     unsigned long long cnt = 0;
     __shared__ int s_item;
+    __shared__ int s_row[kMaxBatch];  // block rows of the current batch
 
     // persistent workgroups drain the work queue; every workgroup leaves when
     // the counter passes n_items
@@ -735,6 +736,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             // (pillar, trace) pair table; the max-aperture test is the
             // oracle's f64 one (numba_cpu_optimized.py:167-173)
             int inap = 0;
+            if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
             if (pt_thread) {
                 const int pil = pil0 + pt_p;
                 float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
@@ -822,19 +824,33 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 double* dst = reinterpret_cast<double*>(rows);
                 const int n = nb * P.rowlen;
                 for (int e = tid; e < n; e += kThreads) dst[e] = src[e];
-            } else if (AA) {
-                double* dst = reinterpret_cast<double*>(rows);
-                for (int r = 0; r < nb; ++r) {
-                    const int row = T.tidx ? T.tidx[j0 + r] : j0 + r;
-                    const double* src = P.prow + (size_t)row * P.rowlen;
-                    for (int e = tid; e < P.rowlen; e += kThreads) dst[r * P.rowlen + e] = src[e];
-                }
             } else {
-                float* dst = reinterpret_cast<float*>(rows);
-                for (int r = 0; r < nb; ++r) {
-                    const int row = T.tidx ? T.tidx[j0 + r] : j0 + r;
-                    const float* src = P.amp + (size_t)row * P.ns;
-                    for (int e = tid; e < P.ns; e += kThreads) dst[r * P.rowlen + e] = src[e];
+                // rows picked through the trace list: one flat loop over the
+                // batch's nb x rowlen elements, every thread busy, each row
+                // read coalesced
+                Row* dst = reinterpret_cast<Row*>(rows);
+                const int n = nb * P.rowlen;
+                int r = 0, c = tid;
+                while (c >= P.rowlen) {
+                    c -= P.rowlen;
+                    ++r;
+                }
+                auto row_src = [&](int rr) -> const Row* {
+                    const int rw = rr < nb ? s_row[rr] : 0;
+                    if constexpr (AA) return P.prow + (size_t)rw * P.rowlen;
+                    else return P.amp + (size_t)rw * P.ns;
+                };
+                const Row* src = row_src(r);
+                for (int e = tid; e < n; e += kThreads) {
+                    dst[e] = src[c];
+                    c += kThreads;
+                    if (c >= P.rowlen) {
+                        do {
+                            c -= P.rowlen;
+                            ++r;
+                        } while (c >= P.rowlen);
+                        src = row_src(r);
+                    }
                 }
             }
             __syncthreads();
@@ -1211,7 +1227,9 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     const int nwin = (nt + 1023) / 1024;
     const int per_win = (nt + nwin - 1) / nwin;
     int mw_cap = g_tune.mw_cap > 0 ? g_tune.mw_cap : env_int("PSTM_HIP_MW");
-    if (mw_cap <= 0) mw_cap = 4;
+    // built-in choice: the largest MW whose gather keeps its registers (the
+    // anti-aliased gather spills beyond MW = 2)
+    if (mw_cap <= 0) mw_cap = sh.aa ? 2 : 4;
     mw_cap = std::min(4, mw_cap);
     sh.mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
     // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
