@@ -1,0 +1,175 @@
+"""Minimal Zarr v2 array access for trace data, no zarr package needed.
+
+The reference stores traces as a 2-D Zarr array (n_traces, n_samples),
+float32, chunked by rows, with sample_rate_ms / start_time_ms attributes
+(pstm/data/zarr_reader.py:56-301; chunks of (1000, n_samples),
+:416-436).  The on-disk format is Zarr v2: a ``.zarray`` JSON document
+(shape, chunks, dtype, compressor, fill_value, order, filters), an optional
+``.zattrs`` JSON document, and one file per chunk named ``"<i>.<j>"``
+holding the chunk's C-order bytes (full chunk shape, the edge chunk padded).
+This module reads and writes the uncompressed form (``compressor: null``).
+The reference's own files use Blosc/zstd compression, whose decoder is not
+part of this build: opening such an array raises NotImplementedError.
+"""
+
+from __future__ import annotations
+
+import json
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_DTYPES = {"<f4": np.float32, "<f8": np.float64}
+
+
+@dataclass
+class ZarrV2Array:
+    """A 2-D row-chunked Zarr v2 array on disk (traces x samples)."""
+
+    path: Path
+    shape: tuple[int, int]
+    chunk_rows: int
+    dtype: np.dtype
+    attrs: dict
+
+    @classmethod
+    def open(cls, path) -> "ZarrV2Array":
+        path = Path(path)
+        meta_path = path / ".zarray"
+        if not meta_path.exists():
+            raise FileNotFoundError(f"Zarr array not found: {path}")
+        meta = json.loads(meta_path.read_text())
+        if meta.get("zarr_format") != 2:
+            raise ValueError(f"{path}: zarr_format {meta.get('zarr_format')} (expected 2)")
+        if meta.get("compressor") is not None or meta.get("filters"):
+            raise NotImplementedError(
+                f"{path}: compressed or filtered chunks ({meta.get('compressor')}); only uncompressed "
+                "Zarr v2 arrays are read here")
+        if meta.get("order", "C") != "C":
+            raise NotImplementedError(f"{path}: order {meta.get('order')} (expected C)")
+        dt = meta.get("dtype")
+        if dt not in _DTYPES:
+            raise NotImplementedError(f"{path}: dtype {dt} (expected <f4 or <f8)")
+        shape = tuple(int(s) for s in meta["shape"])
+        chunks = tuple(int(c) for c in meta["chunks"])
+        if len(shape) != 2 or len(chunks) != 2:
+            raise ValueError(f"{path}: expected a 2-D array, got shape {shape}")
+        if chunks[1] != shape[1]:
+            raise NotImplementedError(f"{path}: chunks {chunks} split the sample axis; expected whole rows")
+        if meta.get("dimension_separator", ".") != ".":
+            raise NotImplementedError(f"{path}: dimension_separator {meta['dimension_separator']!r}")
+        attrs_path = path / ".zattrs"
+        attrs = json.loads(attrs_path.read_text()) if attrs_path.exists() else {}
+        return cls(path, shape, chunks[0], np.dtype(_DTYPES[dt]), attrs)
+
+    @property
+    def n_traces(self) -> int:
+        return self.shape[0]
+
+    @property
+    def n_samples(self) -> int:
+        return self.shape[1]
+
+    @property
+    def n_chunks(self) -> int:
+        return (self.shape[0] + self.chunk_rows - 1) // self.chunk_rows
+
+    def chunk_range(self, i: int) -> tuple[int, int]:
+        r0 = i * self.chunk_rows
+        return r0, min(r0 + self.chunk_rows, self.shape[0])
+
+    def chunk_file(self, i: int) -> Path:
+        return self.path / f"{i}.0"
+
+    def read_chunk_into(self, i: int, out: np.ndarray) -> int:
+        """Read the valid rows of chunk i into ``out`` (C-contiguous, at least
+        rows x n_samples of this array's dtype; a missing chunk file is the
+        fill value 0).  Returns the number of rows.  Releases the GIL while
+        reading, so several threads read in parallel."""
+        r0, r1 = self.chunk_range(i)
+        n = (r1 - r0) * self.shape[1]
+        flat = out.reshape(-1)[:n]
+        if flat.dtype != self.dtype or not flat.flags.c_contiguous:
+            raise ValueError("output buffer must be C-contiguous with the array's dtype")
+        f = self.chunk_file(i)
+        if not f.exists():
+            flat[:] = 0
+            return r1 - r0
+        view = memoryview(flat.view(np.uint8))
+        with open(f, "rb", buffering=0) as fh:
+            got = 0
+            while got < view.nbytes:
+                k = fh.readinto(view[got:])
+                if not k:
+                    raise IOError(f"{f}: short chunk ({got} of {view.nbytes} bytes)")
+                got += k
+        return r1 - r0
+
+    def read_rows(self, r0: int, r1: int) -> np.ndarray:
+        """Rows [r0, r1) as a new array (host; for tests and small reads)."""
+        out = np.empty((r1 - r0, self.shape[1]), self.dtype)
+        buf = np.empty((self.chunk_rows, self.shape[1]), self.dtype)
+        for i in range(r0 // self.chunk_rows, (r1 - 1) // self.chunk_rows + 1 if r1 > r0 else 0):
+            c0, c1 = self.chunk_range(i)
+            self.read_chunk_into(i, buf)
+            a, b = max(r0, c0), min(r1, c1)
+            out[a - r0:b - r0] = buf[a - c0:b - c0]
+        return out
+
+
+def write_zarr_v2(path, data, chunk_rows: int = 1000, attrs: dict | None = None) -> ZarrV2Array:
+    """Write a 2-D float32 array (NumPy, or a torch tensor on any device,
+    written chunk by chunk) as an uncompressed Zarr v2 array."""
+    path = Path(path)
+    path.mkdir(parents=True, exist_ok=True)
+    n, ns = (int(s) for s in data.shape)
+    meta = {"zarr_format": 2, "shape": [n, ns], "chunks": [chunk_rows, ns], "dtype": "<f4",
+            "compressor": None, "fill_value": 0.0, "order": "C", "filters": None}
+    (path / ".zarray").write_text(json.dumps(meta, indent=1))
+    if attrs:
+        (path / ".zattrs").write_text(json.dumps(attrs, indent=1))
+    pad = np.zeros((chunk_rows, ns), np.float32)
+    for i in range((n + chunk_rows - 1) // chunk_rows):
+        r0, r1 = i * chunk_rows, min((i + 1) * chunk_rows, n)
+        blk = data[r0:r1]
+        blk = blk.detach().cpu().numpy() if hasattr(blk, "detach") else np.asarray(blk)
+        pad[: r1 - r0] = blk
+        pad[r1 - r0:] = 0.0
+        with open(path / f"{i}.0", "wb") as fh:
+            fh.write(pad.tobytes())
+    return ZarrV2Array.open(path)
+
+
+#: header columns, as the reference's ColumnMapping defaults (pstm/config/models.py:135-151)
+HEADER_COLUMNS = ("trace_idx", "SOU_X", "SOU_Y", "REC_X", "REC_Y")
+
+
+def write_headers(path, source_x, source_y, receiver_x, receiver_y, extra: dict | None = None) -> None:
+    """Trace headers as a Parquet file with the reference's default column
+    names (plus optional extra columns, e.g. an offset-bin number)."""
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+
+    n = len(source_x)
+    cols = {"trace_idx": np.arange(n, dtype=np.int64), "SOU_X": np.asarray(source_x, np.float64),
+            "SOU_Y": np.asarray(source_y, np.float64), "REC_X": np.asarray(receiver_x, np.float64),
+            "REC_Y": np.asarray(receiver_y, np.float64)}
+    for k, v in (extra or {}).items():
+        cols[k] = np.asarray(v)
+    pq.write_table(pa.table(cols), str(path))
+
+
+def read_headers(path) -> dict:
+    """Columns of a header Parquet file as NumPy arrays, in trace_idx order."""
+    import pyarrow.parquet as pq
+
+    t = pq.read_table(str(path))
+    missing = [c for c in HEADER_COLUMNS if c not in t.column_names]
+    if missing:
+        raise ValueError(f"{path}: missing required columns {missing}")
+    cols = {c: t.column(c).to_numpy() for c in t.column_names}
+    order = np.argsort(cols["trace_idx"], kind="stable")
+    if not np.array_equal(cols["trace_idx"][order], np.arange(len(order))):
+        raise ValueError(f"{path}: trace_idx must be 0 .. n-1")
+    return {k: v[order] for k, v in cols.items()}

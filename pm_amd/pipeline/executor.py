@@ -63,9 +63,10 @@ class TileExecutor:
         mx, my = job.midpoint_x, job.midpoint_y
         perm = morton_order(mx, my) if spatial_sort else np.arange(len(mx))
         self.perm = perm
-        pt = torch.as_tensor(perm, device=dev)
         amp = job.amplitudes if job.amplitudes.is_cuda else job.amplitudes.to(dev)
-        amp = amp.index_select(0, pt).contiguous()
+        if spatial_sort:
+            amp = amp.index_select(0, torch.as_tensor(perm, device=dev)).contiguous()
+        # else: the job's own tensor (a streaming destination is filled later)
 
         def d(a):
             return torch.as_tensor(np.ascontiguousarray(np.asarray(a, dtype=np.float64)[perm]), dtype=f64,
@@ -134,14 +135,21 @@ class TileExecutor:
         return total if count_effective else None
 
     def plan(self, tiles: list[int] | None = None, traveltime: int | None = None, surrogate=None,
-             node_spacing: int = 16, count_effective: bool = False):
+             node_spacing: int = 16, count_effective: bool = False, row_range: tuple[int, int] | None = None):
         """A device-resident work queue over the listed tiles (all by default):
         HipPlan.run() migrates them in one persistent, hipGraph-captured gather
         launch, heaviest tiles first.  Trace lists may hold the rows of any
-        number of common-offset bins; all of them land in the shared image."""
+        number of common-offset bins; all of them land in the shared image.
+        ``row_range`` (r0, r1) keeps only the executor rows in [r0, r1) (one
+        group of a streamed job)."""
         sel = range(len(self.tiles)) if tiles is None else tiles
-        descs = [dict(image=t.image, fold=t.fold, x_axis=t.x_axis, y_axis=t.y_axis, vrms=t.vrms,
-                      trace_index=t.rows, eta=t.eta) for t in (self.tiles[i] for i in sel)]
+        descs = []
+        for t in (self.tiles[i] for i in sel):
+            rows = t.rows
+            if row_range is not None:
+                rows = rows[(rows >= row_range[0]) & (rows < row_range[1])].contiguous()
+            descs.append(dict(image=t.image, fold=t.fold, x_axis=t.x_axis, y_axis=t.y_axis, vrms=t.vrms,
+                              trace_index=rows, eta=t.eta))
         return self.kernel.create_plan(self.traces, descs, self.t_axis, self.config, traveltime=traveltime,
                                        count_effective=count_effective, surrogate=surrogate,
                                        node_spacing=node_spacing)
