@@ -65,6 +65,7 @@ constexpr int kWaves = kTW * kPG;      // 16 waves
 constexpr int kThreads = 64 * kWaves;  // 1024 threads
 constexpr int kNPil = kPW * kPG;       // 8 pillars per workgroup
 constexpr int kMaxBatch = 16;          // traces per LDS batch (upper bound)
+constexpr int kMaxHalfWidth = 64;      // largest anti-aliasing half width (aa_max_half_width)
 constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS uses the rest
 
 // per-(pillar, sample) terms, f32, built once per call in f64
@@ -551,8 +552,10 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         const float a1 = fmaf(e.x, cs, q);
         const float a2 = fmaf(e.y, cs, q);
         if (AA) {
-            y1 = a1 > 0.f ? frsq(a1) : 0.f;
-            y2 = a2 > 0.f ? frsq(a2) : 0.f;
+            // a >= 0; at a = 0 the leg is 0 (a * y = 0) and so is its dip term
+            // (the offsets multiplying y are 0 there)
+            y1 = frsq(fmaxf(a1, 1e-30f));
+            y2 = frsq(fmaxf(a2, 1e-30f));
             t = a1 * y1 + a2 * y2;
         } else {
             t = fsqrt(a1) + fsqrt(a2);
@@ -592,7 +595,7 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
 // difference interpolated between i0 and i0+1.
 template <bool AA, typename Row>
 __device__ __forceinline__ float contrib_amp(const KParams& P, const Row* row, int i0, float f0, const float4& f,
-                                             float cs, float y1, float y2) {
+                                             float cs, float y1, float y2, const float2* invsq) {
     if (!AA) {
         const float s0 = row[i0];
         const float s1 = row[i0 + 1];
@@ -608,12 +611,14 @@ __device__ __forceinline__ float contrib_amp(const KParams& P, const Row* row, i
         const double am1 = r0[i0 - lf - 1], am0 = r0[i0 - lf], am1p = r0[i0 - lf + 1];
         const double c0 = r0[i0], c1 = r0[i0 + 1];
         const double b0 = r0[i0 + lf], b1 = r0[i0 + lf + 1], b2 = r0[i0 + lf + 2];
-        const float dl0 = (float)((b0 - c0) - (c0 - am0));   // T_lf[i0] * lf^2
-        const float dh0 = (float)((b1 - c0) - (c0 - am1));   // T_lf+1[i0] * (lf+1)^2
-        const float dl1 = (float)((b1 - c1) - (c1 - am1p));  // T_lf[i0+1] * lf^2
-        const float dh1 = (float)((b2 - c1) - (c1 - am0));   // T_lf+1[i0+1] * (lf+1)^2
-        const float wl = (1.0f - lam) * frcp((float)(lf * lf));
-        const float wh = lam * frcp((float)((lf + 1) * (lf + 1)));
+        // second differences P[n + L] - 2 P[n] + P[n - L] (one add, one fma in f64)
+        const float dl0 = (float)fma(-2.0, c0, b0 + am0);   // T_lf[i0] * lf^2
+        const float dh0 = (float)fma(-2.0, c0, b1 + am1);   // T_lf+1[i0] * (lf+1)^2
+        const float dl1 = (float)fma(-2.0, c1, b1 + am1p);  // T_lf[i0+1] * lf^2
+        const float dh1 = (float)fma(-2.0, c1, b2 + am0);   // T_lf+1[i0+1] * (lf+1)^2
+        const float2 iw = invsq[lf];                        // {1 / lf^2, 1 / (lf + 1)^2}
+        const float wl = (1.0f - lam) * iw.x;
+        const float wh = lam * iw.y;
         const float tr0 = fmaf(dl0, wl, dh0 * wh);
         const float tr1 = fmaf(dl1, wl, dh1 * wh);
         return fmaf(f0, tr1 - tr0, tr0);
@@ -649,6 +654,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     unsigned long long cnt = 0;
     __shared__ int s_item;
     __shared__ int s_row[kMaxBatch];  // block rows of the current batch
+    __shared__ float2 s_invsq[kMaxHalfWidth + 2];  // AA: {1 / l^2, 1 / (l + 1)^2}
+    if (AA) {
+        for (int i = tid; i < kMaxHalfWidth + 2; i += kThreads)
+            s_invsq[i] = make_float2(i > 0 ? 1.0f / (float)(i * i) : 0.0f, 1.0f / (float)((i + 1) * (i + 1)));
+    }  // visible after the first barrier of the item loop
 
     // persistent workgroups drain the work queue; every workgroup leaves when
     // the counter passes n_items
@@ -922,7 +932,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         const float us = ok[m] ? u[m] : 0.0f;
                         const int i0 = (int)us;
                         const float f0 = __builtin_amdgcn_fractf(us);
-                        const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m]);
+                        const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m], s_invsq);
                         float w = ok[m] ? tw[m] : 0.0f;
                         if (P.wpow != 0) {
                             const float r = frcp(t[m]);
@@ -964,7 +974,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             if (exact_decision<TT>(P, T, pil, trow, k, &u64)) {
                                 const int i0 = (int)u64;
                                 const float f0 = (float)(u64 - (double)i0);
-                                acc[p][m] = fmaf(contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2),
+                                acc[p][m] = fmaf(contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq),
                                                  contrib_weight(P, e.w, xt, t, cw[p][m]), acc[p][m]);
                                 if (COUNT) ++cnt;
                             }
@@ -1108,7 +1118,7 @@ int validate(const PstmMigrateArgs* a) {
     if (a->aa_mode != PSTM_AA_NONE && a->aa_mode != PSTM_AA_TRIANGLE) return fail(1, "invalid aa_mode");
     if (a->aa_mode == PSTM_AA_TRIANGLE && a->traveltime != PSTM_TT_STRAIGHT)
         return fail(1, "triangle anti-aliasing is implemented for the straight-ray traveltime only");
-    if (a->aa_mode == PSTM_AA_TRIANGLE && (a->aa_max_half_width < 1 || a->aa_max_half_width > 64))
+    if (a->aa_mode == PSTM_AA_TRIANGLE && (a->aa_max_half_width < 1 || a->aa_max_half_width > kMaxHalfWidth))
         return fail(1, "aa_max_half_width must be in [1, 64]");
     if (a->traveltime == PSTM_TT_VTI || a->traveltime == PSTM_TT_VTI_LEARNED) {
         if (a->eta_mode < PSTM_ETA_CONST || a->eta_mode > PSTM_ETA_3D) return fail(1, "invalid eta_mode");
