@@ -11,6 +11,7 @@ from __future__ import annotations
 import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -36,20 +37,34 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> Path:
-    OUT.parent.mkdir(parents=True, exist_ok=True)
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(REPO / "include"),
-           *map(str, SOURCES), "-o", str(tmp)]
+def _run(cmd: list[str], verbose: bool) -> subprocess.Popen:
     if verbose:
         print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+
+
+def _check(p: subprocess.Popen, verbose: bool) -> None:
+    _, stderr = p.communicate()
     noise = "is not a recognized feature for this target"
-    err = "\n".join(l for l in res.stderr.splitlines() if noise not in l)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
+    err = "\n".join(l for l in stderr.splitlines() if noise not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({p.returncode}):\n{err}")
     if err.strip() and verbose:
         print(err, file=sys.stderr)
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every source to an object in parallel, then link the library."""
+    OUT.parent.mkdir(parents=True, exist_ok=True)
+    tmp = OUT.with_suffix(".so.tmp")
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="pm_amd_build_") as d:
+        objs = [Path(d) / (src.stem + ".o") for src in SOURCES]
+        procs = [_run([hipcc(), *flags, "-I", str(REPO / "include"), "-c", str(src), "-o", str(obj)], verbose)
+                 for src, obj in zip(SOURCES, objs)]
+        for p in procs:
+            _check(p, verbose)
+        _check(_run([hipcc(), *HIPCC_FLAGS, *map(str, objs), "-o", str(tmp)], verbose), verbose)
     tmp.replace(OUT)
     return OUT
 

@@ -601,12 +601,13 @@ __device__ __forceinline__ float contrib_amp(const KParams& P, const Row* row, i
         const float s1 = row[i0 + 1];
         return fmaf(f0, s1 - s0, s0);
     } else {
-        const float gx = cs * fmaf(f.x, y1, f.z * y2);
-        const float gy = cs * fmaf(f.y, y1, f.w * y2);
-        float L = fmaxf(fabsf(gx) * P.gdx_idt, fabsf(gy) * P.gdy_idt);
-        L = fminf(fmaxf(L, 1.0f), P.lmax);
+        // half width L = operator time dip x bin size in samples (offsets in f
+        // carry the bin size / dt), clamped to [1, lmax]
+        const float gx = fmaf(f.x, y1, f.z * y2);
+        const float gy = fmaf(f.y, y1, f.w * y2);
+        const float L = __builtin_amdgcn_fmed3f(cs * fmaxf(fabsf(gx), fabsf(gy)), 1.0f, P.lmax);
         const int lf = (int)L;
-        const float lam = L - (float)lf;
+        const float lam = __builtin_amdgcn_fractf(L);
         const Row* r0 = row + P.lo;
         const double am1 = r0[i0 - lf - 1], am0 = r0[i0 - lf], am1p = r0[i0 - lf + 1];
         const double c0 = r0[i0], c1 = r0[i0 + 1];
@@ -637,7 +638,9 @@ __device__ __forceinline__ float contrib_weight(const KParams& P, float wtr, flo
     return w;
 }
 
-template <int MW, int TT, bool AA, bool COUNT, bool LOB>
+// W2: spreading and obliquity both on (weight factor cw / t^2, the common case)
+// as a compile-time constant; otherwise the power of 1/t (0 or 1) is read from P
+template <int MW, int TT, bool AA, bool COUNT, bool LOB, bool W2>
 __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     using Row = typename std::conditional<AA, double, float>::type;
@@ -776,7 +779,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             f = make_float4(1.0f / (den_s * den_r), 0.0f, 0.0f, 0.0f);
                         } else {
                             e = make_float4(ds2, dr2, dm, g1.z);
-                            f = make_float4(dxs, dys, dxr, dyr);
+                            // AA: offsets pre-scaled by bin size / dt, so the operator
+                            // dip in samples along x is cs * (f.x y1 + f.z y2)
+                            f = make_float4(dxs * P.gdx_idt, dys * P.gdy_idt, dxr * P.gdx_idt, dyr * P.gdy_idt);
                         }
                     }
                 }
@@ -934,12 +939,12 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         const float f0 = __builtin_amdgcn_fractf(us);
                         const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m], s_invsq);
                         float w = ok[m] ? tw[m] : 0.0f;
-                        if (P.wpow != 0) {
+                        if (W2 || P.wpow != 0) {
                             const float r = frcp(t[m]);
-                            const float wf = cw[p][m] * (P.wpow == 2 ? r * r : r);
+                            float wf = cw[p][m] * ((W2 || P.wpow == 2) ? r * r : r);
                             // t > 1 ms is certain unless the block holds t0 <= 2 ms (early)
-                            if (early & (1u << m)) w = t[m] > 0.001f ? w * wf : w;
-                            else w *= wf;
+                            if (early & (1u << m)) wf = t[m] > 0.001f ? wf : 1.0f;
+                            w *= wf;
                         }
                         acc[p][m] = fmaf(amp, w, acc[p][m]);
                         if (COUNT) cnt += ok[m] ? 1 : 0;
@@ -1144,9 +1149,9 @@ int validate(const PstmMigrateArgs* a) {
 // nothing but stream work happens while a plan's launch sequence is captured
 bool g_attr_only = false;
 
-template <int MW, int TT, bool AA, bool COUNT, bool LOB>
+template <int MW, int TT, bool AA, bool COUNT, bool LOB, bool W2>
 int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
-    auto fn = k_migrate<MW, TT, AA, COUNT, LOB>;
+    auto fn = k_migrate<MW, TT, AA, COUNT, LOB, W2>;
     static size_t lds_set = 0;  // per instantiation
     if (lds > lds_set) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1169,14 +1174,20 @@ int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
     return 0;
 }
 
+template <int TT, bool AA, bool COUNT, bool LOB, bool W2>
+int dispatch_mw4(int mw, const KParams& kp, int grid, size_t lds, hipStream_t st) {
+    switch (mw) {
+        case 1: return launch_migrate<1, TT, AA, COUNT, LOB, W2>(kp, grid, lds, st);
+        case 2: return launch_migrate<2, TT, AA, COUNT, LOB, W2>(kp, grid, lds, st);
+        case 3: return launch_migrate<3, TT, AA, COUNT, LOB, W2>(kp, grid, lds, st);
+        default: return launch_migrate<4, TT, AA, COUNT, LOB, W2>(kp, grid, lds, st);
+    }
+}
+
 template <int TT, bool AA, bool COUNT, bool LOB>
 int dispatch_mw3(int mw, const KParams& kp, int grid, size_t lds, hipStream_t st) {
-    switch (mw) {
-        case 1: return launch_migrate<1, TT, AA, COUNT, LOB>(kp, grid, lds, st);
-        case 2: return launch_migrate<2, TT, AA, COUNT, LOB>(kp, grid, lds, st);
-        case 3: return launch_migrate<3, TT, AA, COUNT, LOB>(kp, grid, lds, st);
-        default: return launch_migrate<4, TT, AA, COUNT, LOB>(kp, grid, lds, st);
-    }
+    return kp.wpow == 2 ? dispatch_mw4<TT, AA, COUNT, LOB, true>(mw, kp, grid, lds, st)
+                        : dispatch_mw4<TT, AA, COUNT, LOB, false>(mw, kp, grid, lds, st);
 }
 
 template <int TT, bool AA>
