@@ -870,123 +870,131 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
             __syncthreads();
 
-            for (int jj = 0; jj < nb; ++jj) {
-                const Row* row = rows + (size_t)jj * P.rowlen;
-                bool any_near = false;  // some slot within eps of a threshold (rare)
-#pragma unroll
-                for (int p = 0; p < kPW; ++p) {
-                    const int pi = (pg * kPW + p) * P.batch + jj;
-                    const float4 e = pairs[pi * kPairVec];
-                    if (e.z == INFINITY) continue;  // pair outside max aperture (wave-uniform)
-                    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
-                    // Every slot of the pair is evaluated without per-slot branches so
-                    // that the MW independent chains (sqrt, LDS reads) interleave;
-                    // slots outside the aperture or the trace read a safe sample and
-                    // add zero.
-                    float xt[MW];
-                    bool any_in = false;
-#pragma unroll
-                    for (int m = 0; m < MW; ++m) {
-                        xt[m] = fmaf(e.z, ci[p][m], P.taper_off);
-                        any_in |= xt[m] <= 1.0f + P.eps_x;
-                    }
-                    if (!any_in) continue;  // time-dependent aperture excludes every slot
-                    float t[MW], u[MW], y1[MW], y2[MW];
-                    bool ok[MW];
-                    if (TT == PSTM_TT_VTI_LEARNED) {
-                        const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * P.lat_nodes;
-#pragma unroll
-                        for (int m = 0; m < MW; ++m) {
-                            const float2 c0 = lp[nidx[m]], c1 = lp[nidx[m] + 1];
-                            e2[p][m] = fmaf(nfr, c1.x - c0.x, c0.x);
-                            e1[p][m] = fmaf(nfr, c1.y - c0.y, c0.y);
-                        }
-                    }
-#pragma unroll
-                    for (int m = 0; m < MW; ++m) {
-                        t[m] = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m], y1[m],
-                                                    y2[m]);
-                        u[m] = fmaf(t[m], P.idt, P.uoff);
-                        // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
-                        const bool in_ap = xt[m] <= ((COUNT && TT != PSTM_TT_VTI_LEARNED) ? 1.0f - P.eps_x : 1.0f);
-                        const bool below = u[m] < P.umax_lo;
-                        bool near = in_ap && !below && u[m] < P.umax_hi;
-                        if (LOB) near = near || (in_ap && u[m] < P.lo_eps && u[m] > -P.lo_eps);
-                        if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
-                        // the learned time has no f64 counterpart to decide against
-                        if (TT == PSTM_TT_VTI_LEARNED) near = false;
-                        any_near |= near;
-                        ok[m] = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u[m] < P.umax : below) && !near;
-                        if (LOB) ok[m] = ok[m] && u[m] >= 0.0f;
-                    }
-                    float tw[MW];
-                    bool any_taper = false;
-#pragma unroll
-                    for (int m = 0; m < MW; ++m) {
-                        tw[m] = e.w;
-                        any_taper |= ok[m] && xt[m] > 0.0f;
-                    }
-                    if (any_taper) {
-#pragma unroll
-                        for (int m = 0; m < MW; ++m)
-                            if (xt[m] > 0.0f) tw[m] = e.w * taper_weight(fminf(xt[m], 1.0f));
-                    }
-#pragma unroll
-                    for (int m = 0; m < MW; ++m) {
-                        const float us = ok[m] ? u[m] : 0.0f;
-                        const int i0 = (int)us;
-                        const float f0 = __builtin_amdgcn_fractf(us);
-                        const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m], s_invsq);
-                        float w = ok[m] ? tw[m] : 0.0f;
-                        if (W2 || P.wpow != 0) {
-                            const float r = frcp(t[m]);
-                            float wf = cw[p][m] * ((W2 || P.wpow == 2) ? r * r : r);
-                            // t > 1 ms is certain unless the block holds t0 <= 2 ms (early)
-                            if (early & (1u << m)) wf = t[m] > 0.001f ? wf : 1.0f;
-                            w *= wf;
-                        }
-                        acc[p][m] = fmaf(amp, w, acc[p][m]);
-                        if (COUNT) cnt += ok[m] ? 1 : 0;
-                    }
-                }
-                // slots near a threshold: recompute which, decide in f64 as the
-                // oracle does (rare; the slots were left out above)
-                if (TT != PSTM_TT_VTI_LEARNED && any_near) {
-#pragma unroll
+            // the traces of the batch; EARLY: this wave holds t0 <= 2 ms samples,
+            // where t <= 1 ms can switch the spreading/obliquity weights off
+            // (wave-uniform, rare: one copy of the loop without that test)
+            auto trace_loop = [&](auto early_c) {
+                constexpr bool EARLY = decltype(early_c)::value;
+                for (int jj = 0; jj < nb; ++jj) {
+                    const Row* row = rows + (size_t)jj * P.rowlen;
+                    bool any_near = false;  // some slot within eps of a threshold (rare)
+    #pragma unroll
                     for (int p = 0; p < kPW; ++p) {
                         const int pi = (pg * kPW + p) * P.batch + jj;
                         const float4 e = pairs[pi * kPairVec];
-                        if (e.z == INFINITY) continue;
+                        if (e.z == INFINITY) continue;  // pair outside max aperture (wave-uniform)
                         float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
                         if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
-#pragma unroll
+                        // Every slot of the pair is evaluated without per-slot branches so
+                        // that the MW independent chains (sqrt, LDS reads) interleave;
+                        // slots outside the aperture or the trace read a safe sample and
+                        // add zero.
+                        float xt[MW];
+                        bool any_in = false;
+    #pragma unroll
                         for (int m = 0; m < MW; ++m) {
-                            const float xt = fmaf(e.z, ci[p][m], P.taper_off);
-                            float y1, y2;
-                            const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m],
-                                                                 t0r[m], y1, y2);
-                            const float u = fmaf(t, P.idt, P.uoff);
-                            const bool in_ap = xt <= (COUNT ? 1.0f - P.eps_x : 1.0f);
-                            bool near = in_ap && u >= P.umax_lo && u < P.umax_hi;
-                            if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
-                            if (COUNT) near = near || (fabsf(xt - 1.0f) < P.eps_x);
-                            if (!near) continue;
-                            const int pil = pil0 + pg * kPW + p;
-                            const int k = kbase + 64 * (tw + kTW * m) + lane;
-                            const int trow = T.tidx ? T.tidx[j0 + jj] : j0 + jj;
-                            double u64;
-                            if (exact_decision<TT>(P, T, pil, trow, k, &u64)) {
-                                const int i0 = (int)u64;
-                                const float f0 = (float)(u64 - (double)i0);
-                                acc[p][m] = fmaf(contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq),
-                                                 contrib_weight(P, e.w, xt, t, cw[p][m]), acc[p][m]);
-                                if (COUNT) ++cnt;
+                            xt[m] = fmaf(e.z, ci[p][m], P.taper_off);
+                            any_in |= xt[m] <= 1.0f + P.eps_x;
+                        }
+                        if (!any_in) continue;  // time-dependent aperture excludes every slot
+                        float t[MW], u[MW], y1[MW], y2[MW];
+                        bool ok[MW];
+                        if (TT == PSTM_TT_VTI_LEARNED) {
+                            const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * P.lat_nodes;
+    #pragma unroll
+                            for (int m = 0; m < MW; ++m) {
+                                const float2 c0 = lp[nidx[m]], c1 = lp[nidx[m] + 1];
+                                e2[p][m] = fmaf(nfr, c1.x - c0.x, c0.x);
+                                e1[p][m] = fmaf(nfr, c1.y - c0.y, c0.y);
+                            }
+                        }
+    #pragma unroll
+                        for (int m = 0; m < MW; ++m) {
+                            t[m] = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m], y1[m],
+                                                        y2[m]);
+                            u[m] = fmaf(t[m], P.idt, P.uoff);
+                            // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
+                            const bool in_ap = xt[m] <= ((COUNT && TT != PSTM_TT_VTI_LEARNED) ? 1.0f - P.eps_x : 1.0f);
+                            const bool below = u[m] < P.umax_lo;
+                            bool near = in_ap && !below && u[m] < P.umax_hi;
+                            if (LOB) near = near || (in_ap && u[m] < P.lo_eps && u[m] > -P.lo_eps);
+                            if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
+                            // the learned time has no f64 counterpart to decide against
+                            if (TT == PSTM_TT_VTI_LEARNED) near = false;
+                            any_near |= near;
+                            ok[m] = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u[m] < P.umax : below) && !near;
+                            if (LOB) ok[m] = ok[m] && u[m] >= 0.0f;
+                        }
+                        float tw[MW];
+                        bool any_taper = false;
+    #pragma unroll
+                        for (int m = 0; m < MW; ++m) {
+                            tw[m] = e.w;
+                            any_taper |= ok[m] && xt[m] > 0.0f;
+                        }
+                        if (any_taper) {
+    #pragma unroll
+                            for (int m = 0; m < MW; ++m)
+                                if (xt[m] > 0.0f) tw[m] = e.w * taper_weight(fminf(xt[m], 1.0f));
+                        }
+    #pragma unroll
+                        for (int m = 0; m < MW; ++m) {
+                            const float us = ok[m] ? u[m] : 0.0f;
+                            const int i0 = (int)us;
+                            const float f0 = __builtin_amdgcn_fractf(us);
+                            const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m], s_invsq);
+                            float w = ok[m] ? tw[m] : 0.0f;
+                            if (W2 || P.wpow != 0) {
+                                const float r = frcp(t[m]);
+                                float wf = cw[p][m] * ((W2 || P.wpow == 2) ? r * r : r);
+                                // t > 1 ms is certain unless the block holds t0 <= 2 ms (early)
+                                if (EARLY && (early & (1u << m))) wf = t[m] > 0.001f ? wf : 1.0f;
+                                w *= wf;
+                            }
+                            acc[p][m] = fmaf(amp, w, acc[p][m]);
+                            if (COUNT) cnt += ok[m] ? 1 : 0;
+                        }
+                    }
+                    // slots near a threshold: recompute which, decide in f64 as the
+                    // oracle does (rare; the slots were left out above)
+                    if (TT != PSTM_TT_VTI_LEARNED && any_near) {
+    #pragma unroll
+                        for (int p = 0; p < kPW; ++p) {
+                            const int pi = (pg * kPW + p) * P.batch + jj;
+                            const float4 e = pairs[pi * kPairVec];
+                            if (e.z == INFINITY) continue;
+                            float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+                            if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
+    #pragma unroll
+                            for (int m = 0; m < MW; ++m) {
+                                const float xt = fmaf(e.z, ci[p][m], P.taper_off);
+                                float y1, y2;
+                                const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m],
+                                                                     t0r[m], y1, y2);
+                                const float u = fmaf(t, P.idt, P.uoff);
+                                const bool in_ap = xt <= (COUNT ? 1.0f - P.eps_x : 1.0f);
+                                bool near = in_ap && u >= P.umax_lo && u < P.umax_hi;
+                                if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
+                                if (COUNT) near = near || (fabsf(xt - 1.0f) < P.eps_x);
+                                if (!near) continue;
+                                const int pil = pil0 + pg * kPW + p;
+                                const int k = kbase + 64 * (tw + kTW * m) + lane;
+                                const int trow = T.tidx ? T.tidx[j0 + jj] : j0 + jj;
+                                double u64;
+                                if (exact_decision<TT>(P, T, pil, trow, k, &u64)) {
+                                    const int i0 = (int)u64;
+                                    const float f0 = (float)(u64 - (double)i0);
+                                    acc[p][m] = fmaf(contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq),
+                                                     contrib_weight(P, e.w, xt, t, cw[p][m]), acc[p][m]);
+                                    if (COUNT) ++cnt;
+                                }
                             }
                         }
                     }
                 }
-            }
+            };
+            if (early) trace_loop(std::true_type{});
+            else trace_loop(std::false_type{});
 #pragma unroll
             for (int p = 0; p < kPW; ++p)
 #pragma unroll
