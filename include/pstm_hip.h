@@ -158,6 +158,12 @@ int pstm_hip_profile_read(double* total_ms, long long* launches);
  * heuristic is host-side, pstm/pipeline/tile_planner.py:26-88). */
 int pstm_hip_set_tuning(int mw_cap, int batch_cap);
 
+/* Pipelined staging of trace rows (the next batch's rows are loaded into
+ * registers while the current batch computes; the batch is capped so that
+ * its rows fit): -1 automatic (on for f32 rows, off with anti-aliasing), 0 off,
+ * 1 on.  Used by the autotuner; no counterpart in the reference. */
+int pstm_hip_set_pipeline(int mode);
+
 /* Shape of the most recent gather launch: mw, traces per batch, workgroups
  * launched and work items (pillar groups x time windows). */
 int pstm_hip_last_launch(int* mw, int* batch, int* grid, int* items);

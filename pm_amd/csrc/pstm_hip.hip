@@ -66,6 +66,7 @@ constexpr int kThreads = 64 * kWaves;  // 1024 threads
 constexpr int kNPil = kPW * kPG;       // 8 pillars per workgroup
 constexpr int kMaxBatch = 16;          // traces per LDS batch (upper bound)
 constexpr int kMaxHalfWidth = 64;      // largest anti-aliasing half width (aa_max_half_width)
+constexpr int kPF = 12;                // dwords of the next batch's rows each thread holds in flight
 constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS uses the rest
 
 // per-(pillar, sample) terms, f32, built once per call in f64
@@ -109,6 +110,7 @@ struct KParams {
     int ns, nt;
     int rowlen, lo;              // AA row length / index offset
     int batch;                   // traces per LDS batch
+    int pipe;                    // 1: next batch's rows loaded into registers during the current batch
     int pair_off;                // byte offset of the pair table in LDS
     int tot_off;                 // byte offset of the f64 running totals in LDS
     float taper_off;             // x_t = d * slope + taper_off (taper_off = -(1-tf)/tf)
@@ -657,6 +659,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     unsigned long long cnt = 0;
     __shared__ int s_item;
     __shared__ int s_row[kMaxBatch];  // block rows of the current batch
+    __shared__ int s_rownext[kMaxBatch];  // and of the next one (pipelined staging)
     __shared__ float2 s_invsq[kMaxHalfWidth + 2];  // AA: {1 / l^2, 1 / (l + 1)^2}
     if (AA) {
         for (int i = tid; i < kMaxHalfWidth + 2; i += kThreads)
@@ -744,12 +747,52 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             nfr = (float)(lane & ((1 << P.lat_shift) - 1)) * P.lat_ifrac;
         }
 
+        // Pipelined staging (P.pipe): while a batch computes, each thread holds
+        // up to kPF dwords of the next batch's rows in flight in registers; they
+        // are written to LDS after the batch's last barrier.
+        uint32_t pf[kPF];
+        const int rl_dw = P.rowlen * (int)(sizeof(Row) / 4);
+        auto row_dw = [&](int jrow, bool from_lds, int r) -> const uint32_t* {
+            if (AA && !P.prow_global) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)jrow * P.rowlen);
+            const int rw = from_lds ? s_rownext[r] : (T.tidx ? T.tidx[jrow] : jrow);
+            if constexpr (AA) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)rw * P.rowlen);
+            else return reinterpret_cast<const uint32_t*>(P.amp + (size_t)rw * P.ns);
+        };
+        auto prefetch = [&](int jb, bool from_lds) {
+            const int nbb = min(P.batch, T.ntr - jb);
+            if (nbb <= 0) return;
+            const int n = nbb * rl_dw;
+            int r = 0, c = tid;
+            while (c >= rl_dw) {
+                c -= rl_dw;
+                ++r;
+            }
+            const uint32_t* src = row_dw(jb + min(r, nbb - 1), from_lds, min(r, nbb - 1));
+#pragma unroll
+            for (int i = 0; i < kPF; ++i) {
+                if (tid + i * kThreads < n) pf[i] = src[c];
+                c += kThreads;
+                bool moved = false;
+                while (c >= rl_dw) {
+                    c -= rl_dw;
+                    ++r;
+                    moved = true;
+                }
+                if (moved && r < nbb) src = row_dw(jb + r, from_lds, r);
+            }
+        };
+        if (P.pipe) prefetch(0, false);
+
         for (int j0 = 0; j0 < T.ntr; j0 += P.batch) {
             const int nb = min(P.batch, T.ntr - j0);
             // (pillar, trace) pair table; the max-aperture test is the
             // oracle's f64 one (numba_cpu_optimized.py:167-173)
             int inap = 0;
             if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
+            if (P.pipe && tid >= 64 && tid < 64 + P.batch) {
+                const int jn = j0 + P.batch + (tid - 64);
+                s_rownext[tid - 64] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
+            }
             if (pt_thread) {
                 const int pil = pil0 + pt_p;
                 float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
@@ -788,7 +831,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 pairs[(pt_p * P.batch + pt_j) * kPairVec] = e;
                 if (kPairVec == 2) pairs[(pt_p * P.batch + pt_j) * kPairVec + 1] = f;
             }
-            if (!__syncthreads_or(inap)) continue;
+            if (!__syncthreads_or(inap)) {
+                if (P.pipe) prefetch(j0 + P.batch, true);  // this batch's rows are not needed
+                continue;
+            }
 
             if (TT == PSTM_TT_VTI_LEARNED) {
                 // network on the t0 lattice of every in-aperture pair of the
@@ -834,7 +880,13 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
 
             // stage the batch of trace rows in LDS (coalesced)
-            if (AA && !P.prow_global) {
+            if (P.pipe) {
+                uint32_t* dst = reinterpret_cast<uint32_t*>(rows);
+                const int n = nb * rl_dw;
+#pragma unroll
+                for (int i = 0; i < kPF; ++i)
+                    if (tid + i * kThreads < n) dst[tid + i * kThreads] = pf[i];
+            } else if (AA && !P.prow_global) {
                 const double* src = P.prow + (size_t)j0 * P.rowlen;
                 double* dst = reinterpret_cast<double*>(rows);
                 const int n = nb * P.rowlen;
@@ -869,6 +921,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 }
             }
             __syncthreads();
+            if (P.pipe) prefetch(j0 + P.batch, true);  // in flight during the batch
 
             // the traces of the batch; EARLY: this wave holds t0 <= 2 ms samples,
             // where t <= 1 ms can switch the spreading/obliquity weights off
@@ -1067,6 +1120,7 @@ DeviceState g_state;
 struct Tuning {
     int mw_cap = 0;
     int batch_cap = 0;
+    int pipe = -1;  // pipelined staging: -1 automatic, 0 off, 1 on
 };
 Tuning g_tune;
 
@@ -1237,6 +1291,7 @@ struct Shape {
     int mw = 1, batch = 1, lat_nodes = 0, win = 256, nwin = 1;
     size_t tot_bytes = 0, lds = 0;
     int occupancy = 1;
+    bool pipe = false;
 };
 
 Shape launch_shape(const PstmMigrateArgs& a) {
@@ -1256,9 +1311,9 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     const int nwin = (nt + 1023) / 1024;
     const int per_win = (nt + nwin - 1) / nwin;
     int mw_cap = g_tune.mw_cap > 0 ? g_tune.mw_cap : env_int("PSTM_HIP_MW");
-    // built-in choice: the largest MW whose gather keeps its registers (the
-    // anti-aliased gather spills beyond MW = 2)
-    if (mw_cap <= 0) mw_cap = sh.aa ? 2 : 4;
+    // built-in choice: MW = 2 for the anti-aliased gather (it spills beyond),
+    // 4 otherwise
+    if (mw_cap <= 0) mw_cap = sh.aa ? 2 : 4;  // measured best on the reflector job
     mw_cap = std::min(4, mw_cap);
     sh.mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
     // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
@@ -1271,6 +1326,16 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
     sh.batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
     while (sh.batch > 1 && lds_for(sh.batch) > kLdsBudget) --sh.batch;
+    // pipelined staging needs the batch's rows within kPF dwords per thread;
+    // automatic: on for f32 rows, off for the f64 anti-aliasing rows (whose
+    // batch would shrink to 4 and whose gather then spills)
+    int pipe = g_tune.pipe;
+    if (pipe < 0 && getenv("PSTM_HIP_PIPE")) pipe = atoi(getenv("PSTM_HIP_PIPE")) != 0;
+    if (pipe < 0) pipe = sh.aa ? 0 : 1;
+    const int rl_dw = sh.rowlen * (int)(sh.elem / 4);
+    const int pipe_batch = kPF * kThreads / rl_dw;
+    sh.pipe = pipe == 1 && pipe_batch >= 1;
+    if (sh.pipe) sh.batch = std::min(sh.batch, pipe_batch);
     sh.lds = lds_for(sh.batch);
     sh.win = 256 * sh.mw;
     sh.nwin = (nt + sh.win - 1) / sh.win;
@@ -1294,6 +1359,7 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.rowlen = sh.rowlen;
     kp.lo = sh.lo;
     kp.batch = sh.batch;
+    kp.pipe = sh.pipe ? 1 : 0;
     kp.pair_off = (int)align_up((size_t)sh.batch * sh.rowlen * sh.elem, 16);
     kp.tot_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * sh.batch * sh.pair_vec * 16, 16);
     kp.taper_off = a.taper_fraction > 0.0 ? (float)(-(1.0 - a.taper_fraction) / a.taper_fraction) : -1e30f;
@@ -1874,6 +1940,13 @@ int pstm_hip_set_tuning(int mw_cap, int batch_cap) {
         return fail(1, "pstm_hip_set_tuning: mw_cap in [0, 4], batch_cap in [0, 16] (0 = default)");
     g_tune.mw_cap = mw_cap;
     g_tune.batch_cap = batch_cap;
+    return 0;
+}
+
+int pstm_hip_set_pipeline(int mode) {
+    g_last_error.clear();
+    if (mode < -1 || mode > 1) return fail(1, "pstm_hip_set_pipeline: mode -1 (automatic), 0 (off) or 1 (on)");
+    g_tune.pipe = mode;
     return 0;
 }
 

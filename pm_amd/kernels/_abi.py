@@ -71,7 +71,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_init", "pstm_hip_release", "pstm_hip_stream_sync",
     "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
-    "pstm_hip_set_tuning", "pstm_hip_last_launch",
+    "pstm_hip_set_tuning", "pstm_hip_set_pipeline", "pstm_hip_last_launch",
     "pstm_hip_plan_create", "pstm_hip_plan_run", "pstm_hip_plan_info", "pstm_hip_plan_destroy",
     "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_set_precision", "pstm_mlp_forward",
     "pstm_mlp_forward_mfma", "pstm_mlp_loss_grad",
@@ -116,6 +116,8 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
                                           ctypes.POINTER(ctypes.c_longlong)]
     lib.pstm_hip_set_tuning.restype = ctypes.c_int
     lib.pstm_hip_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.pstm_hip_set_pipeline.restype = ctypes.c_int
+    lib.pstm_hip_set_pipeline.argtypes = [ctypes.c_int]
     ip = ctypes.POINTER(ctypes.c_int)
     lib.pstm_hip_last_launch.restype = ctypes.c_int
     lib.pstm_hip_last_launch.argtypes = [ip, ip, ip, ip]

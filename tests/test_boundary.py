@@ -152,6 +152,22 @@ def test_invalid_arguments_are_rejected_without_device_work():
     gs = (ctypes.c_int32 * 3)(0, 5, 3)
     assert lib.pstm_hip_migrate_bins(ctypes.byref(a), gs, 2, None) == 1
     assert "non-decreasing" in _abi.last_error(lib)
+    # learned traveltime without a network, bad lattice spacing
+    a.aa_mode = _abi.AA_NONE
+    a.traveltime = _abi.TT_VTI_LEARNED
+    assert lib.pstm_hip_migrate_tile(ctypes.byref(a), None) == 1
+    assert "mlp_params" in _abi.last_error(lib)
+    a.mlp_params, a.mlp_node_spacing = 8, 12
+    assert lib.pstm_hip_migrate_tile(ctypes.byref(a), None) == 1
+    assert "mlp_node_spacing" in _abi.last_error(lib)
+    # plans, tuning knobs
+    h = ctypes.c_void_p()
+    assert lib.pstm_hip_plan_create(None, None, 0, ctypes.byref(h)) == 1
+    assert lib.pstm_hip_plan_run(None, 1, None) == 1
+    assert lib.pstm_hip_set_tuning(5, 0) == 1 and lib.pstm_hip_set_tuning(0, 17) == 1
+    assert lib.pstm_hip_set_pipeline(2) == 1
+    assert lib.pstm_hip_set_tuning(0, 0) == 0 and lib.pstm_hip_set_pipeline(-1) == 0
+    assert lib.pstm_mlp_set_precision(3) == 1 and lib.pstm_mlp_set_precision(0) == 0
 
 
 def test_roofline_constants_frozen():
