@@ -339,6 +339,9 @@ def main() -> None:
           "effective_contributions_per_launch": int(e_per_launch),
           "avg_launch_ms": round(launch_ms, 3), "launches_timed": launches,
           "frac_of_unpacked_fp32": round(achieved / roofline.MI355X_FP32_VALU_UNPACKED_TFLOPS, 4)}
+    # the VTI job has its own launch shape: back to the built-in choice
+    lib.pstm_hip_set_tuning(0, 0)
+    lib.pstm_hip_set_pipeline(-1)
     learned = learned_vs_exact(kernel, args) if args.learned else None
     cpu = None
     if args.cpu_seconds > 0:

@@ -942,12 +942,17 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         // that the MW independent chains (sqrt, LDS reads) interleave;
                         // slots outside the aperture or the trace read a safe sample and
                         // add zero.
+                        // blk[m]: some lane of sample block m is inside the
+                        // time-dependent aperture (wave-uniform); the aperture grows
+                        // with t0, so the early blocks of distant pairs are skipped
                         float xt[MW];
+                        bool blk[MW];
                         bool any_in = false;
     #pragma unroll
                         for (int m = 0; m < MW; ++m) {
                             xt[m] = fmaf(e.z, ci[p][m], P.taper_off);
-                            any_in |= xt[m] <= 1.0f + P.eps_x;
+                            blk[m] = __any(xt[m] <= 1.0f + P.eps_x);
+                            any_in |= blk[m];
                         }
                         if (!any_in) continue;  // time-dependent aperture excludes every slot
                         float t[MW], u[MW], y1[MW], y2[MW];
@@ -963,6 +968,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         }
     #pragma unroll
                         for (int m = 0; m < MW; ++m) {
+                            if (!blk[m]) {
+                                ok[m] = false;
+                                t[m] = u[m] = y1[m] = y2[m] = 0.0f;
+                                continue;
+                            }
                             t[m] = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m], y1[m],
                                                         y2[m]);
                             u[m] = fmaf(t[m], P.idt, P.uoff);
@@ -992,6 +1002,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         }
     #pragma unroll
                         for (int m = 0; m < MW; ++m) {
+                            if (!blk[m]) continue;
                             const float us = ok[m] ? u[m] : 0.0f;
                             const int i0 = (int)us;
                             const float f0 = __builtin_amdgcn_fractf(us);
