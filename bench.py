@@ -209,6 +209,71 @@ def learned_vs_exact(kernel, args) -> dict:
     return out
 
 
+def streamed_section(kernel, args) -> dict:
+    """Config-5 family, scaled: common-offset bins with lognormal counts,
+    2001-sample traces, written to local disk as an uncompressed Zarr v2 array
+    (1000-trace chunks) + Parquet headers, then migrated while streaming
+    (pinned buffers, copy stream, one plan per bin launched as the bin
+    arrives) and, for comparison, from memory (one plan)."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from pm_amd.io.zarr_v2 import write_headers, write_zarr_v2
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.pipeline.streaming import migrate_streamed
+    from pm_amd.synthetic import bins_job
+
+    job = bins_job(n_bins=args.stream_bins, total=args.stream_traces, n_samples=2001, n_out=256, nt=1000,
+                   tile=64, device="cuda")
+    d = Path(tempfile.mkdtemp(prefix="pstm_stream_", dir=os.environ.get("TMPDIR", "/tmp")))
+    try:
+        t0 = time.perf_counter()
+        bins = np.repeat(np.arange(args.stream_bins), np.diff(job.meta["bin_start"]))
+        write_zarr_v2(d / "traces.zarr", job.amplitudes, chunk_rows=1000,
+                      attrs={"sample_rate_ms": job.dt_ms, "start_time_ms": job.t_start_ms})
+        write_headers(d / "headers.parquet", job.source_x, job.source_y, job.receiver_x, job.receiver_y,
+                      extra={"BIN": bins})
+        write_s = time.perf_counter() - t0
+        nbytes = job.n_traces * job.n_samples * 4
+        log(f"stream: wrote {nbytes / 1e9:.2f} GB in {write_s:.1f} s")
+        # in memory: one plan over every tile (file order, as streamed)
+        ex = TileExecutor(kernel, job, spatial_sort=False)
+        plan = ex.plan()
+        ex.reset()
+        plan.run()                      # capture
+        ex.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run()
+        torch.cuda.synchronize()
+        mem_s = time.perf_counter() - t0
+        img_mem, fold_mem = ex.assemble()
+        img_mem = img_mem.clone()
+        plan.destroy()
+        del ex
+        torch.cuda.empty_cache()
+        res = migrate_streamed(kernel, d / "traces.zarr", d / "headers.parquet", job.x_axis, job.y_axis,
+                               job.t_axis_ms, job.config, cube=job.cube, tile=job.tile, n_buffers=16,
+                               n_threads=8)
+        rel = float(torch.linalg.norm((res.image - img_mem).double()) / torch.linalg.norm(img_mem.double()))
+        out = {"workload": (f"{args.stream_bins} common-offset bins, {job.n_traces} traces x 2001 samples "
+                            f"({nbytes / 1e9:.2f} GB f32) in uncompressed Zarr v2, 256x256 x 1000 output"),
+               "bytes": nbytes, "groups": res.groups, "chunks": res.stats.chunks,
+               "streamed_wall_s": round(res.wall_s, 2), "in_memory_gather_s": round(mem_s, 2),
+               "streaming_overhead_s": round(res.wall_s - mem_s, 2),
+               "read_threads_busy_s": round(res.stats.read_s, 2),
+               "file_read_gb_s": round(nbytes / 1e9 / max(res.stats.read_s, 1e-9), 2),
+               "note": "files written just before: reads are served from the page cache",
+               "fold_equal": bool(torch.equal(res.fold, fold_mem)), "image_rel_l2_vs_in_memory": rel}
+        log(f"stream: {out}")
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -231,6 +296,10 @@ def main() -> None:
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=40_000)
     ap.add_argument("--node-spacing", type=int, default=16)
+    ap.add_argument("--stream", action="store_true",
+                    help="also measure migration streamed from disk (scaled config-5 bins job)")
+    ap.add_argument("--stream-bins", type=int, default=60)
+    ap.add_argument("--stream-traces", type=int, default=240_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the migration shards "
@@ -343,6 +412,7 @@ def main() -> None:
     lib.pstm_hip_set_tuning(0, 0)
     lib.pstm_hip_set_pipeline(-1)
     learned = learned_vs_exact(kernel, args) if args.learned else None
+    streamed = streamed_section(kernel, args) if args.stream else None
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(ex, job, args.cpu_seconds)
@@ -361,6 +431,8 @@ def main() -> None:
     }
     if learned is not None:
         out["learned_vs_exact"] = learned
+    if streamed is not None:
+        out["streamed"] = streamed
     if tune is not None:
         out["autotune"] = tune.as_dict()
     if cpu:
