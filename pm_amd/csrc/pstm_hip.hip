@@ -110,7 +110,10 @@ struct KParams {
     int ns, nt;
     int rowlen, lo;              // AA row length / index offset
     int batch;                   // traces per LDS batch
-    int pipe;                    // 1: next batch's rows loaded into registers during the current batch
+    int pipe;                    // 1: the next batch's rows are staged during the current batch
+                                 //    (f32 rows: through registers; AA rows: direct global->LDS
+                                 //    loads into the second of two row buffers)
+    int rows_bytes;              // bytes of one row buffer
     int pair_off;                // byte offset of the pair table in LDS
     int tot_off;                 // byte offset of the f64 running totals in LDS
     float taper_off;             // x_t = d * slope + taper_off (taper_off = -(1-tf)/tf)
@@ -647,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     using Row = typename std::conditional<AA, double, float>::type;
     constexpr int kPairVec = (AA || TT == PSTM_TT_CURVED) ? 2 : 1;
-    Row* rows = reinterpret_cast<Row*>(smem);
+    Row* rows = reinterpret_cast<Row*>(smem);  // current row buffer
     float4* pairs = reinterpret_cast<float4*>(smem + P.pair_off);
 
     const int tid = threadIdx.x;
@@ -781,7 +784,40 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 if (moved && r < nbb) src = row_dw(jb + r, from_lds, r);
             }
         };
-        if (P.pipe) prefetch(0, false);
+        // AA rows: double-buffered direct global -> LDS loads, 64 dwords per
+        // wave instruction (wave-uniform LDS destination, per-lane source)
+        int cur = 0;
+        auto dma = [&](int jb, int buf, bool from_lds) {
+            if constexpr (AA) {
+                const int nbb = min(P.batch, T.ntr - jb);
+                if (nbb <= 0) return;
+                const int n = nbb * rl_dw;
+                uint32_t* dst0 = reinterpret_cast<uint32_t*>(smem + (size_t)buf * P.rows_bytes);
+                int e = wave * 64 + lane;
+                int r = 0, col = e;
+                while (col >= rl_dw) {
+                    col -= rl_dw;
+                    ++r;
+                }
+                const uint32_t* src = row_dw(jb + min(r, nbb - 1), from_lds, min(r, nbb - 1));
+                for (int c = wave; c * 64 < n; c += kWaves) {
+                    if (e < n) __builtin_amdgcn_global_load_lds(src + col, dst0 + c * 64, 4, 0, 0);
+                    e += kWaves * 64;
+                    col += kWaves * 64;
+                    bool moved = false;
+                    while (col >= rl_dw) {
+                        col -= rl_dw;
+                        ++r;
+                        moved = true;
+                    }
+                    if (moved && r < nbb) src = row_dw(jb + r, from_lds, r);
+                }
+            }
+        };
+        if (P.pipe) {
+            if (AA) dma(0, 0, false);
+            else prefetch(0, false);
+        }
 
         for (int j0 = 0; j0 < T.ntr; j0 += P.batch) {
             const int nb = min(P.batch, T.ntr - j0);
@@ -832,7 +868,14 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 if (kPairVec == 2) pairs[(pt_p * P.batch + pt_j) * kPairVec + 1] = f;
             }
             if (!__syncthreads_or(inap)) {
-                if (P.pipe) prefetch(j0 + P.batch, true);  // this batch's rows are not needed
+                if (P.pipe) {  // this batch's rows are not needed
+                    if (AA) {
+                        dma(j0 + P.batch, cur ^ 1, true);
+                        cur ^= 1;
+                    } else {
+                        prefetch(j0 + P.batch, true);
+                    }
+                }
                 continue;
             }
 
@@ -880,7 +923,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
 
             // stage the batch of trace rows in LDS (coalesced)
-            if (P.pipe) {
+            if (AA && P.pipe) {
+                rows = reinterpret_cast<Row*>(smem + (size_t)cur * P.rows_bytes);
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
+            } else if (P.pipe) {
                 uint32_t* dst = reinterpret_cast<uint32_t*>(rows);
                 const int n = nb * rl_dw;
 #pragma unroll
@@ -921,7 +967,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 }
             }
             __syncthreads();
-            if (P.pipe) prefetch(j0 + P.batch, true);  // in flight during the batch
+            if (P.pipe) {  // in flight during the batch
+                if (AA) dma(j0 + P.batch, cur ^ 1, true);
+                else prefetch(j0 + P.batch, true);
+            }
 
             // the traces of the batch; EARLY: this wave holds t0 <= 2 ms samples,
             // where t <= 1 ms can switch the spreading/obliquity weights off
@@ -1067,6 +1116,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     acc[p][m] = 0.0f;
                 }
             __syncthreads();
+            if (AA && P.pipe) cur ^= 1;
         }
 
         // the workgroup is the only writer of these cells
@@ -1330,23 +1380,25 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
     sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * sizeof(double);
     sh.lat_nodes = sh.learned ? 256 * sh.mw / a.mlp_node_spacing + 1 : 0;
+    // pipelined staging: f32 rows through kPF registers per thread (the batch's
+    // rows must fit), AA rows by direct global -> LDS loads into a second row
+    // buffer (LDS holds two buffers); automatic: on
+    int pipe = g_tune.pipe;
+    if (pipe < 0 && getenv("PSTM_HIP_PIPE")) pipe = atoi(getenv("PSTM_HIP_PIPE")) != 0;
+    if (pipe < 0) pipe = 1;
+    const int rl_dw = sh.rowlen * (int)(sh.elem / 4);
+    const int pipe_batch = sh.aa ? kMaxBatch : kPF * kThreads / rl_dw;
+    sh.pipe = pipe == 1 && pipe_batch >= 1;
+    const int nbuf = (sh.pipe && sh.aa) ? 2 : 1;
     auto lds_for = [&](int b) {
-        return align_up(align_up((size_t)b * sh.rowlen * sh.elem, 16) + (size_t)kNPil * b * sh.pair_vec * 16, 16) +
+        return align_up((size_t)nbuf * align_up((size_t)b * sh.rowlen * sh.elem, 16) +
+                            (size_t)kNPil * b * sh.pair_vec * 16, 16) +
                sh.tot_bytes + align_up((size_t)kNPil * b * sh.lat_nodes * 8, 16);
     };
     int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
     sh.batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
-    while (sh.batch > 1 && lds_for(sh.batch) > kLdsBudget) --sh.batch;
-    // pipelined staging needs the batch's rows within kPF dwords per thread;
-    // automatic: on for f32 rows, off for the f64 anti-aliasing rows (whose
-    // batch would shrink to 4 and whose gather then spills)
-    int pipe = g_tune.pipe;
-    if (pipe < 0 && getenv("PSTM_HIP_PIPE")) pipe = atoi(getenv("PSTM_HIP_PIPE")) != 0;
-    if (pipe < 0) pipe = sh.aa ? 0 : 1;
-    const int rl_dw = sh.rowlen * (int)(sh.elem / 4);
-    const int pipe_batch = kPF * kThreads / rl_dw;
-    sh.pipe = pipe == 1 && pipe_batch >= 1;
     if (sh.pipe) sh.batch = std::min(sh.batch, pipe_batch);
+    while (sh.batch > 1 && lds_for(sh.batch) > kLdsBudget) --sh.batch;
     sh.lds = lds_for(sh.batch);
     sh.win = 256 * sh.mw;
     sh.nwin = (nt + sh.win - 1) / sh.win;
@@ -1371,7 +1423,8 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.lo = sh.lo;
     kp.batch = sh.batch;
     kp.pipe = sh.pipe ? 1 : 0;
-    kp.pair_off = (int)align_up((size_t)sh.batch * sh.rowlen * sh.elem, 16);
+    kp.rows_bytes = (int)align_up((size_t)sh.batch * sh.rowlen * sh.elem, 16);
+    kp.pair_off = kp.rows_bytes * ((sh.pipe && sh.aa) ? 2 : 1);
     kp.tot_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * sh.batch * sh.pair_vec * 16, 16);
     kp.taper_off = a.taper_fraction > 0.0 ? (float)(-(1.0 - a.taper_fraction) / a.taper_fraction) : -1e30f;
     kp.wpow = (a.apply_spreading ? 1 : 0) + (a.apply_obliquity ? 1 : 0);

@@ -188,3 +188,45 @@ class TestHipSurrogate:
         print("loss history", s.history, "held-out error (samples, bf16)", err)
         assert s.history[-1][1] < s.history[0][1] * 1e-2
         assert err["max"] < 0.25
+
+
+def test_anisotropic_factor_closed_form():
+    """c from the exact VTI leg time equals -2 eta s^2 / (1 + 2 eta s^2) with s
+    the straight-ray sine (the identity the input encoding rests on), its
+    limit at tau = 0, and 0 for vanishing distance; the learned leg time
+    rebuilt from c is the exact one."""
+    import torch
+
+    from oracle import oracle as orc
+    from pm_amd.surrogate import anisotropic_factor, hyperbolic, learned_leg_time
+
+    rng = np.random.default_rng(21)
+    tau = rng.uniform(0.01, 2.0, 400)
+    d = rng.uniform(1.0, 5000.0, 400)
+    v = rng.uniform(1500.0, 4500.0, 400)
+    eta = rng.uniform(0.0, 0.2, 400)
+    t = np.array([orc.vti_leg(*a) for a in zip(tau, d, v, eta)])
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64)  # noqa: E731
+    c = anisotropic_factor(T(t), T(tau), T(d), T(v), T(eta)).numpy()
+    s = d / (v * hyperbolic(T(tau), T(d), T(v)).numpy())
+    np.testing.assert_allclose(c, -2 * eta * s ** 2 / (1 + 2 * eta * s ** 2), rtol=1e-7, atol=1e-10)
+    np.testing.assert_allclose(learned_leg_time(T(tau), T(d), T(v), T(c)).numpy(), t, rtol=1e-12)
+    c0 = anisotropic_factor(T([0.0]), T([0.0]), T([300.0]), T([2000.0]), T([0.1])).numpy()
+    np.testing.assert_allclose(c0, [-0.2 / 1.2])
+    cz = anisotropic_factor(T([0.5]), T([0.5]), T([0.0]), T([2000.0]), T([0.1])).numpy()
+    assert cz[0] == 0.0
+
+
+def test_encoding_ranges():
+    import torch
+
+    from pm_amd.surrogate import Normalizer
+
+    norm = Normalizer(tau_max=1.0, d_max=2000.0, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
+    x = torch.tensor([[0.0, 0.0, 2000.0, 1500.0, 0.0], [1.0, 2000.0, 0.0, 4500.0, 0.2],
+                      [0.0, 500.0, 0.0, 3000.0, 0.1]], dtype=torch.float64)
+    e = norm.apply(x).numpy()
+    np.testing.assert_allclose(e[0], [-1, -1, 1, -1, -1], atol=1e-7)   # tau = 0, d > 0: sine 1
+    np.testing.assert_allclose(e[1, [0, 3, 4]], [1, 1, 1], atol=1e-7)
+    np.testing.assert_allclose(e[2], [-1, 1, -1, 0, 0], atol=1e-7)
+    assert np.all(np.abs(e) <= 1 + 1e-6)
