@@ -282,7 +282,7 @@ def main() -> None:
     ap.add_argument("--workload", default="reflectors", choices=["reflectors", "point_diffractor", "bins"])
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip timing candidate launch shapes on the busiest tile")
-    ap.add_argument("--candidates", default="2x8x0,2x16x0,2x4x1,4x8x0,4x16x1,2x16x1",
+    ap.add_argument("--candidates", default="4x8x0,4x16x1,2x10x0,4x3x1",
                     help="autotune candidates as MWxBATCHxPIPE,... (PIPE: 0 off, 1 pipelined staging)")
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: one elementwise kernel over 256 MiB (traffic calibration), "
@@ -334,7 +334,7 @@ def main() -> None:
         if args.candidates:
             cands = [tuple(int(v) for v in c.split("x")) for c in args.candidates.split(",")]
         tune = autotune(kernel, ex, candidates=cands)
-        log(f"autotune: mw={tune.mw} batch={tune.batch} pipe={tune.pipe} ({tune.ms:.1f} ms on the busiest tile)")
+        log(f"autotune: mw={tune.mw} batch={tune.batch} pipe={tune.pipe} ({tune.ms:.1f} ms on the 4 busiest tiles)")
     lib = _abi.load_library()
 
     if args.profile:

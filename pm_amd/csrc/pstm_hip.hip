@@ -1372,9 +1372,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     const int nwin = (nt + 1023) / 1024;
     const int per_win = (nt + nwin - 1) / nwin;
     int mw_cap = g_tune.mw_cap > 0 ? g_tune.mw_cap : env_int("PSTM_HIP_MW");
-    // built-in choice: MW = 2 for the anti-aliased gather (it spills beyond),
-    // 4 otherwise
-    if (mw_cap <= 0) mw_cap = sh.aa ? 2 : 4;  // measured best on the reflector job
+    if (mw_cap <= 0) mw_cap = 4;  // measured best on the reflector job, with and without AA
     mw_cap = std::min(4, mw_cap);
     sh.mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
     // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
@@ -1382,10 +1380,11 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     sh.lat_nodes = sh.learned ? 256 * sh.mw / a.mlp_node_spacing + 1 : 0;
     // pipelined staging: f32 rows through kPF registers per thread (the batch's
     // rows must fit), AA rows by direct global -> LDS loads into a second row
-    // buffer (LDS holds two buffers); automatic: on
+    // buffer (LDS holds two buffers, so the batch shrinks); automatic: on for
+    // f32 rows, off for AA rows (measured: the larger single-buffered batch wins)
     int pipe = g_tune.pipe;
     if (pipe < 0 && getenv("PSTM_HIP_PIPE")) pipe = atoi(getenv("PSTM_HIP_PIPE")) != 0;
-    if (pipe < 0) pipe = 1;
+    if (pipe < 0) pipe = sh.aa ? 0 : 1;
     const int rl_dw = sh.rowlen * (int)(sh.elem / 4);
     const int pipe_batch = sh.aa ? kMaxBatch : kPF * kThreads / rl_dw;
     sh.pipe = pipe == 1 && pipe_batch >= 1;
