@@ -170,6 +170,10 @@ def learned_vs_exact(kernel, args) -> dict:
         torch.cuda.synchronize()
         return kms.value, img.double()
 
+    from pm_amd.autotune import autotune
+
+    vt = autotune(kernel, ex, candidates=[(2, 10, 0), (2, 16, 1), (4, 8, 0), (4, 16, 1)])
+    log(f"vti autotune: mw={vt.mw} batch={vt.batch} pipe={vt.pipe}")
     exact_plan = ex.plan()
     ms_exact, img_exact = timed([exact_plan])
     plans = []
@@ -201,6 +205,7 @@ def learned_vs_exact(kernel, args) -> dict:
         "exact_gcontrib_s": round(c_total / (ms_exact / 1e3) / 1e9, 3),
         "learned_gcontrib_s": round(c_total / (ms_learned / 1e3) / 1e9, 3),
         "exact_ms": round(ms_exact, 2), "learned_ms": round(ms_learned, 2),
+        "launch_shape": {"mw": vt.mw, "batch": vt.batch, "pipe": vt.pipe},
         "learned_speedup": round(ms_exact / ms_learned, 3),
         "image_rel_l2_learned_vs_exact": rel,
         "image_rel_l2_without_network_vs_exact": rel_iso,

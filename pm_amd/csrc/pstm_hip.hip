@@ -1004,8 +1004,6 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             any_in |= blk[m];
                         }
                         if (!any_in) continue;  // time-dependent aperture excludes every slot
-                        float t[MW], u[MW], y1[MW], y2[MW];
-                        bool ok[MW];
                         if (TT == PSTM_TT_VTI_LEARNED) {
                             const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * P.lat_nodes;
     #pragma unroll
@@ -1015,57 +1013,43 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                 e1[p][m] = fmaf(nfr, c1.y - c0.y, c0.y);
                             }
                         }
+                        // one pass per sample block, so that only one block's
+                        // temporaries are live at a time (fewer registers)
     #pragma unroll
                         for (int m = 0; m < MW; ++m) {
-                            if (!blk[m]) {
-                                ok[m] = false;
-                                t[m] = u[m] = y1[m] = y2[m] = 0.0f;
-                                continue;
-                            }
-                            t[m] = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m], y1[m],
-                                                        y2[m]);
-                            u[m] = fmaf(t[m], P.idt, P.uoff);
+                            if (!blk[m]) continue;
+                            float y1, y2;
+                            const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m],
+                                                                 y1, y2);
+                            const float u = fmaf(t, P.idt, P.uoff);
                             // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
                             const bool in_ap = xt[m] <= ((COUNT && TT != PSTM_TT_VTI_LEARNED) ? 1.0f - P.eps_x : 1.0f);
-                            const bool below = u[m] < P.umax_lo;
-                            bool near = in_ap && !below && u[m] < P.umax_hi;
-                            if (LOB) near = near || (in_ap && u[m] < P.lo_eps && u[m] > -P.lo_eps);
+                            const bool below = u < P.umax_lo;
+                            bool near = in_ap && !below && u < P.umax_hi;
+                            if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
                             if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
                             // the learned time has no f64 counterpart to decide against
                             if (TT == PSTM_TT_VTI_LEARNED) near = false;
                             any_near |= near;
-                            ok[m] = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u[m] < P.umax : below) && !near;
-                            if (LOB) ok[m] = ok[m] && u[m] >= 0.0f;
-                        }
-                        float tw[MW];
-                        bool any_taper = false;
-    #pragma unroll
-                        for (int m = 0; m < MW; ++m) {
-                            tw[m] = e.w;
-                            any_taper |= ok[m] && xt[m] > 0.0f;
-                        }
-                        if (any_taper) {
-    #pragma unroll
-                            for (int m = 0; m < MW; ++m)
-                                if (xt[m] > 0.0f) tw[m] = e.w * taper_weight(fminf(xt[m], 1.0f));
-                        }
-    #pragma unroll
-                        for (int m = 0; m < MW; ++m) {
-                            if (!blk[m]) continue;
-                            const float us = ok[m] ? u[m] : 0.0f;
+                            bool ok = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u < P.umax : below) && !near;
+                            if (LOB) ok = ok && u >= 0.0f;
+                            float w = ok ? e.w : 0.0f;
+                            if (__any(ok && xt[m] > 0.0f)) {  // taper zone (wave-uniform test)
+                                if (xt[m] > 0.0f) w *= taper_weight(fminf(xt[m], 1.0f));
+                            }
+                            const float us = ok ? u : 0.0f;
                             const int i0 = (int)us;
                             const float f0 = __builtin_amdgcn_fractf(us);
-                            const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1[m], y2[m], s_invsq);
-                            float w = ok[m] ? tw[m] : 0.0f;
+                            const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq);
                             if (W2 || P.wpow != 0) {
-                                const float r = frcp(t[m]);
+                                const float r = frcp(t);
                                 float wf = cw[p][m] * ((W2 || P.wpow == 2) ? r * r : r);
                                 // t > 1 ms is certain unless the block holds t0 <= 2 ms (early)
-                                if (EARLY && (early & (1u << m))) wf = t[m] > 0.001f ? wf : 1.0f;
+                                if (EARLY && (early & (1u << m))) wf = t > 0.001f ? wf : 1.0f;
                                 w *= wf;
                             }
                             acc[p][m] = fmaf(amp, w, acc[p][m]);
-                            if (COUNT) cnt += ok[m] ? 1 : 0;
+                            if (COUNT) cnt += ok ? 1 : 0;
                         }
                     }
                     // slots near a threshold: recompute which, decide in f64 as the
