@@ -80,12 +80,12 @@ def cpu_baseline(ex, job, budget_s: float) -> dict:
     v1 = tile.vrms.cpu().numpy() if tile.vrms.dim() == 1 else None
     xa, ya = tile.x_axis.cpu().numpy(), tile.y_axis.cpu().numpy()
 
-    def run(n_p: int):
+    def run(n_p: int, n_x: int = 1):
         inp = orc.MigrateInputs(
             amplitudes=h["amplitudes"], source_x=h["source_x"], source_y=h["source_y"],
             receiver_x=h["receiver_x"], receiver_y=h["receiver_y"], midpoint_x=h["midpoint_x"],
-            midpoint_y=h["midpoint_y"], x_axis=xa[ix:ix + 1], y_axis=ya[:n_p],
-            t_axis_ms=job.t_axis_ms, vrms=(v3[ix:ix + 1, :n_p] if v3 is not None else v1),
+            midpoint_y=h["midpoint_y"], x_axis=xa[ix:ix + n_x], y_axis=ya[:n_p],
+            t_axis_ms=job.t_axis_ms, vrms=(v3[ix:ix + n_x, :n_p] if v3 is not None else v1),
             dt_in_ms=job.dt_ms, max_aperture_m=cfg.max_aperture_m,
             min_aperture_m=cfg.min_aperture_m, max_dip_deg=cfg.max_dip_degrees,
             taper_fraction=cfg.taper_fraction, apply_spreading=cfg.apply_spreading,
@@ -96,14 +96,14 @@ def cpu_baseline(ex, job, budget_s: float) -> dict:
         res = orc.migrate(inp, threads=threads, fast=True)
         return time.perf_counter() - t0, int(res.fold2d.sum()) * len(job.t_axis_ms), res
 
-    run(1)                      # warm-up: library load, OpenMP thread start
-    dt2, _, _ = run(2)
     ny = tile.y1 - tile.y0
-    n_p = int(max(1, min(ny, round(2 * budget_s / max(dt2, 1e-3)))))
-    dt, c, _ = run(n_p)
-    log(f"cpu baseline: {n_p} pillars x {len(rows)} traces in {dt:.1f} s on {threads} threads")
+    run(1)                      # warm-up: library load, OpenMP thread start
+    dt1, _, _ = run(ny)         # one inline: every thread busy
+    n_x = int(max(1, min(tile.x1 - tile.x0 - ix, round(budget_s / max(dt1, 1e-3)))))
+    dt, c, _ = run(ny, n_x)
+    log(f"cpu baseline: {n_x} x {ny} pillars x {len(rows)} traces in {dt:.1f} s on {threads} threads")
     return {"value": c / dt / 1e9, "unit": "Gcontrib/s", "cores": threads, "kind": "port",
-            "sample": (f"{n_p} pillars (one inline of the busiest tile) x all {len(rows)} "
+            "sample": (f"{n_x} x {ny} pillars (whole inlines of the busiest tile) x all {len(rows)} "
                        f"traces of that tile x {len(job.t_axis_ms)} samples, same settings; "
                        f"C_sample={c}, {dt:.2f} s; oracle C++ FP64 -O3 -ffast-math OpenMP")}
 
@@ -111,7 +111,7 @@ def cpu_baseline(ex, job, budget_s: float) -> dict:
 # HBM bytes per gather launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per
 # dispatch scaled by true bytes / counter of the calibration kernel), per workload
 # (profiles/k_migrate_reflectors_aa.md: FETCH x 2.000 + WRITE x 1.000, one plan launch)
-TRAFFIC: dict = {"reflectors": (12_882_893_089 * 2.000 + 256_520 * 1.000) * 1024}
+TRAFFIC: dict = {"reflectors": (6_328_551_838 * 2.000 + 266_505 * 1.000) * 1024}
 
 
 def learned_vs_exact(kernel, args) -> dict:
