@@ -141,8 +141,8 @@ def learned_vs_exact(kernel, args) -> dict:
     log(f"vti job ready in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     tr = make_training_set(vc, ec, args.train_samples, d_max, norm, seed=1, t_max_ms=3000.0)
-    sur = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=args.train_steps, batch=8192,
-                                                          qat_fraction=0.25, log_every=args.train_steps // 4)
+    sur = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=args.train_steps, batch=args.train_batch,
+                                                          qat_fraction=0.25, log_every=5000, verbose=True)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     del tr
@@ -197,7 +197,7 @@ def learned_vs_exact(kernel, args) -> dict:
         "workload": (f"VTI diffractors: {job.n_traces} traces x 2001 samples, 3-D v_rms 1500-4500 and eta "
                      f"0-0.2 on 128x128x201 grids, output {args.vti_out}x{args.vti_out} x 1500 samples in "
                      f"{len(ex.tiles)} tiles, aperture 2500 m"),
-        "surrogate": {"train_samples": args.train_samples, "steps": args.train_steps, "batch": 8192,
+        "surrogate": {"train_samples": args.train_samples, "steps": args.train_steps, "batch": args.train_batch,
                       "train_s": round(train_s, 1), "node_spacing": K,
                       "heldout_max_samples_per_tile": [round(e, 4) for e in errs],
                       "tiles_learned": len(learned_tiles), "tiles_exact_fallback": len(exact_tiles)},
@@ -299,7 +299,8 @@ def main() -> None:
     ap.add_argument("--vti-traces", type=int, default=100_000)
     ap.add_argument("--vti-out", type=int, default=128)
     ap.add_argument("--train-samples", type=int, default=10_000_000)
-    ap.add_argument("--train-steps", type=int, default=40_000)
+    ap.add_argument("--train-steps", type=int, default=80_000)
+    ap.add_argument("--train-batch", type=int, default=16_384)
     ap.add_argument("--node-spacing", type=int, default=16)
     ap.add_argument("--stream", action="store_true",
                     help="also measure migration streamed from disk (scaled config-5 bins job)")

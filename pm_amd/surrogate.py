@@ -241,10 +241,14 @@ class TraveltimeSurrogate:
         return loss, self.grads
 
     def train(self, ts: TrainingSet, steps: int, batch: int = 8192, lr: float = 2e-3,
-              lr_final: float = 5e-5, seed: int = 0, log_every: int = 0, qat_fraction: float = 0.0):
+              lr_final: float = 5e-5, seed: int = 0, log_every: int = 0, qat_fraction: float = 0.0,
+              verbose: bool = False):
         """Adam with a cosine-decayed learning rate over `steps` minibatches;
         the last qat_fraction of the steps train through the bf16 numerics of
-        the fused inference (quantisation-aware)."""
+        the fused inference (quantisation-aware).  Every log_every steps the
+        loss is recorded in history (and printed to stderr when verbose)."""
+        import sys
+
         import torch
 
         lib = _abi.load_library()
@@ -268,6 +272,8 @@ class TraveltimeSurrogate:
                 set_precision(False)
             if log_every and (i % log_every == 0 or i == steps - 1):
                 self.history.append((i, float(loss.item())))
+                if verbose:
+                    print(f"[surrogate] step {i}: loss {self.history[-1][1]:.3e}", file=sys.stderr, flush=True)
         return self
 
     def leg_times(self, ts: TrainingSet, bf16: bool = False):
