@@ -159,3 +159,35 @@ def test_learned_plan_equals_tile_runs(kernel):
     img, fold = _state(ex)
     assert torch.equal(img, ref_img) and torch.equal(fold, ref_fold)
     plan.destroy()
+
+
+@pytest.mark.parametrize("kind", ["vti", "curved"])
+def test_plan_other_traveltimes_equal_tile_runs(kernel, kind):
+    """Exact VTI (3-D eta slabs per tile) and curved-ray plans equal their
+    tile-by-tile launches bit for bit."""
+    import torch
+    from dataclasses import replace
+
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.synthetic import reflector_job, vti_diffractor_job
+
+    if kind == "vti":
+        job = vti_diffractor_job(n_traces=6000, n_samples=801, n_out=32, nt=300, tile=16, n_diffractors=10,
+                                 device="cuda")
+    else:
+        job = reflector_job(n_traces=6000, n_samples=801, n_out=32, nt=300, tile=16, antialias="none",
+                            n_planes=5, n_diffractors=10, device="cuda")
+        job.config = replace(job.config, kernel_type="curved_ray", curved_ray_enabled=True, curved_ray_v0=1600.0,
+                             curved_ray_k=0.4)
+    ex = TileExecutor(kernel, job)
+    assert kind != "vti" or all(t.eta is not None for t in ex.tiles)
+    ex.reset()
+    ex.run()
+    ref_img, ref_fold = _state(ex)
+    assert float(ref_img.abs().max()) > 0
+    plan = ex.plan()
+    ex.reset()
+    plan.run()
+    img, fold = _state(ex)
+    assert torch.equal(img, ref_img) and torch.equal(fold, ref_fold)
+    plan.destroy()
