@@ -231,6 +231,53 @@ int pstm_hip_plan_info(const PstmPlan* plan, long long* n_items, int* grid, int*
 int pstm_hip_plan_destroy(PstmPlan* plan);
 
 /* ---------------------------------------------------------------------------
+ * Output step: the executor's per-tile header accumulation, gather-bin trace
+ * selection and finalize normalisation, on device arrays.  All pointers are
+ * device pointers; calls are asynchronous on `stream`.
+ * ------------------------------------------------------------------------- */
+
+/* Per-output-bin trace-header sums of one tile, added after the tile as the
+ * executor does (pstm/pipeline/executor.py:1783-1806).  For every row r of
+ * trace_index (NULL = rows 0 .. n_traces-1) of the block (source_x .. :
+ * (rows,) f64), with midpoint m = (s + r) / 2 and offset vector o = r - s:
+ *   ix = clip(floor((m_x - x_min) / dx), 0, tile_nx - 1) + x_start,
+ *   iy = clip(floor((m_y - y_min) / dy), 0, tile_ny - 1) + y_start,
+ *   trace_count[ix, iy] += 1, offset_sum[ix, iy] += |o|,
+ *   azimuth_sin_sum[ix, iy] += sin a, azimuth_cos_sum[ix, iy] += cos a,
+ *   a = atan2(o_x, o_y) (north = 0; a = 0 for o = 0),
+ * into (grid_nx, grid_ny) row-major arrays.  Traces outside the tile (its
+ * aperture halo) land in its edge bins, as in the reference.  Sums use
+ * atomics: counts are exact, the f64 sums exact up to summation order. */
+int pstm_hip_bin_headers(const double* source_x, const double* source_y, const double* receiver_x,
+                         const double* receiver_y, const int32_t* trace_index, int32_t n_traces, double x_min,
+                         double y_min, double dx, double dy, int32_t x_start, int32_t y_start, int32_t tile_nx,
+                         int32_t tile_ny, int32_t grid_nx, int32_t grid_ny, int32_t* trace_count,
+                         double* offset_sum, double* azimuth_sin_sum, double* azimuth_cos_sum, void* stream);
+
+/* Gather-bin filter (executor.py:1537-1554): keep[i] = 1 when row
+ * trace_index[i] passes every check whose bit is set in `checks`, else 0.
+ * offset = |r - s|, o_x = r_x - s_x, o_y = r_y - s_y; bounds are closed, as in
+ * the reference (an offset bin is GatherBin.offset_min <= offset <= offset_max,
+ * an offset-vector tile bounds o_x and o_y; pstm/config/models.py:677-760). */
+#define PSTM_SEL_OFFSET_MIN 1 /* offset >= offset_min */
+#define PSTM_SEL_OFFSET_MAX 2 /* offset <= offset_max */
+#define PSTM_SEL_OVT_X_MIN 4  /* o_x >= ovt_x_min     */
+#define PSTM_SEL_OVT_X_MAX 8  /* o_x <= ovt_x_max     */
+#define PSTM_SEL_OVT_Y_MIN 16 /* o_y >= ovt_y_min     */
+#define PSTM_SEL_OVT_Y_MAX 32 /* o_y <= ovt_y_max     */
+int pstm_hip_select_rows(const double* source_x, const double* source_y, const double* receiver_x,
+                         const double* receiver_y, const int32_t* trace_index, int32_t n_traces, int32_t checks,
+                         double offset_min, double offset_max, double ovt_x_min, double ovt_x_max, double ovt_y_min,
+                         double ovt_y_max, uint8_t* keep, void* stream);
+
+/* Gather-volume normalisation at finalize (executor.py:2155-2160):
+ * out[p, k] = image[p, k] / fold[p] where fold[p] > 0, else 0, for pillars
+ * p < n_pillars and samples k < nt (the reference's 3-D fold is the pillar
+ * fold repeated along t).  f32, correctly rounded; out may equal image. */
+int pstm_hip_fold_normalize(const float* image, const int32_t* fold, long long n_pillars, int32_t nt, float* out,
+                            void* stream);
+
+/* ---------------------------------------------------------------------------
  * Traveltime surrogate (no counterpart in the reference: the learned part of
  * this backend).  A 5-64-64-64-64-2 SiLU MLP; inputs are the encoded
  * (tau, s_s, s_r, v, eta) of PstmMigrateArgs.mlp_*, outputs out_scale * c per

@@ -16,7 +16,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 REPO = PKG.parent
-SOURCES = [PKG / "csrc" / "pstm_hip.hip", PKG / "csrc" / "pstm_mlp.hip"]
+SOURCES = [PKG / "csrc" / "pstm_hip.hip", PKG / "csrc" / "pstm_mlp.hip", PKG / "csrc" / "pstm_output.hip"]
 OUT = PKG / "lib" / "libpstm_hip.so"
 
 HIPCC_FLAGS = [

@@ -1649,6 +1649,10 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
 
 }  // namespace
 
+// Failure message setter for the library's other translation units (the
+// output kernels), so that pstm_hip_last_error() reports their failures too.
+__attribute__((visibility("hidden"))) void pstm_set_last_error(const std::string& msg) { g_last_error = msg; }
+
 // A device-resident work queue over many output tiles (common-offset bins
 // included: a tile's trace list may hold rows of every bin): per-tile
 // geometry, velocity and fold terms are built once; a run is one persistent

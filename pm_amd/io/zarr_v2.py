@@ -141,6 +141,87 @@ def write_zarr_v2(path, data, chunk_rows: int = 1000, attrs: dict | None = None)
     return ZarrV2Array.open(path)
 
 
+_ND_DTYPES = {"<f4": np.float32, "<f8": np.float64, "<i4": np.int32}
+
+
+class ZarrV2Writer:
+    """An uncompressed N-D Zarr v2 array written chunk by chunk (the output
+    volumes: image (nx, ny, nt) f32 and fold (nx, ny, nt) i32, chunked
+    (chunk_x, chunk_y, chunk_t) as the reference writes them,
+    pstm/pipeline/executor.py:2066-2101).  Chunk files are named
+    ``"i.j.k"`` and hold the full chunk shape in C order; an edge chunk is
+    padded with the fill value, as Zarr pads it."""
+
+    def __init__(self, path, shape, chunks, dtype, attrs: dict | None = None):
+        self.path = Path(path)
+        self.shape = tuple(int(s) for s in shape)
+        self.chunks = tuple(int(c) for c in chunks)
+        if len(self.chunks) != len(self.shape) or any(c < 1 for c in self.chunks):
+            raise ValueError(f"chunks {chunks} do not fit shape {shape}")
+        self.dtype = np.dtype(dtype)
+        code = {np.dtype(v): k for k, v in _ND_DTYPES.items()}.get(self.dtype)
+        if code is None:
+            raise ValueError(f"unsupported dtype {self.dtype}")
+        self.path.mkdir(parents=True, exist_ok=True)
+        fill = 0.0 if self.dtype.kind == "f" else 0
+        meta = {"zarr_format": 2, "shape": list(self.shape), "chunks": list(self.chunks), "dtype": code,
+                "compressor": None, "fill_value": fill, "order": "C", "filters": None}
+        (self.path / ".zarray").write_text(json.dumps(meta, indent=1))
+        if attrs:
+            (self.path / ".zattrs").write_text(json.dumps(attrs, indent=1))
+
+    @property
+    def grid(self) -> tuple[int, ...]:
+        """Chunks along each axis."""
+        return tuple((s + c - 1) // c for s, c in zip(self.shape, self.chunks))
+
+    def bounds(self, idx) -> tuple[slice, ...]:
+        """Array region of chunk ``idx``."""
+        return tuple(slice(i * c, min((i + 1) * c, s)) for i, c, s in zip(idx, self.chunks, self.shape))
+
+    def write_chunk(self, idx, block) -> int:
+        """Write chunk ``idx`` from ``block`` (the region's shape, any layout;
+        broadcast views are fine).  Returns the bytes written.  Releases the
+        GIL while writing, so several threads write in parallel."""
+        region = tuple(b.stop - b.start for b in self.bounds(idx))
+        block = np.asarray(block)
+        if block.shape != region:
+            raise ValueError(f"chunk {idx}: block shape {block.shape} != region {region}")
+        if region == self.chunks:
+            buf = np.ascontiguousarray(block, dtype=self.dtype)
+        else:
+            buf = np.zeros(self.chunks, self.dtype)
+            buf[tuple(slice(0, r) for r in region)] = block
+        with open(self.path / ".".join(str(i) for i in idx), "wb") as fh:
+            fh.write(memoryview(buf).cast("B"))
+        return buf.nbytes
+
+
+def read_zarr_v2(path) -> tuple[np.ndarray, dict]:
+    """An uncompressed N-D Zarr v2 array read whole, with its attributes
+    (tests and small arrays; a missing chunk is the fill value)."""
+    path = Path(path)
+    meta = json.loads((path / ".zarray").read_text())
+    if meta.get("zarr_format") != 2 or meta.get("compressor") is not None or meta.get("filters"):
+        raise NotImplementedError(f"{path}: only uncompressed Zarr v2 arrays are read here")
+    if meta.get("order", "C") != "C" or meta.get("dtype") not in _ND_DTYPES:
+        raise NotImplementedError(f"{path}: order {meta.get('order')}, dtype {meta.get('dtype')}")
+    sep = meta.get("dimension_separator", ".")
+    shape, chunks = tuple(meta["shape"]), tuple(meta["chunks"])
+    dt = np.dtype(_ND_DTYPES[meta["dtype"]])
+    out = np.full(shape, meta.get("fill_value") or 0, dt)
+    for idx in np.ndindex(*((s + c - 1) // c for s, c in zip(shape, chunks))):
+        f = path / sep.join(str(i) for i in idx)
+        if not f.exists():
+            continue
+        blk = np.fromfile(f, dtype=dt).reshape(chunks)
+        region = tuple(slice(i * c, min((i + 1) * c, s)) for i, c, s in zip(idx, chunks, shape))
+        out[region] = blk[tuple(slice(0, r.stop - r.start) for r in region)]
+    attrs_path = path / ".zattrs"
+    attrs = json.loads(attrs_path.read_text()) if attrs_path.exists() else {}
+    return out, attrs
+
+
 #: header columns, as the reference's ColumnMapping defaults (pstm/config/models.py:135-151)
 HEADER_COLUMNS = ("trace_idx", "SOU_X", "SOU_Y", "REC_X", "REC_Y")
 

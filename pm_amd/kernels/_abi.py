@@ -19,6 +19,8 @@ TT_STRAIGHT, TT_CURVED, TT_VTI, TT_VTI_LEARNED = 0, 1, 2, 3
 AA_NONE, AA_TRIANGLE = 0, 1
 VEL_1D, VEL_3D = 0, 1
 ETA_CONST, ETA_1D, ETA_3D = 0, 1, 2
+SEL_OFFSET_MIN, SEL_OFFSET_MAX = 1, 2
+SEL_OVT_X_MIN, SEL_OVT_X_MAX, SEL_OVT_Y_MIN, SEL_OVT_Y_MAX = 4, 8, 16, 32
 
 _vp = ctypes.c_void_p
 
@@ -73,6 +75,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
     "pstm_hip_set_tuning", "pstm_hip_set_pipeline", "pstm_hip_last_launch",
     "pstm_hip_plan_create", "pstm_hip_plan_run", "pstm_hip_plan_info", "pstm_hip_plan_destroy",
+    "pstm_hip_bin_headers", "pstm_hip_select_rows", "pstm_hip_fold_normalize",
     "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_set_precision", "pstm_mlp_forward",
     "pstm_mlp_forward_mfma", "pstm_mlp_loss_grad",
     "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_vti_leg_times",
@@ -129,6 +132,14 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_hip_plan_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong), ip, ip, ip]
     lib.pstm_hip_plan_destroy.restype = ctypes.c_int
     lib.pstm_hip_plan_destroy.argtypes = [_vp]
+    d, i32 = ctypes.c_double, ctypes.c_int32
+    lib.pstm_hip_bin_headers.restype = ctypes.c_int
+    lib.pstm_hip_bin_headers.argtypes = [_vp, _vp, _vp, _vp, _vp, i32, d, d, d, d, i32, i32, i32, i32, i32, i32,
+                                         _vp, _vp, _vp, _vp, _vp]
+    lib.pstm_hip_select_rows.restype = ctypes.c_int
+    lib.pstm_hip_select_rows.argtypes = [_vp, _vp, _vp, _vp, _vp, i32, i32, d, d, d, d, d, d, _vp, _vp]
+    lib.pstm_hip_fold_normalize.restype = ctypes.c_int
+    lib.pstm_hip_fold_normalize.argtypes = [_vp, _vp, ctypes.c_longlong, i32, _vp, _vp]
     f = ctypes.c_float
     lib.pstm_mlp_param_count.restype = ctypes.c_int
     lib.pstm_mlp_param_count.argtypes = []

@@ -240,6 +240,78 @@ class HipKernel:
         self._check(lib, rc, "pstm_hip_plan_create")
         return HipPlan(self, handle, keep, counter)
 
+    # -- output step ---------------------------------------------------------
+    def bin_headers(self, dt: DeviceTraces, rows, x_min: float, y_min: float, dx: float, dy: float,
+                    x_start: int, y_start: int, tile_nx: int, tile_ny: int, sums) -> None:
+        """Add one tile's per-bin trace-header sums (count, offset, azimuth
+        sine / cosine) into ``sums`` (a pm_amd.pipeline.output.HeaderSums on
+        this device), as the reference's executor does after each tile
+        (pstm/pipeline/executor.py:1783-1806).  ``rows``: int32 device rows of
+        ``dt`` (the tile's trace query), or None for every row.  Asynchronous
+        on the current stream."""
+        self._require_init()
+        torch = _torch()
+        lib = self._lib
+        if rows is not None and (rows.dtype != torch.int32 or not rows.is_cuda):
+            raise ValueError("rows must be an int32 device tensor")
+        n = dt.n_rows if rows is None else int(rows.shape[0])
+        gnx, gny = sums.shape
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self._device_index).cuda_stream)
+        rc = lib.pstm_hip_bin_headers(_ptr(dt.source_x), _ptr(dt.source_y), _ptr(dt.receiver_x), _ptr(dt.receiver_y),
+                                      _ptr(rows) if rows is not None else None, n, float(x_min), float(y_min),
+                                      float(dx), float(dy), int(x_start), int(y_start), int(tile_nx), int(tile_ny),
+                                      gnx, gny, _ptr(sums.trace_count), _ptr(sums.offset_sum),
+                                      _ptr(sums.azimuth_sin_sum), _ptr(sums.azimuth_cos_sum), stream)
+        self._check(lib, rc, "pstm_hip_bin_headers")
+
+    def select_rows(self, dt: DeviceTraces, rows, gather_bin):
+        """The rows (int32 device tensor, in their order) of ``rows`` (None:
+        every row of ``dt``) that pass ``gather_bin``'s offset or
+        offset-vector-tile filter (executor.py:1537-1554; closed bounds, a
+        bound of None is not checked)."""
+        self._require_init()
+        torch = _torch()
+        lib = self._lib
+        if rows is None:
+            rows = torch.arange(dt.n_rows, dtype=torch.int32, device=self.device)
+        if rows.dtype != torch.int32 or not rows.is_cuda:
+            raise ValueError("rows must be an int32 device tensor")
+        n = int(rows.shape[0])
+        keep = torch.empty(n, dtype=torch.uint8, device=self.device)
+        checks, bounds = 0, []
+        for bit, v in ((_abi.SEL_OFFSET_MIN, gather_bin.offset_min), (_abi.SEL_OFFSET_MAX, gather_bin.offset_max),
+                       (_abi.SEL_OVT_X_MIN, gather_bin.ovt_x_min), (_abi.SEL_OVT_X_MAX, gather_bin.ovt_x_max),
+                       (_abi.SEL_OVT_Y_MIN, gather_bin.ovt_y_min), (_abi.SEL_OVT_Y_MAX, gather_bin.ovt_y_max)):
+            if v is not None:
+                checks |= bit
+            bounds.append(0.0 if v is None else float(v))
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self._device_index).cuda_stream)
+        rc = lib.pstm_hip_select_rows(_ptr(dt.source_x), _ptr(dt.source_y), _ptr(dt.receiver_x), _ptr(dt.receiver_y),
+                                      _ptr(rows), n, checks, *bounds, _ptr(keep), stream)
+        self._check(lib, rc, "pstm_hip_select_rows")
+        return rows[keep.bool()]
+
+    def fold_normalize(self, image, fold2d, out=None):
+        """image (nx, ny, nt) f32 divided by the pillar fold (nx, ny) i32 where
+        it is positive, 0 elsewhere (executor.py:2155-2160), into ``out``
+        (a new tensor by default; may be ``image``).  Returns ``out``."""
+        self._require_init()
+        torch = _torch()
+        lib = self._lib
+        if image.dtype != torch.float32 or image.dim() != 3 or not image.is_contiguous():
+            raise ValueError("image must be a contiguous (nx, ny, nt) float32 device tensor")
+        if fold2d.dtype != torch.int32 or tuple(fold2d.shape) != tuple(image.shape[:2]) or not fold2d.is_contiguous():
+            raise ValueError("fold must be a contiguous (nx, ny) int32 device tensor matching the image")
+        if out is None:
+            out = torch.empty_like(image)
+        elif out.shape != image.shape or out.dtype != torch.float32 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float32 tensor shaped like image")
+        nx, ny, nt = (int(s) for s in image.shape)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self._device_index).cuda_stream)
+        rc = lib.pstm_hip_fold_normalize(_ptr(image), _ptr(fold2d), nx * ny, nt, _ptr(out), stream)
+        self._check(lib, rc, "pstm_hip_fold_normalize")
+        return out
+
     def _args(self, dt, image, fold2d, x_axis, y_axis, t_axis_ms, vrms, config, traveltime, trace_index, eta,
               counter, surrogate, node_spacing):
         nx, ny, nt = (int(s) for s in image.shape)

@@ -203,6 +203,46 @@ class TileExecutor:
             out.append(err)
         return out
 
+    def output_grid(self):
+        """The job's output axes and bin sizes (KernelConfig.grid_dx / dy)."""
+        from pm_amd.pipeline.output import OutputGrid
+
+        return OutputGrid.from_axes(self.job.x_axis, self.job.y_axis, self.job.t_axis_ms, self.config.grid_dx,
+                                    self.config.grid_dy)
+
+    def header_sums(self, sums=None, rows_of=None):
+        """Per-output-bin trace-header sums of the whole job: every tile adds
+        its trace query, binned by midpoint and clipped to the tile, as the
+        reference's executor does after each tile (executor.py:1783-1806).
+        ``rows_of(i)`` overrides tile i's rows (a gather bin's selection).
+        Asynchronous; returns the (new, or given) HeaderSums."""
+        from pm_amd.pipeline.output import HeaderSums
+
+        nx, ny, _ = self.shape
+        sums = sums if sums is not None else HeaderSums.zeros(nx, ny, self.device)
+        dx, dy = float(self.config.grid_dx), float(self.config.grid_dy)
+        for i, t in enumerate(self.tiles):
+            rows = t.rows if rows_of is None else rows_of(i)
+            self.kernel.bin_headers(self.traces, rows, float(self.job.x_axis[t.x0]), float(self.job.y_axis[t.y0]),
+                                    dx, dy, t.x0, t.y0, t.x1 - t.x0, t.y1 - t.y0, sums)
+        return sums
+
+    def finalize(self, out_dir, headers=None, **kw):
+        """Write the stacked image, fold volume and bin headers
+        (pm_amd.pipeline.output.finalize_stack); computes the header sums when
+        none are given.  Returns the WriteStats."""
+        from pm_amd.pipeline.output import finalize_stack
+
+        headers = headers if headers is not None else self.header_sums()
+        image, fold = self.assemble()
+        return finalize_stack(out_dir, image, fold, self.output_grid(), headers, **kw)
+
+    def gathers(self, bins) -> "GatherSet":
+        """Gather mode: one image, fold and header sums per GatherBin, each
+        tile's trace list narrowed to the bin's traces (executor.py:960-975,
+        1533-1590)."""
+        return GatherSet(self, list(bins))
+
     def contributions(self) -> int:
         """C = sum of the 3-D fold = sum over pillars of fold2d * nt."""
         return int(sum(int(t.fold.sum()) for t in self.tiles)) * self.nt
@@ -227,3 +267,75 @@ class TileExecutor:
         for name in ("source_x", "source_y", "receiver_x", "receiver_y", "midpoint_x", "midpoint_y"):
             out[name] = getattr(tr, name)[sel].cpu().numpy()
         return out
+
+
+class GatherSet:
+    """Per-bin output volumes of a gather-mode job.  The reference migrates
+    each bin in its own pass over the tiles into its own memmaps
+    (pstm/pipeline/executor.py:960-975, 838-860); here every (tile, bin) pair
+    is one entry of a single plan, so one persistent launch fills all bins'
+    images, and the header sums of each bin are accumulated on the device."""
+
+    def __init__(self, ex: TileExecutor, bins: list):
+        import torch
+
+        if not bins:
+            raise ValueError("gather mode needs at least one bin")
+        self.ex = ex
+        self.bins = bins
+        dev = ex.device
+        slab = int(ex.image_storage.shape[0])
+        self.images = torch.zeros((len(bins), slab), dtype=torch.float32, device=dev)
+        self.folds = [[torch.zeros((t.x1 - t.x0, t.y1 - t.y0), dtype=torch.int32, device=dev) for t in ex.tiles]
+                      for _ in bins]
+        self.rows = [[ex.kernel.select_rows(ex.traces, t.rows, gb) for t in ex.tiles] for gb in bins]
+        self._views = []
+        for b in range(len(bins)):
+            off, views = 0, []
+            for t in ex.tiles:
+                n = (t.x1 - t.x0) * (t.y1 - t.y0) * ex.nt
+                views.append(self.images[b, off:off + n].view(t.x1 - t.x0, t.y1 - t.y0, ex.nt))
+                off += n
+            self._views.append(views)
+        self.headers = [ex.header_sums(rows_of=lambda i, b=b: self.rows[b][i]) for b in range(len(bins))]
+
+    def reset(self) -> None:
+        self.images.zero_()
+        for fb in self.folds:
+            for f in fb:
+                f.zero_()
+
+    def plan(self, **kw):
+        """One HipPlan over every (tile, bin) pair with traces."""
+        ex = self.ex
+        descs = []
+        for b in range(len(self.bins)):
+            for i, t in enumerate(ex.tiles):
+                rows = self.rows[b][i]
+                if int(rows.shape[0]) == 0:
+                    continue
+                descs.append(dict(image=self._views[b][i], fold=self.folds[b][i], x_axis=t.x_axis, y_axis=t.y_axis,
+                                  vrms=t.vrms, trace_index=rows, eta=t.eta))
+        return ex.kernel.create_plan(ex.traces, descs, ex.t_axis, ex.config, **kw)
+
+    def assemble(self, b: int):
+        """Bin b's (nx, ny, nt) f32 image and (nx, ny) i32 fold on the device."""
+        import torch
+
+        nx, ny, nt = self.ex.shape
+        img = torch.empty((nx, ny, nt), dtype=torch.float32, device=self.ex.device)
+        fold = torch.empty((nx, ny), dtype=torch.int32, device=self.ex.device)
+        for t, v, f in zip(self.ex.tiles, self._views[b], self.folds[b]):
+            img[t.x0:t.x1, t.y0:t.y1] = v
+            fold[t.x0:t.x1, t.y0:t.y1] = f
+        return img, fold
+
+    def finalize(self, out_dir, **kw):
+        """Write every bin (pm_amd.pipeline.output.finalize_gathers)."""
+        from pm_amd.pipeline.output import finalize_gathers
+
+        def volumes(b):
+            img, fold = self.assemble(b)
+            return img, fold, self.headers[b]
+
+        return finalize_gathers(out_dir, self.ex.kernel, self.bins, volumes, self.ex.output_grid(), **kw)
