@@ -279,6 +279,60 @@ def streamed_section(kernel, args) -> dict:
         shutil.rmtree(d, ignore_errors=True)
 
 
+def output_section(kernel, ex) -> dict:
+    """The output step on the migrated job: per-bin header sums over every
+    tile's trace query (HIP atomics), the gather-mode fold normalisation of
+    the whole volume (HBM-bound), and the stack finalize write-out
+    (migrated_stack.zarr + fold.zarr + bin_headers.parquet) into a temporary
+    directory that is removed afterwards."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    from pm_amd.pipeline.output import HeaderSums
+
+    def timed_ms(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    nx, ny, nt = ex.shape
+    rows = sum(int(t.rows.shape[0]) for t in ex.tiles)
+    sums = HeaderSums.zeros(nx, ny, ex.device)
+
+    def headers():
+        for s in (sums.trace_count, sums.offset_sum, sums.azimuth_sin_sum, sums.azimuth_cos_sum):
+            s.zero_()
+        ex.header_sums(sums)
+
+    h_ms = timed_ms(headers, 5)
+    image, fold = ex.assemble()
+    out = torch.empty_like(image)
+    n_ms = timed_ms(lambda: kernel.fold_normalize(image, fold, out), 20)
+    n_bytes = 2 * image.numel() * 4 + fold.numel() * 4
+    d = tempfile.mkdtemp(prefix="pm_amd_out_")
+    try:
+        torch.cuda.synchronize()
+        st = ex.finalize(d, headers=sums)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"output: headers {h_ms:.2f} ms, normalise {n_ms:.3f} ms, write {st.bytes / 1e9:.2f} GB "
+        f"in {st.seconds:.2f} s")
+    return {"header_sums_ms": round(h_ms, 3), "header_rows": rows,
+            "fold_normalize_ms": round(n_ms, 4), "fold_normalize_GBps": round(n_bytes / n_ms / 1e6, 1),
+            "fold_normalize_frac_of_hbm": round(n_bytes / n_ms / 1e6 / 8000.0, 3),
+            "write_GB": round(st.bytes / 1e9, 3), "write_s": round(st.seconds, 3),
+            "write_GBps": round(st.gb_per_s, 2),
+            "files": ["migrated_stack.zarr", "fold.zarr", "bin_headers.parquet"]}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -302,6 +356,8 @@ def main() -> None:
     ap.add_argument("--train-steps", type=int, default=80_000)
     ap.add_argument("--train-batch", type=int, default=16_384)
     ap.add_argument("--node-spacing", type=int, default=16)
+    ap.add_argument("--no-output", dest="output", action="store_false",
+                    help="skip the output-step section (header sums, normalisation, finalize write-out)")
     ap.add_argument("--stream", action="store_true",
                     help="also measure migration streamed from disk (scaled config-5 bins job)")
     ap.add_argument("--stream-bins", type=int, default=60)
@@ -414,6 +470,7 @@ def main() -> None:
           "effective_contributions_per_launch": int(e_per_launch),
           "avg_launch_ms": round(launch_ms, 3), "launches_timed": launches,
           "frac_of_unpacked_fp32": round(achieved / roofline.MI355X_FP32_VALU_UNPACKED_TFLOPS, 4)}
+    output = output_section(kernel, ex) if args.output else None
     # the VTI job has its own launch shape: back to the built-in choice
     lib.pstm_hip_set_tuning(0, 0)
     lib.pstm_hip_set_pipeline(-1)
@@ -439,6 +496,8 @@ def main() -> None:
         out["learned_vs_exact"] = learned
     if streamed is not None:
         out["streamed"] = streamed
+    if output is not None:
+        out["output"] = output
     if tune is not None:
         out["autotune"] = tune.as_dict()
     if cpu:

@@ -8,10 +8,12 @@
 // (executor.py:2155-2160).  The traces, images and folds already live in HBM,
 // so these steps run there as three small kernels:
 //
-//   * k_bin_headers — one thread per trace row; hardware atomics into the
-//     (nx, ny) header arrays.  The bin index and the offset are computed with
-//     the reference's operations in f64 with explicit rounding (no fused
-//     multiply-add), so the bin of every trace is the reference's bin;
+//   * k_bin_headers_lds — the tile's bins summed in LDS per workgroup, then
+//     added to the (nx, ny) header arrays with hardware atomics (k_bin_headers,
+//     one global atomic per row, for tiles too large for LDS).  The bin index
+//     and the offset are computed with the reference's operations in f64 with
+//     explicit rounding (no fused multiply-add), so the bin of every trace is
+//     the reference's bin;
 //   * k_select_rows — the gather-bin filter as a byte mask per row (the host
 //     compacts it in row order);
 //   * k_fold_normalize — HBM-bound elementwise pass, 16-byte accesses.
@@ -20,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -43,6 +46,7 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr int kBlock = 256;
+constexpr size_t kHeaderLds = 160 * 1024;   // LDS bins of one tile: up to 5851 bins
 
 // offset = sqrt(dx^2 + dy^2), rounded as NumPy rounds it (compute_offset,
 // pstm/utils/units.py:180-200)
@@ -50,15 +54,14 @@ __device__ __forceinline__ double offset_of(double dx, double dy) {
     return __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_bin_headers(const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ rx,
-              const double* __restrict__ ry, const int32_t* __restrict__ rows, int n, double x_min,
-              double y_min, double dx, double dy, int x_start, int y_start, int tnx, int tny, int grid_ny,
-              int* __restrict__ count, double* __restrict__ osum, double* __restrict__ ssum,
-              double* __restrict__ csum) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int r = rows ? rows[i] : i;
+struct BinTerms {
+    int bin;        // tile-local bin, ix * tny + iy
+    double off, s, c;
+};
+
+__device__ __forceinline__ BinTerms bin_terms(const double* __restrict__ sx, const double* __restrict__ sy,
+                                              const double* __restrict__ rx, const double* __restrict__ ry, int r,
+                                              double x_min, double y_min, double dx, double dy, int tnx, int tny) {
     const double ax = sx[r], ay = sy[r], bx = rx[r], by = ry[r];
     // midpoint (sx + rx) / 2 (offset_to_midpoint, units.py:157-177); halving is exact
     const double mx = __dadd_rn(ax, bx) * 0.5;
@@ -69,20 +72,79 @@ k_bin_headers(const double* __restrict__ sx, const double* __restrict__ sy, cons
     // clip in f64 first: a far trace's quotient can exceed the int range
     const int ix = static_cast<int>(fmin(fmax(fx, 0.0), static_cast<double>(tnx - 1)));
     const int iy = static_cast<int>(fmin(fmax(fy, 0.0), static_cast<double>(tny - 1)));
-    const long long cell = static_cast<long long>(ix + x_start) * grid_ny + (iy + y_start);
     const double ox = __dsub_rn(bx, ax), oy = __dsub_rn(by, ay);
-    const double off = offset_of(ox, oy);
+    BinTerms t{ix * tny + iy, offset_of(ox, oy), 0.0, 1.0};
     // azimuth a = atan2(ox, oy) (north = 0): sin a = ox / |o|, cos a = oy / |o|;
     // a zero offset has a = 0
-    double s = 0.0, c = 1.0;
-    if (off > 0.0) {
-        s = __ddiv_rn(ox, off);
-        c = __ddiv_rn(oy, off);
+    if (t.off > 0.0) {
+        t.s = __ddiv_rn(ox, t.off);
+        t.c = __ddiv_rn(oy, t.off);
     }
+    return t;
+}
+
+// Most rows of a tile's query lie in its aperture halo and clip into a few
+// edge bins, so global atomics would queue on a handful of addresses.  Each
+// workgroup therefore sums its share of the rows in an LDS copy of the
+// tile's bins (count i32, then offset / sin / cos f64) and adds only the
+// bins it touched to the global arrays.
+constexpr int kHBlock = 1024;
+
+__global__ void __launch_bounds__(kHBlock)
+k_bin_headers_lds(const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ rx,
+                  const double* __restrict__ ry, const int32_t* __restrict__ rows, int n, int per_block,
+                  double x_min, double y_min, double dx, double dy, int x_start, int y_start, int tnx, int tny,
+                  int grid_ny, int* __restrict__ count, double* __restrict__ osum, double* __restrict__ ssum,
+                  double* __restrict__ csum) {
+    extern __shared__ double lds[];
+    const int nb = tnx * tny;
+    double* l_off = lds;
+    double* l_s = lds + nb;
+    double* l_c = lds + 2 * nb;
+    int* l_n = reinterpret_cast<int*>(lds + 3 * nb);
+    for (int b = threadIdx.x; b < nb; b += kHBlock) {
+        l_off[b] = 0.0;
+        l_s[b] = 0.0;
+        l_c[b] = 0.0;
+        l_n[b] = 0;
+    }
+    __syncthreads();
+    const int i0 = blockIdx.x * per_block;
+    const int i1 = static_cast<int>(min(static_cast<long long>(n), static_cast<long long>(i0) + per_block));
+    for (int i = i0 + static_cast<int>(threadIdx.x); i < i1; i += kHBlock) {
+        const BinTerms t = bin_terms(sx, sy, rx, ry, rows ? rows[i] : i, x_min, y_min, dx, dy, tnx, tny);
+        atomicAdd(l_n + t.bin, 1);
+        atomicAdd(l_off + t.bin, t.off);
+        atomicAdd(l_s + t.bin, t.s);
+        atomicAdd(l_c + t.bin, t.c);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += kHBlock) {
+        const int k = l_n[b];
+        if (k == 0) continue;
+        const long long cell = static_cast<long long>(b / tny + x_start) * grid_ny + (b % tny + y_start);
+        atomicAdd(count + cell, k);
+        unsafeAtomicAdd(osum + cell, l_off[b]);
+        unsafeAtomicAdd(ssum + cell, l_s[b]);
+        unsafeAtomicAdd(csum + cell, l_c[b]);
+    }
+}
+
+// tiles whose bins do not fit in LDS: one global atomic per row and sum
+__global__ void __launch_bounds__(kBlock)
+k_bin_headers(const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ rx,
+              const double* __restrict__ ry, const int32_t* __restrict__ rows, int n, double x_min,
+              double y_min, double dx, double dy, int x_start, int y_start, int tnx, int tny, int grid_ny,
+              int* __restrict__ count, double* __restrict__ osum, double* __restrict__ ssum,
+              double* __restrict__ csum) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const BinTerms t = bin_terms(sx, sy, rx, ry, rows ? rows[i] : i, x_min, y_min, dx, dy, tnx, tny);
+    const long long cell = static_cast<long long>(t.bin / tny + x_start) * grid_ny + (t.bin % tny + y_start);
     atomicAdd(count + cell, 1);
-    unsafeAtomicAdd(osum + cell, off);
-    unsafeAtomicAdd(ssum + cell, s);
-    unsafeAtomicAdd(csum + cell, c);
+    unsafeAtomicAdd(osum + cell, t.off);
+    unsafeAtomicAdd(ssum + cell, t.s);
+    unsafeAtomicAdd(csum + cell, t.c);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -164,10 +226,29 @@ int pstm_hip_bin_headers(const double* source_x, const double* source_y, const d
         !azimuth_cos_sum)
         return fail(1, "pstm_hip_bin_headers: null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_bin_headers, dim3((n_traces + kBlock - 1) / kBlock), dim3(kBlock), 0, st, source_x,
-                       source_y, receiver_x, receiver_y, trace_index, n_traces, x_min, y_min, dx, dy, x_start,
-                       y_start, tile_nx, tile_ny, grid_ny, trace_count, offset_sum, azimuth_sin_sum,
-                       azimuth_cos_sum);
+    const long long nb = static_cast<long long>(tile_nx) * tile_ny;
+    const size_t lds = static_cast<size_t>(nb) * (3 * sizeof(double) + sizeof(int));
+    if (lds <= kHeaderLds) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            OUT_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(k_bin_headers_lds),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kHeaderLds)));
+            attr_set = true;
+        }
+        // about 4096 rows per workgroup: the zeroing and flush of the LDS bins
+        // stay small next to the rows
+        const int blocks = static_cast<int>(std::min<long long>((n_traces + 4095) / 4096, 1024));
+        const int per_block = (n_traces + blocks - 1) / blocks;
+        hipLaunchKernelGGL(k_bin_headers_lds, dim3(blocks), dim3(kHBlock), lds, st, source_x, source_y,
+                           receiver_x, receiver_y, trace_index, n_traces, per_block, x_min, y_min, dx, dy, x_start,
+                           y_start, tile_nx, tile_ny, grid_ny, trace_count, offset_sum, azimuth_sin_sum,
+                           azimuth_cos_sum);
+    } else {
+        hipLaunchKernelGGL(k_bin_headers, dim3((n_traces + kBlock - 1) / kBlock), dim3(kBlock), 0, st, source_x,
+                           source_y, receiver_x, receiver_y, trace_index, n_traces, x_min, y_min, dx, dy, x_start,
+                           y_start, tile_nx, tile_ny, grid_ny, trace_count, offset_sum, azimuth_sin_sum,
+                           azimuth_cos_sum);
+    }
     OUT_TRY(hipGetLastError());
     return 0;
 }
