@@ -333,6 +333,65 @@ def output_section(kernel, ex) -> dict:
             "files": ["migrated_stack.zarr", "fold.zarr", "bin_headers.parquet"]}
 
 
+def time_variant_section(kernel, job, uniform_image, uniform_ms: float, uniform_e: int) -> dict:
+    """The same job migrated on the time-variant windows of the reference's
+    default frequency table (80 Hz at 0 s to 20 Hz at 5 s): launch shape
+    autotuned for the window axis, E counted, then one plan timed with events
+    around one graph replay after the capturing run; the window image
+    resampled onto the uniform axis on the device and compared with the
+    uniform image of the main measurement."""
+    import torch
+
+    from pm_amd import time_variant as tv
+    from pm_amd.autotune import autotune
+    from pm_amd.pipeline.executor import TileExecutor
+
+    t = job.t_axis_ms
+    table = tv.FrequencyTimeTable.from_pairs(tv.CONFIG_DEFAULT_TABLE)
+    w = tv.compute_time_windows(float(t[0]), float(t[-1]), float(t[1] - t[0]), table)
+    ex = TileExecutor(kernel, job, time_windows=w)
+    tune = autotune(kernel, ex, candidates=[(2, 8, 0), (2, 16, 1), (1, 8, 0), (1, 16, 1)])
+    cplan = ex.plan(count_effective=True)
+    cplan.run(use_graph=False)
+    torch.cuda.synchronize()
+    e_tv = int(cplan.counter.item())
+    cplan.destroy()
+    plan = ex.plan()
+    ex.reset()
+    plan.run()
+    torch.cuda.synchronize()
+    ex.reset()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    plan.run()
+    b.record()
+    b.synchronize()
+    tv_ms = a.elapsed_time(b)
+    plan.destroy()
+    img, _ = ex.assemble()
+    a.record()
+    full = ex.assemble_uniform()
+    b.record()
+    b.synchronize()
+    rs_ms = a.elapsed_time(b)
+    ref = uniform_image.double()
+    rel = float((full.double() - ref).norm() / ref.norm())
+    n_tv = int(ex.nt)
+    nbytes = img.numel() * 4 + full.numel() * 4
+    log(f"time-variant: {len(w)} windows, {n_tv} of {len(t)} samples, {tv_ms:.0f} ms vs {uniform_ms:.0f} ms, "
+        f"rel-L2 {rel:.3e}")
+    del ex, img, full
+    torch.cuda.empty_cache()
+    return {"table": [list(p) for p in tv.CONFIG_DEFAULT_TABLE], "windows": len(w),
+            "factors": [x.downsample_factor for x in w], "samples": n_tv, "uniform_samples": len(t),
+            "migrate_ms": round(tv_ms, 1), "uniform_migrate_ms": round(uniform_ms, 1),
+            "speedup": round(uniform_ms / tv_ms, 3), "effective_contributions": e_tv,
+            "effective_fraction_of_uniform": round(e_tv / uniform_e, 4),
+            "launch_shape": {"mw": tune.mw, "batch": tune.batch, "pipe": tune.pipe},
+            "resample_ms": round(rs_ms, 3), "resample_GBps_incl_assemble": round(nbytes / rs_ms / 1e6, 1),
+            "rel_l2_vs_uniform_image": round(rel, 5)}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -356,6 +415,8 @@ def main() -> None:
     ap.add_argument("--train-steps", type=int, default=80_000)
     ap.add_argument("--train-batch", type=int, default=16_384)
     ap.add_argument("--node-spacing", type=int, default=16)
+    ap.add_argument("--no-time-variant", dest="time_variant", action="store_false",
+                    help="skip the time-variant sampling section")
     ap.add_argument("--no-output", dest="output", action="store_false",
                     help="skip the output-step section (header sums, normalisation, finalize write-out)")
     ap.add_argument("--stream", action="store_true",
@@ -471,6 +532,11 @@ def main() -> None:
           "avg_launch_ms": round(launch_ms, 3), "launches_timed": launches,
           "frac_of_unpacked_fp32": round(achieved / roofline.MI355X_FP32_VALU_UNPACKED_TFLOPS, 4)}
     output = output_section(kernel, ex) if args.output else None
+    tvar = None
+    if args.time_variant:
+        img_u, _ = ex.assemble()
+        tvar = time_variant_section(kernel, job, img_u, ms_step, e_total)
+        del img_u
     # the VTI job has its own launch shape: back to the built-in choice
     lib.pstm_hip_set_tuning(0, 0)
     lib.pstm_hip_set_pipeline(-1)
@@ -498,6 +564,8 @@ def main() -> None:
         out["streamed"] = streamed
     if output is not None:
         out["output"] = output
+    if tvar is not None:
+        out["time_variant"] = tvar
     if tune is not None:
         out["autotune"] = tune.as_dict()
     if cpu:

@@ -277,6 +277,18 @@ int pstm_hip_select_rows(const double* source_x, const double* source_y, const d
 int pstm_hip_fold_normalize(const float* image, const int32_t* fold, long long n_pillars, int32_t nt, float* out,
                             void* stream);
 
+/* Time-variant output back onto a uniform axis (resample_to_uniform,
+ * pstm/algorithm/time_variant.py:354-397, called at
+ * pstm/pipeline/executor.py:1720-1726): for pillars p < n_pillars and output
+ * samples k < n_out, out[p, k] = np.interp(out_times_ms[k], tv_times_ms,
+ * tv_image[p, :]) computed in f64 with numpy's operations (first / last value
+ * outside the time-variant axis) and rounded to f32; tv_image is
+ * (n_pillars, n_tv) f32, tv_times_ms must increase.  All zeros when n_tv < 2,
+ * as the reference leaves its image then. */
+int pstm_hip_resample_uniform(const float* tv_image, const double* tv_times_ms, int32_t n_tv,
+                              const double* out_times_ms, int32_t n_out, long long n_pillars, float* out,
+                              void* stream);
+
 /* ---------------------------------------------------------------------------
  * Traveltime surrogate (no counterpart in the reference: the learned part of
  * this backend).  A 5-64-64-64-64-2 SiLU MLP; inputs are the encoded

@@ -238,6 +238,14 @@ __global__ void k_prep_eta(PstmMigrateArgs a, int nev, float2* ek) {
     ek[e] = make_float2((float)(2.0 * eta), (float)(1.0 + 2.0 * eta));
 }
 
+// f64 distance rounded operation by operation, as the oracle's unfused
+// arithmetic: the __d*_rn intrinsics alone do not keep the compiler from
+// fusing a product and a sum into one FMA (HIP contracts by default).
+__device__ __forceinline__ double dist64(double dx, double dy) {
+#pragma clang fp contract(off)
+    return __dsqrt_rn(dx * dx + dy * dy);
+}
+
 // Fold: count of traces with f64 midpoint distance <= max aperture, one
 // workgroup per pillar (numba_cpu_optimized.py:167-175,231).  The products and
 // sums are kept unfused so the test is the oracle's, bit for bit.
@@ -250,7 +258,7 @@ __global__ __launch_bounds__(256) void k_fold(PstmMigrateArgs a) {
         const int j = a.trace_index ? a.trace_index[jj] : jj;
         const double dx = __dsub_rn(ox, a.midpoint_x[j]);
         const double dy = __dsub_rn(oy, a.midpoint_y[j]);
-        const double d = __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
+        const double d = dist64(dx, dy);
         c += (d <= a.max_aperture_m) ? 1 : 0;
     }
     __shared__ int red[256];
@@ -335,10 +343,12 @@ __global__ __launch_bounds__(256) void k_integrate(PstmMigrateArgs a, int rowlen
 template <int TT>
 __device__ __forceinline__ bool exact_decision(const KParams& P, const TileView& T, int pil, int row, int k,
                                                double* u_out) {
+    // every product and sum rounded on its own, as in the oracle
+#pragma clang fp contract(off)
     const int ix = pil / T.ny, iy = pil % T.ny;
     const double ox = T.xax[ix], oy = T.yax[iy];
-    const double dxm = __dsub_rn(ox, P.mx64[row]), dym = __dsub_rn(oy, P.my64[row]);
-    const double dist_mid = __dsqrt_rn(__dadd_rn(__dmul_rn(dxm, dxm), __dmul_rn(dym, dym)));
+    const double dxm = ox - P.mx64[row], dym = oy - P.my64[row];
+    const double dist_mid = dist64(dxm, dym);
     const double v = T.vrms64[(size_t)pil * T.kc_pstride + k];
     const double t_s = P.tax64[k] / 1000.0;
     double ap = v * t_s * 0.5 * P.tan_dip;
@@ -346,8 +356,8 @@ __device__ __forceinline__ bool exact_decision(const KParams& P, const TileView&
     else if (ap > P.max_ap64) ap = P.max_ap64;
     const double dxs = ox - P.sx64[row], dys = oy - P.sy64[row];
     const double dxr = ox - P.rx64[row], dyr = oy - P.ry64[row];
-    const double ds2 = __dadd_rn(__dmul_rn(dxs, dxs), __dmul_rn(dys, dys));
-    const double dr2 = __dadd_rn(__dmul_rn(dxr, dxr), __dmul_rn(dyr, dyr));
+    const double ds2 = dxs * dxs + dys * dys;
+    const double dr2 = dxr * dxr + dyr * dyr;
     double t;
     if (TT == PSTM_TT_STRAIGHT) {
         // straight ray (numba_cpu_optimized.py:192-194); a curved-ray request
@@ -839,7 +849,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     const int trow = T.tidx ? T.tidx[j0 + pt_j] : j0 + pt_j;
                     const double dx64 = __dsub_rn(pox64, P.mx64[trow]);
                     const double dy64 = __dsub_rn(poy64, P.my64[trow]);
-                    const double dm64 = __dsqrt_rn(__dadd_rn(__dmul_rn(dx64, dx64), __dmul_rn(dy64, dy64)));
+                    const double dm64 = dist64(dx64, dy64);
                     if (!(dm64 > P.max_ap64)) {
                         inap = 1;
                         const float4 g0 = T.geo[2 * (j0 + pt_j)];

@@ -49,9 +49,12 @@ constexpr int kBlock = 256;
 constexpr size_t kHeaderLds = 160 * 1024;   // LDS bins of one tile: up to 5851 bins
 
 // offset = sqrt(dx^2 + dy^2), rounded as NumPy rounds it (compute_offset,
-// pstm/utils/units.py:180-200)
+// pstm/utils/units.py:180-200).  Contraction is switched off wherever the
+// reference's rounding matters: the __d*_rn intrinsics alone do not keep the
+// compiler from fusing a product and a sum into one FMA.
 __device__ __forceinline__ double offset_of(double dx, double dy) {
-    return __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
+#pragma clang fp contract(off)
+    return __dsqrt_rn(dx * dx + dy * dy);
 }
 
 struct BinTerms {
@@ -197,6 +200,65 @@ k_fold_normalize1(const float* image, const int32_t* __restrict__ fold, long lon
     }
 }
 
+// Time-variant image back onto the uniform axis: out[p, k] is np.interp of
+// x_out[k] over (xp, image[p, :]) with numpy's arithmetic in f64 — the
+// bracket j is the last xp[j] <= x; below xp[0] the first value, above the
+// last xp the last value, on a node its value, else
+// slope * (x - xp[j]) + fp[j] with slope = (fp[j+1] - fp[j]) / (xp[j+1] - xp[j])
+// (and the form from xp[j+1] if that is NaN) — rounded to f32.  The bracket
+// is searched once per output sample and reused for kRsPil pillars.
+constexpr int kRsPil = 16;
+
+__global__ void __launch_bounds__(kBlock)
+k_resample_uniform(const float* __restrict__ tv, const double* __restrict__ xp, int n_tv,
+                   const double* __restrict__ x_out, int n_out, long long n_pillars, float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.y * kBlock + threadIdx.x;
+    if (k >= n_out) return;
+    const long long p0 = static_cast<long long>(blockIdx.x) * kRsPil;
+    const long long p1 = p0 + kRsPil < n_pillars ? p0 + kRsPil : n_pillars;
+    if (n_tv < 2) {  // the reference leaves the uniform image zero
+        for (long long p = p0; p < p1; ++p) out[p * n_out + k] = 0.f;
+        return;
+    }
+    const double x = x_out[k];
+    // 0: interpolate at j; 1: value at j
+    int mode = 1, j;
+    if (x > xp[n_tv - 1]) {
+        j = n_tv - 1;
+    } else if (x < xp[0]) {
+        j = 0;
+    } else {
+        int lo = 0, hi = n_tv;  // first index with xp > x, by bisection
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (x >= xp[mid]) lo = mid + 1;
+            else hi = mid;
+        }
+        j = lo - 1;
+        mode = (j == n_tv - 1 || xp[j] == x) ? 1 : 0;
+    }
+    const double x0 = xp[j];
+    const double x1 = mode ? x0 : xp[j + 1];
+#pragma unroll 4
+    for (long long p = p0; p < p1; ++p) {
+        const float* fp = tv + p * n_tv;
+        double r;
+        if (mode) {
+            r = fp[j];
+        } else {
+            const double f0 = fp[j], f1 = fp[j + 1];
+            const double slope = (f1 - f0) / (x1 - x0);
+            r = slope * (x - x0) + f0;
+            if (isnan(r)) {
+                r = slope * (x - x1) + f1;
+                if (isnan(r) && f0 == f1) r = f0;
+            }
+        }
+        out[p * n_out + k] = static_cast<float>(r);
+    }
+}
+
 int grid_for(long long work) {
     long long b = (work + kBlock - 1) / kBlock;
     if (b > 16384) b = 16384;
@@ -288,6 +350,24 @@ int pstm_hip_fold_normalize(const float* image, const int32_t* fold, long long n
     } else {
         hipLaunchKernelGGL(k_fold_normalize1, dim3(grid_for(n)), dim3(kBlock), 0, st, image, fold, n, nt, out);
     }
+    OUT_TRY(hipGetLastError());
+    return 0;
+}
+
+int pstm_hip_resample_uniform(const float* tv_image, const double* tv_times_ms, int32_t n_tv,
+                              const double* out_times_ms, int32_t n_out, long long n_pillars, float* out,
+                              void* stream) {
+    pstm_set_last_error("");
+    if (n_tv < 0 || n_out < 0 || n_pillars < 0) return fail(1, "pstm_hip_resample_uniform: negative size");
+    if (n_out == 0 || n_pillars == 0) return 0;
+    if (!out || !out_times_ms || (n_tv >= 2 && (!tv_image || !tv_times_ms)))
+        return fail(1, "pstm_hip_resample_uniform: null pointer");
+    const long long groups = (n_pillars + kRsPil - 1) / kRsPil;
+    const int kblocks = (n_out + kBlock - 1) / kBlock;
+    if (groups > 0x7FFFFFFFLL || kblocks > 65535) return fail(1, "pstm_hip_resample_uniform: volume too large");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_resample_uniform, dim3(static_cast<unsigned>(groups), kblocks), dim3(kBlock), 0, st,
+                       tv_image, tv_times_ms, n_tv, out_times_ms, n_out, n_pillars, out);
     OUT_TRY(hipGetLastError());
     return 0;
 }

@@ -75,7 +75,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
     "pstm_hip_set_tuning", "pstm_hip_set_pipeline", "pstm_hip_last_launch",
     "pstm_hip_plan_create", "pstm_hip_plan_run", "pstm_hip_plan_info", "pstm_hip_plan_destroy",
-    "pstm_hip_bin_headers", "pstm_hip_select_rows", "pstm_hip_fold_normalize",
+    "pstm_hip_bin_headers", "pstm_hip_select_rows", "pstm_hip_fold_normalize", "pstm_hip_resample_uniform",
     "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_set_precision", "pstm_mlp_forward",
     "pstm_mlp_forward_mfma", "pstm_mlp_loss_grad",
     "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_vti_leg_times",
@@ -140,6 +140,8 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_hip_select_rows.argtypes = [_vp, _vp, _vp, _vp, _vp, i32, i32, d, d, d, d, d, d, _vp, _vp]
     lib.pstm_hip_fold_normalize.restype = ctypes.c_int
     lib.pstm_hip_fold_normalize.argtypes = [_vp, _vp, ctypes.c_longlong, i32, _vp, _vp]
+    lib.pstm_hip_resample_uniform.restype = ctypes.c_int
+    lib.pstm_hip_resample_uniform.argtypes = [_vp, _vp, i32, _vp, i32, ctypes.c_longlong, _vp, _vp]
     f = ctypes.c_float
     lib.pstm_mlp_param_count.restype = ctypes.c_int
     lib.pstm_mlp_param_count.argtypes = []

@@ -143,6 +143,51 @@ class HipKernel:
                          config: KernelConfig | None = None) -> KernelMetrics:
         return self._migrate(traces, output, velocity, config, _abi.TT_VTI)
 
+    def migrate_tile_time_variant(self, traces: TraceBlock, output: OutputTile, velocity: VelocitySlice,
+                                  config: KernelConfig | None, windows) -> KernelMetrics:
+        """Migrate only the samples of the time windows
+        (pm_amd.time_variant.compute_time_windows): ``output.image`` is
+        replaced by the (nx, ny, n_window_samples) image — f64 NumPy, or the
+        f32 device tensor for a device-resident tile — for the caller to
+        resample (``pm_amd.time_variant.resample_to_uniform``), as the
+        reference's executor does with its Metal kernel
+        (pstm/pipeline/executor.py:1710-1726,
+        pstm/kernels/metal_compiled.py:940-1230); the pillar fold is added to
+        ``output.fold``.  Exact straight-ray, curved-ray and VTI traveltimes."""
+        if not windows:
+            raise ValueError("time-variant migration needs at least one window")
+        return self._migrate(traces, output, velocity, config, None, windows=windows)
+
+    def resample_uniform(self, tv_image, tv_times_ms, out_times_ms, out=None):
+        """A (..., n_tv) f32 device image on the time-variant axis
+        ``tv_times_ms`` resampled onto ``out_times_ms`` as np.interp does
+        (f64 arithmetic, f32 result): a new (..., n_out) tensor, or ``out``."""
+        self._require_init()
+        torch = _torch()
+        lib = self._lib
+        if tv_image.dtype != torch.float32 or not tv_image.is_cuda or not tv_image.is_contiguous():
+            raise ValueError("tv_image must be a contiguous float32 device tensor")
+        xp = np.asarray(tv_times_ms, dtype=np.float64)
+        x = np.asarray(out_times_ms, dtype=np.float64)
+        n_tv = int(tv_image.shape[-1])
+        if xp.shape != (n_tv,):
+            raise ValueError(f"tv_times_ms has {xp.size} samples, the image {n_tv}")
+        if n_tv > 1 and np.any(np.diff(xp) <= 0):
+            raise ValueError("tv_times_ms must increase")
+        shape = tuple(tv_image.shape[:-1]) + (x.size,)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float32, device=self.device)
+        elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous float32 tensor of shape {shape}")
+        n_pil = int(np.prod(shape[:-1], dtype=np.int64))
+        xp_d = _dev(xp, torch.float64, self.device)
+        x_d = _dev(x, torch.float64, self.device)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self._device_index).cuda_stream)
+        rc = lib.pstm_hip_resample_uniform(_ptr(tv_image), _ptr(xp_d), n_tv, _ptr(x_d), int(x.size), n_pil,
+                                           _ptr(out), stream)
+        self._check(lib, rc, "pstm_hip_resample_uniform")
+        return out
+
     def synchronize(self) -> None:
         if self._lib is not None:
             stream = _torch().cuda.current_stream(self._device_index).cuda_stream
@@ -399,7 +444,7 @@ class HipKernel:
             raise ValueError(f"unknown kernel_type {config.kernel_type!r}") from None
 
     def _migrate(self, traces: TraceBlock, output: OutputTile, velocity: VelocitySlice,
-                 config: KernelConfig | None, traveltime: int | None) -> KernelMetrics:
+                 config: KernelConfig | None, traveltime: int | None, windows=None) -> KernelMetrics:
         self._require_init()
         torch = _torch()
         cfg = config if config is not None else self._config
@@ -407,6 +452,12 @@ class HipKernel:
         dev = self.device
         nx, ny, nt = output.nx, output.ny, output.nt
         t_axis = np.asarray(output.t_axis_ms, dtype=np.float64)
+        t_index = None
+        if windows is not None:
+            from pm_amd.time_variant import sample_times, uniform_index
+
+            t_tv = sample_times(windows)
+            t_index = uniform_index(t_tv, t_axis)
         # velocity onto the output grid (numba_cpu_optimized.py:569-578)
         if velocity.is_1d:
             v = np.asarray(velocity.vrms, dtype=np.float64)
@@ -423,12 +474,22 @@ class HipKernel:
         tt = self._traveltime(cfg) if traveltime is None else traveltime
         eta = None
         if tt == _abi.TT_VTI and cfg.vti_eta_array is not None:
-            eta = _dev(np.asarray(cfg.vti_eta_array, dtype=np.float64), torch.float64, dev)
+            eta = np.asarray(cfg.vti_eta_array, dtype=np.float64)
+        if t_index is not None:
+            # the window samples, with the velocity (and eta) of the uniform
+            # sample each one falls on
+            t_axis = t_tv
+            v = np.ascontiguousarray(np.asarray(v)[..., t_index])
+            if eta is not None and eta.ndim >= 1 and eta.shape[-1] == nt:
+                eta = np.ascontiguousarray(eta[..., t_index])
+            nt = len(t_tv)
+        if eta is not None:
+            eta = _dev(eta, torch.float64, dev)
 
         dt = DeviceTraces.from_block(traces, dev)
         f64 = torch.float64
         on_device = isinstance(output.image, torch.Tensor) and output.image.is_cuda
-        if on_device:
+        if on_device and t_index is None:
             image = output.image
             if image.dtype != torch.float32:
                 raise ValueError("device-resident OutputTile.image must be float32")
@@ -439,12 +500,17 @@ class HipKernel:
                             _dev(output.y_axis, f64, dev), _dev(t_axis, f64, dev),
                             _dev(v, f64, dev), cfg, traveltime=tt, eta=eta)
         if on_device:
+            if t_index is not None:
+                output.image = image
             f = output.fold
             f += fold2d.view(nx, ny, 1) if f.dim() == 3 else fold2d
         else:
             img_h = image.cpu().numpy()
             fold_h = fold2d.cpu().numpy()
-            output.image += img_h
+            if t_index is not None:
+                output.image = img_h.astype(np.float64)
+            else:
+                output.image += img_h
             if output.fold.ndim == 3:
                 output.fold += fold_h[:, :, None]
             else:

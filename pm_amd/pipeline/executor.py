@@ -51,7 +51,11 @@ class TileExecutor:
     """Runs one migration job tile by tile on the GPU through HipKernel."""
 
     def __init__(self, kernel: HipKernel, job, config: KernelConfig | None = None,
-                 spatial_sort: bool = True):
+                 spatial_sort: bool = True, time_windows=None):
+        """``time_windows`` (pm_amd.time_variant.compute_time_windows over the
+        job's output axis): migrate only the window samples; the images are
+        then (tnx, tny, n_window_samples) and ``assemble_uniform`` resamples
+        them onto the uniform axis."""
         import torch
 
         self.kernel = kernel
@@ -75,17 +79,31 @@ class TileExecutor:
         self.traces = DeviceTraces(amp, d(job.source_x), d(job.source_y), d(job.receiver_x),
                                    d(job.receiver_y), d(mx), d(my), None, job.dt_ms,
                                    job.t_start_ms)
-        self.t_axis = torch.as_tensor(job.t_axis_ms, dtype=f64, device=dev)
-        self.nt = len(job.t_axis_ms)
+        self.time_windows = list(time_windows) if time_windows else None
+        t_axis = np.asarray(job.t_axis_ms, dtype=np.float64)
+        tsel = None
+        if self.time_windows:
+            from pm_amd.time_variant import sample_times, uniform_index
+
+            t_axis = sample_times(self.time_windows)
+            tsel = torch.as_tensor(uniform_index(t_axis, job.t_axis_ms), device=dev)
+        self.t_axis = torch.as_tensor(t_axis, dtype=f64, device=dev)
+        self.nt = len(t_axis)
         nx, ny = len(job.x_axis), len(job.y_axis)
         self.shape = (nx, ny, self.nt)
+
+        def on_axis(a):
+            # velocity / eta on the uniform axis, then at the window samples
+            return a if tsel is None else a.index_select(-1, tsel).contiguous()
+
         if job.cube is not None:
-            vfull = job.cube.resample(job.x_axis, job.y_axis, job.t_axis_ms, device=dev)
+            vfull = on_axis(job.cube.resample(job.x_axis, job.y_axis, job.t_axis_ms, device=dev))
         else:
             vfull = None
-            v1 = torch.as_tensor(job.vrms_1d, dtype=f64, device=dev)
+            v1 = on_axis(torch.as_tensor(job.vrms_1d, dtype=f64, device=dev))
         ecube = job.meta.get("eta_cube") if self.config.vti_enabled else None
-        efull = ecube.resample(job.x_axis, job.y_axis, job.t_axis_ms, device=dev) if ecube is not None else None
+        efull = (on_axis(ecube.resample(job.x_axis, job.y_axis, job.t_axis_ms, device=dev))
+                 if ecube is not None else None)
         tiles = job.tiles()
         slab = sum((x1 - x0) * (y1 - y0) for x0, x1, y0, y1 in tiles) * self.nt
         self.image_storage = torch.zeros(slab, dtype=torch.float32, device=dev)
@@ -120,6 +138,8 @@ class TileExecutor:
         the exact one."""
         from pm_amd.kernels import _abi
 
+        if surrogate is not None and self.time_windows:
+            raise ValueError("the learned traveltime needs a uniform output axis (no time windows)")
         total = 0
         for i in (range(len(self.tiles)) if tiles is None else tiles):
             t = self.tiles[i]
@@ -142,6 +162,8 @@ class TileExecutor:
         number of common-offset bins; all of them land in the shared image.
         ``row_range`` (r0, r1) keeps only the executor rows in [r0, r1) (one
         group of a streamed job)."""
+        if surrogate is not None and self.time_windows:
+            raise ValueError("the learned traveltime needs a uniform output axis (no time windows)")
         sel = range(len(self.tiles)) if tiles is None else tiles
         descs = []
         for t in (self.tiles[i] for i in sel):
@@ -258,6 +280,19 @@ class TileExecutor:
             img[t.x0:t.x1, t.y0:t.y1] = t.image
             fold[t.x0:t.x1, t.y0:t.y1] = t.fold
         return img, fold
+
+    def assemble_uniform(self):
+        """(nx, ny, nt_uniform) f32 image on the job's uniform axis: the
+        window samples resampled as the reference does after a time-variant
+        tile (pstm/pipeline/executor.py:1720-1726; np.interp onto
+        k * dt, k < nt), or ``assemble()``'s image without time windows."""
+        from pm_amd.time_variant import sample_times, uniform_times
+
+        img, _ = self.assemble()
+        if not self.time_windows:
+            return img
+        t = np.asarray(self.job.t_axis_ms, dtype=np.float64)
+        return self.kernel.resample_uniform(img, sample_times(self.time_windows), uniform_times(t[1] - t[0], len(t)))
 
     def host_traces(self, rows: np.ndarray | None = None) -> dict:
         """Host copies (in the executor's row order) for the CPU oracle."""
