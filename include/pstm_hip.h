@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PSTM_HIP_ABI_VERSION 2
+#define PSTM_HIP_ABI_VERSION 3
 
 /* traveltime: KernelConfig.kernel_type (pstm/kernels/base.py:220-229) */
 #define PSTM_TT_STRAIGHT 0 /* DSR, numba_cpu_optimized.py:189-194            */
@@ -42,6 +42,22 @@ extern "C" {
 /* anti-aliasing (the reference's registered kernel has none) */
 #define PSTM_AA_NONE 0
 #define PSTM_AA_TRIANGLE 1 /* triangle filter on double-integrated traces */
+
+/* trace interpolation: KernelConfig.interpolation_method, honoured by the
+ * reference's NumbaKernel through interpolate_sample
+ * (pstm/kernels/interpolation.py:276-356, numba_cpu.py:341,632).  The codes
+ * are the reference's get_method_code() numbers except that 0 is linear (the
+ * default of a zeroed struct) and 1 nearest.  Methods other than linear need
+ * the straight-ray traveltime, no anti-aliasing and no E counter, as in the
+ * reference, whose NumbaKernel is the only backend that honours them. */
+#define PSTM_INTERP_LINEAR 0   /* _linear_interp            :68-92   */
+#define PSTM_INTERP_NEAREST 1  /* _nearest_interp           :58-65   */
+#define PSTM_INTERP_CUBIC 2    /* Catmull-Rom, _cubic_interp :95-135 */
+#define PSTM_INTERP_SINC4 3    /* Hann-windowed sinc, half width 2, :154-203 */
+#define PSTM_INTERP_SINC8 4    /* half width 4             */
+#define PSTM_INTERP_SINC16 5   /* half width 8             */
+#define PSTM_INTERP_LANCZOS3 6 /* _lanczos_interp a = 3, :212-268 */
+#define PSTM_INTERP_LANCZOS5 7 /* a = 5                    */
 
 /* velocity slice layout: VelocitySlice.is_1d (pstm/kernels/base.py:180-194) */
 #define PSTM_VEL_1D 0 /* vrms (nt,)        */
@@ -116,7 +132,7 @@ typedef struct PstmMigrateArgs {
     const float* mlp_params;      /* PSTM_MLP_PARAMS f32 (network layout below);
                                      weights are rounded to bf16 on use       */
     int32_t mlp_node_spacing;     /* K: 8, 16, 32 or 64 output samples        */
-    int32_t mlp_reserved;         /* 0                                        */
+    int32_t interpolation;        /* PSTM_INTERP_* (0 = linear)               */
     double mlp_tau_max;
     double mlp_v_lo;
     double mlp_v_hi;

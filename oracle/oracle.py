@@ -46,6 +46,8 @@ def _load(fast: bool = False) -> ctypes.CDLL:
         getattr(lib, fn).argtypes = [ctypes.c_double] * 5
     lib.oracle_vti_leg.restype = ctypes.c_double
     lib.oracle_vti_leg.argtypes = [ctypes.c_double] * 4
+    lib.oracle_interp_sample.restype = ctypes.c_double
+    lib.oracle_interp_sample.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     lib.oracle_integrate_traces.restype = None
     lib.oracle_integrate_traces.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_void_p]
@@ -100,6 +102,7 @@ class MigrateInputs:
     curved_v0: float = 1500.0
     curved_k: float = 0.5
     eta_constant: float = 0.0
+    interpolation: int = 0           # PSTM_INTERP_* (0 = linear)
     _keep: list = field(default_factory=list, repr=False)
 
 
@@ -158,6 +161,7 @@ def migrate(inp: MigrateInputs, image_init: np.ndarray | None = None,
     a.max_dip_deg, a.taper_fraction = inp.max_dip_deg, inp.taper_fraction
     a.grid_dx, a.grid_dy = inp.grid_dx, inp.grid_dy
     a.curved_v0, a.curved_k, a.eta_constant = inp.curved_v0, inp.curved_k, inp.eta_constant
+    a.interpolation = int(inp.interpolation)
     image = (np.zeros((nx, ny, nt)) if image_init is None
              else np.array(image_init, dtype=np.float64, copy=True))
     nthr = threads if threads is not None else (os.cpu_count() or 1)
@@ -190,3 +194,10 @@ def integrate_traces(amp: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
 
 __all__ = ["MigrateInputs", "OracleResult", "migrate", "build", "curved_dsr", "vti_dsr",
            "vti_leg", "integrate_traces", "AA_TRIANGLE", "ETA_3D"]
+
+
+def interp_sample(trace, t: float, method: int) -> float:
+    """One sample of ``trace`` (rounded to f32) at fractional position t with
+    interpolation code ``method`` (PSTM_INTERP_*), FP64."""
+    tr = np.ascontiguousarray(trace, dtype=np.float32)
+    return float(_load().oracle_interp_sample(tr.ctypes.data, int(tr.size), float(t), int(method)))

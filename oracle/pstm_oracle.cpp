@@ -18,6 +18,9 @@
 //   * curved-ray traveltime   pstm/algorithm/curved_ray.py:19-52,239-262,287-316;
 //     weight velocity v0+k z   pstm/metal/shaders/pstm_curved_ray.metal:209,268-270
 //   * VTI traveltime          pstm/algorithm/anisotropy_vti.py:148-220
+//   * trace interpolation other than linear: interpolate_sample and its kernels
+//     (pstm/kernels/interpolation.py:58-318), as the reference's NumbaKernel
+//     applies it (pstm/kernels/numba_cpu.py:337-341, 632)
 //   * triangle anti-aliasing on double-integrated traces: NOT in the reference
 //     (its Metal shaders use a scalar weight, pstm_migration.metal:95-132).
 //     Method of Lumley, Claerbout & Bevc (1994): a triangle of half width L
@@ -125,9 +128,94 @@ inline double vti_one_way_time(double t0_one_way, double h, double v_nmo, double
     return std::sqrt(std::fmax(t2, 0.0));
 }
 
+// Trace interpolation of the reference's NumbaKernel (interpolate_sample,
+// pstm/kernels/interpolation.py:58-318) in f64 on an f32 trace, codes
+// PSTM_INTERP_* (the reference's get_method_code numbers with linear and
+// nearest swapped).
+inline double interp_linear(const float* tr, int ns, double t) {  // :68-92
+    if (t < 0.0 || t >= ns - 1) return 0.0;
+    const int i0 = (int)t;
+    const int i1 = i0 + 1;
+    if (i1 >= ns) return tr[i0];
+    const double frac = t - i0;
+    return (double)tr[i0] * (1.0 - frac) + (double)tr[i1] * frac;
+}
+
+inline double sinc_value(double x) {  // :138-143
+    if (std::fabs(x) < 1e-10) return 1.0;
+    return std::sin(M_PI * x) / (M_PI * x);
+}
+
+double interp_sinc(const float* tr, int ns, double t, int hw) {  // :146-191
+    if (t < hw || t >= ns - hw) return interp_linear(tr, ns, t);
+    const int ic = (int)(t + 0.5);
+    const double frac = t - ic;
+    double res = 0.0, norm = 0.0;
+    for (int k = -hw + 1; k <= hw; ++k) {
+        const int i = ic + k;
+        if (i < 0 || i >= ns) continue;
+        const double x = frac - k;
+        const double win = std::fabs(x) >= hw ? 0.0 : 0.5 * (1.0 + std::cos(M_PI * x / hw));
+        const double w = sinc_value(x) * win;
+        res += (double)tr[i] * w;
+        norm += w;
+    }
+    return norm > 1e-10 ? res / norm : 0.0;
+}
+
+double interp_lanczos(const float* tr, int ns, double t, int a) {  // :212-256
+    if (t < a || t >= ns - a) return interp_linear(tr, ns, t);
+    const int ic = (int)t;
+    const double frac = t - ic;
+    double res = 0.0, norm = 0.0;
+    for (int k = -a + 1; k <= a; ++k) {
+        const int i = ic + k;
+        if (i < 0 || i >= ns) continue;
+        const double x = frac - k;
+        double w;
+        if (std::fabs(x) < 1e-10) w = 1.0;
+        else if (std::fabs(x) >= a) w = 0.0;
+        else w = sinc_value(x) * sinc_value(x / a);
+        res += (double)tr[i] * w;
+        norm += w;
+    }
+    return norm > 1e-10 ? res / norm : 0.0;
+}
+
+double interp_sample(const float* tr, int ns, double t, int method) {  // :276-318
+    switch (method) {
+        case PSTM_INTERP_NEAREST: {  // :58-65
+            const int i = (int)(t + 0.5);
+            return (i < 0 || i >= ns) ? 0.0 : (double)tr[i];
+        }
+        case PSTM_INTERP_CUBIC: {  // Catmull-Rom, :95-135
+            if (t < 1.0 || t >= ns - 2) return interp_linear(tr, ns, t);
+            const int i = (int)t;
+            const double u = t - i;
+            const double p0 = tr[i - 1], p1 = tr[i], p2 = tr[i + 1], p3 = tr[i + 2];
+            const double u2 = u * u, u3 = u2 * u;
+            const double a0 = -0.5 * p0 + 1.5 * p1 - 1.5 * p2 + 0.5 * p3;
+            const double a1 = p0 - 2.5 * p1 + 2.0 * p2 - 0.5 * p3;
+            const double a2 = -0.5 * p0 + 0.5 * p2;
+            return a0 * u3 + a1 * u2 + a2 * u + p1;
+        }
+        case PSTM_INTERP_SINC4: return interp_sinc(tr, ns, t, 2);
+        case PSTM_INTERP_SINC8: return interp_sinc(tr, ns, t, 4);
+        case PSTM_INTERP_SINC16: return interp_sinc(tr, ns, t, 8);
+        case PSTM_INTERP_LANCZOS3: return interp_lanczos(tr, ns, t, 3);
+        case PSTM_INTERP_LANCZOS5: return interp_lanczos(tr, ns, t, 5);
+        default: return interp_linear(tr, ns, t);
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// One interpolated sample, for the golden-vector checks.
+double oracle_interp_sample(const float* trace, int ns, double t, int method) {
+    return interp_sample(trace, ns, t, method);
+}
 
 int oracle_abi_version(void) { return PSTM_HIP_ABI_VERSION; }
 size_t oracle_args_size(void) { return sizeof(PstmMigrateArgs); }
@@ -241,7 +329,9 @@ int oracle_migrate_tile(const PstmMigrateArgs* a, double* image64, int num_threa
                 const double t_sample = (t_total_s - t_start_in_s) / dt_in_s;
                 if (t_sample < 0.0 || t_sample >= ns - 1) continue;
                 double amp;
-                if (!aa) {
+                if (!aa && a->interpolation != PSTM_INTERP_LINEAR) {
+                    amp = interp_sample(trace, ns, t_sample, a->interpolation);
+                } else if (!aa) {
                     const int i0 = (int)t_sample;
                     const double frac = t_sample - i0;
                     amp = (double)trace[i0] * (1.0 - frac) + (double)trace[i0 + 1] * frac;

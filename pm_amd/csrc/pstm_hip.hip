@@ -119,6 +119,7 @@ struct KParams {
     float taper_off;             // x_t = d * slope + taper_off (taper_off = -(1-tf)/tf)
     int n_items;                 // work items over all tiles (pillar groups x time windows)
     int wpow;                    // 0, 1 or 2: power of 1/t in the weight
+    int interp;                  // PSTM_INTERP_*
     float idt, uoff, umax;       // u = t*idt + uoff, valid u in [0, umax)
     float umax_lo, umax_hi;      // umax -+ eps_u: band decided in f64
     float max_ap;
@@ -603,15 +604,161 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
     return t;
 }
 
+// ---------------------------------------------------------------------------
+// trace interpolation other than linear (interpolate_sample,
+// interpolation.py:276-318) at sample position u = i0 + f0 of an f32 row in
+// LDS, 0 <= i0 <= ns - 2, 0 <= f0 < 1 (both call sites guarantee it).  Every
+// threshold of the reference is a test on i0 alone (u < H <=> i0 < H,
+// u >= ns - H <=> i0 >= ns - H, int(u + 0.5) = i0 + (f0 >= 0.5)), so each
+// index below stays inside the row.  The wider kernels fall back to linear
+// near the trace ends, as the reference's do.  Windowed sinc and Lanczos
+// need sin(pi x) for the taps x = fr - k: sin(pi (fr - k)) = (-1)^k sin(pi fr),
+// and the window / second factor by angle addition with cos, sin(pi k / H)
+// from a table, so a contribution costs two transcendental evaluations, not
+// two per tap.
+// ---------------------------------------------------------------------------
+constexpr float kPiF = 3.14159265358979f;
+
+__device__ __forceinline__ float lerp_at(const float* row, int i0, float f0) {
+    return row[i0] * (1.0f - f0) + row[i0 + 1] * f0;
+}
+
+// {cos(pi k / H), sin(pi k / H)} for k = j - H + 1, j < 2 H
+template <int H>
+__device__ __forceinline__ float2 tap_trig(int j) {
+    if (H == 2) {
+        constexpr float c[4] = {0.0f, 1.0f, 0.0f, -1.0f};
+        constexpr float s[4] = {-1.0f, 0.0f, 1.0f, 0.0f};
+        return make_float2(c[j], s[j]);
+    } else if (H == 3) {
+        constexpr float c[6] = {-0.5f, 0.5f, 1.0f, 0.5f, -0.5f, -1.0f};
+        constexpr float s[6] = {-0.8660253882408142f, -0.8660253882408142f, 0.0f, 0.8660253882408142f,
+                                0.8660253882408142f, 0.0f};
+        return make_float2(c[j], s[j]);
+    } else if (H == 4) {
+        constexpr float c[8] = {-0.7071067690849304f, 0.0f, 0.7071067690849304f, 1.0f, 0.7071067690849304f, 0.0f,
+                                -0.7071067690849304f, -1.0f};
+        constexpr float s[8] = {-0.7071067690849304f, -1.0f, -0.7071067690849304f, 0.0f, 0.7071067690849304f, 1.0f,
+                                0.7071067690849304f, 0.0f};
+        return make_float2(c[j], s[j]);
+    } else if (H == 5) {
+        constexpr float c[10] = {-0.80901700258255f, -0.30901700258255005f, 0.30901700258255005f, 0.80901700258255f,
+                                 1.0f, 0.80901700258255f, 0.30901700258255005f, -0.30901700258255005f,
+                                 -0.80901700258255f, -1.0f};
+        constexpr float s[10] = {-0.5877852439880371f, -0.9510565400123596f, -0.9510565400123596f,
+                                 -0.5877852439880371f, 0.0f, 0.5877852439880371f, 0.9510565400123596f,
+                                 0.9510565400123596f, 0.5877852439880371f, 0.0f};
+        return make_float2(c[j], s[j]);
+    } else {
+        constexpr float c[16] = {-0.9238795042037964f, -0.7071067690849304f, -0.3826834261417389f, 0.0f,
+                                 0.3826834261417389f, 0.7071067690849304f, 0.9238795042037964f, 1.0f,
+                                 0.9238795042037964f, 0.7071067690849304f, 0.3826834261417389f, 0.0f,
+                                 -0.3826834261417389f, -0.7071067690849304f, -0.9238795042037964f, -1.0f};
+        constexpr float s[16] = {-0.3826834261417389f, -0.7071067690849304f, -0.9238795042037964f, -1.0f,
+                                 -0.9238795042037964f, -0.7071067690849304f, -0.3826834261417389f, 0.0f,
+                                 0.3826834261417389f, 0.7071067690849304f, 0.9238795042037964f, 1.0f,
+                                 0.9238795042037964f, 0.7071067690849304f, 0.3826834261417389f, 0.0f};
+        return make_float2(c[j], s[j]);
+    }
+}
+
+// Hann-windowed sinc of half width H (_sinc_interp, interpolation.py:154-191):
+// taps k = -H+1 .. H around the nearest sample, weights sinc(x) *
+// 0.5 (1 + cos(pi x / H)) (0 for |x| >= H), taps past the trace end left out,
+// normalised by the sum of the weights
+template <int H>
+__device__ __forceinline__ float sinc_at(const float* row, int i0, float f0, int ns) {
+    if (i0 < H || i0 >= ns - H) return lerp_at(row, i0, f0);
+    const bool up = f0 >= 0.5f;
+    const int ic = i0 + (up ? 1 : 0);           // in [H, ns - H]
+    const float fr = up ? f0 - 1.0f : f0;       // [-0.5, 0.5), exact
+    const float sp = sinpif(fr);
+    float sa, ca;
+    sincospif(fr / (float)H, &sa, &ca);
+    float res = 0.0f, norm = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2 * H; ++j) {
+        const int k = j - H + 1;
+        const int i = ic + k;                   // in [1, ns]
+        const float x = fr - (float)k;
+        const float2 cs = tap_trig<H>(j);
+        const float sinc = fabsf(x) < 1e-10f ? 1.0f : ((k & 1) ? -sp : sp) / (kPiF * x);
+        const float win = fabsf(x) >= (float)H ? 0.0f : 0.5f * (1.0f + fmaf(ca, cs.x, sa * cs.y));
+        const float w = i < ns ? sinc * win : 0.0f;
+        res = fmaf(row[min(i, ns - 1)], w, res);
+        norm += w;
+    }
+    return norm > 1e-10f ? res / norm : 0.0f;
+}
+
+// Lanczos of order A (_lanczos_interp, interpolation.py:222-256): taps
+// k = -A+1 .. A around floor(u), L(x) = sinc(x) sinc(x / A) (1 at x = 0, 0 for
+// |x| >= A), normalised by the sum of the weights
+template <int A>
+__device__ __forceinline__ float lanczos_at(const float* row, int i0, float f0, int ns) {
+    if (i0 < A || i0 >= ns - A) return lerp_at(row, i0, f0);
+    const float sp = sinpif(f0);
+    float sa, ca;
+    sincospif(f0 / (float)A, &sa, &ca);
+    float res = 0.0f, norm = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2 * A; ++j) {
+        const int k = j - A + 1;                // i0 + k in [1, ns - 1]
+        const float x = f0 - (float)k;
+        const float2 cs = tap_trig<A>(j);
+        const float snx = (k & 1) ? -sp : sp;               // sin(pi x)
+        const float sna = fmaf(sa, cs.x, -ca * cs.y);       // sin(pi x / A)
+        const float px = kPiF * x;
+        float w = (float)A * snx * sna / (px * px);
+        w = fabsf(x) < 1e-10f ? 1.0f : (fabsf(x) >= (float)A ? 0.0f : w);
+        res = fmaf(row[i0 + k], w, res);
+        norm += w;
+    }
+    return norm > 1e-10f ? res / norm : 0.0f;
+}
+
+// interpolators whose value jumps where the fraction crosses 0.5
+template <int INTERP>
+constexpr bool kHalfJump = INTERP == PSTM_INTERP_NEAREST || INTERP == PSTM_INTERP_SINC4 ||
+                           INTERP == PSTM_INTERP_SINC8 || INTERP == PSTM_INTERP_SINC16;
+
+template <int INTERP>
+__device__ __forceinline__ float interp_at(const float* row, int i0, float f0, int ns) {
+    if (INTERP == PSTM_INTERP_NEAREST) {
+        return row[i0 + (f0 >= 0.5f ? 1 : 0)];  // int(u + 0.5) (_nearest_interp, :58-65)
+    } else if (INTERP == PSTM_INTERP_CUBIC) {
+        // Catmull-Rom on 4 samples (_cubic_interp, interpolation.py:95-135)
+        if (i0 < 1 || i0 >= ns - 2) return lerp_at(row, i0, f0);
+        const float p0 = row[i0 - 1], p1 = row[i0], p2 = row[i0 + 1], p3 = row[i0 + 2];
+        const float a0 = -0.5f * p0 + 1.5f * p1 - 1.5f * p2 + 0.5f * p3;
+        const float a1 = p0 - 2.5f * p1 + 2.0f * p2 - 0.5f * p3;
+        const float a2 = -0.5f * p0 + 0.5f * p2;
+        return fmaf(fmaf(fmaf(a0, f0, a1), f0, a2), f0, p1);
+    } else if (INTERP == PSTM_INTERP_SINC4) {
+        return sinc_at<2>(row, i0, f0, ns);
+    } else if (INTERP == PSTM_INTERP_SINC8) {
+        return sinc_at<4>(row, i0, f0, ns);
+    } else if (INTERP == PSTM_INTERP_SINC16) {
+        return sinc_at<8>(row, i0, f0, ns);
+    } else if (INTERP == PSTM_INTERP_LANCZOS3) {
+        return lanczos_at<3>(row, i0, f0, ns);
+    } else {
+        return lanczos_at<5>(row, i0, f0, ns);
+    }
+}
+
 // Trace amplitude at sample position i0 + f0: linear interpolation
-// (interpolation.py:68-92), or, with anti-aliasing, the triangle filter of
-// half width L (from the operator time dip) on the f64 double running sum:
-// a blend of the integer half widths floor(L) and floor(L)+1, each a second
-// difference interpolated between i0 and i0+1.
-template <bool AA, typename Row>
+// (interpolation.py:68-92) or another interpolator (interp_at), or, with
+// anti-aliasing, the triangle filter of half width L (from the operator time
+// dip) on the f64 double running sum: a blend of the integer half widths
+// floor(L) and floor(L)+1, each a second difference interpolated between i0
+// and i0+1.
+template <bool AA, int INTERP, typename Row>
 __device__ __forceinline__ float contrib_amp(const KParams& P, const Row* row, int i0, float f0, const float4& f,
                                              float cs, float y1, float y2, const float2* invsq) {
-    if (!AA) {
+    if (!AA && INTERP != PSTM_INTERP_LINEAR) {
+        return interp_at<INTERP>(reinterpret_cast<const float*>(row), i0, f0, P.ns);
+    } else if (!AA) {
         const float s0 = row[i0];
         const float s1 = row[i0 + 1];
         return fmaf(f0, s1 - s0, s0);
@@ -655,7 +802,7 @@ __device__ __forceinline__ float contrib_weight(const KParams& P, float wtr, flo
 
 // W2: spreading and obliquity both on (weight factor cw / t^2, the common case)
 // as a compile-time constant; otherwise the power of 1/t (0 or 1) is read from P
-template <int MW, int TT, bool AA, bool COUNT, bool LOB, bool W2>
+template <int MW, int TT, bool AA, bool COUNT, bool LOB, bool W2, int INTERP = PSTM_INTERP_LINEAR>
 __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     using Row = typename std::conditional<AA, double, float>::type;
@@ -1038,6 +1185,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             bool near = in_ap && !below && u < P.umax_hi;
                             if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
                             if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
+                            // nearest sample and windowed sinc jump at fraction 0.5 (the
+                            // sample / the tap window moves by one): decided in f64 there
+                            if (kHalfJump<INTERP>)
+                                near = near || (in_ap && below && fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < P.eps_u);
                             // the learned time has no f64 counterpart to decide against
                             if (TT == PSTM_TT_VTI_LEARNED) near = false;
                             any_near |= near;
@@ -1050,7 +1201,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             const float us = ok ? u : 0.0f;
                             const int i0 = (int)us;
                             const float f0 = __builtin_amdgcn_fractf(us);
-                            const float amp = contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq);
+                            const float amp = contrib_amp<AA, INTERP>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq);
                             if (W2 || P.wpow != 0) {
                                 const float r = frcp(t);
                                 float wf = cw[p][m] * ((W2 || P.wpow == 2) ? r * r : r);
@@ -1083,15 +1234,19 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                 bool near = in_ap && u >= P.umax_lo && u < P.umax_hi;
                                 if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
                                 if (COUNT) near = near || (fabsf(xt - 1.0f) < P.eps_x);
+                                if (kHalfJump<INTERP>)
+                                    near = near || (in_ap && u < P.umax_lo &&
+                                                    fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < P.eps_u);
                                 if (!near) continue;
                                 const int pil = pil0 + pg * kPW + p;
                                 const int k = kbase + 64 * (tw + kTW * m) + lane;
                                 const int trow = T.tidx ? T.tidx[j0 + jj] : j0 + jj;
                                 double u64;
                                 if (exact_decision<TT>(P, T, pil, trow, k, &u64)) {
-                                    const int i0 = (int)u64;
-                                    const float f0 = (float)(u64 - (double)i0);
-                                    acc[p][m] = fmaf(contrib_amp<AA>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq),
+                                    // the nearest sample is rounded in f64: (int)(u + 0.5)
+                                    const int i0 = INTERP == PSTM_INTERP_NEAREST ? (int)(u64 + 0.5) : (int)u64;
+                                    const float f0 = INTERP == PSTM_INTERP_NEAREST ? 0.0f : (float)(u64 - (double)i0);
+                                    acc[p][m] = fmaf(contrib_amp<AA, INTERP>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq),
                                                      contrib_weight(P, e.w, xt, t, cw[p][m]), acc[p][m]);
                                     if (COUNT) ++cnt;
                                 }
@@ -1256,6 +1411,12 @@ int validate(const PstmMigrateArgs* a) {
     }
     if (a->traveltime == PSTM_TT_CURVED && !(a->curved_v0 > 0.0))
         return fail(1, "curved_v0 must be positive");
+    if (a->interpolation < PSTM_INTERP_LINEAR || a->interpolation > PSTM_INTERP_LANCZOS5)
+        return fail(1, "invalid interpolation");
+    if (a->interpolation != PSTM_INTERP_LINEAR &&
+        (a->traveltime != PSTM_TT_STRAIGHT || a->aa_mode != PSTM_AA_NONE || a->n_effective))
+        return fail(1, "interpolation other than linear needs the straight-ray traveltime, no anti-aliasing and "
+                       "no effective-contribution counter");
     if (!(a->dt_in_ms > 0.0)) return fail(1, "dt_in_ms must be positive");
     if (!(a->max_aperture_m > 0.0)) return fail(1, "max_aperture_m must be positive");
     if ((long long)a->nx * a->ny * a->nt > (1LL << 40)) return fail(1, "tile too large");
@@ -1266,9 +1427,9 @@ int validate(const PstmMigrateArgs* a) {
 // nothing but stream work happens while a plan's launch sequence is captured
 bool g_attr_only = false;
 
-template <int MW, int TT, bool AA, bool COUNT, bool LOB, bool W2>
+template <int MW, int TT, bool AA, bool COUNT, bool LOB, bool W2, int INTERP = PSTM_INTERP_LINEAR>
 int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
-    auto fn = k_migrate<MW, TT, AA, COUNT, LOB, W2>;
+    auto fn = k_migrate<MW, TT, AA, COUNT, LOB, W2, INTERP>;
     static size_t lds_set = 0;  // per instantiation
     if (lds > lds_set) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1347,6 +1508,7 @@ struct Shape {
     size_t tot_bytes = 0, lds = 0;
     int occupancy = 1;
     bool pipe = false;
+    int interp = PSTM_INTERP_LINEAR;
 };
 
 Shape launch_shape(const PstmMigrateArgs& a) {
@@ -1369,6 +1531,8 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     if (mw_cap <= 0) mw_cap = 4;  // measured best on the reflector job, with and without AA
     mw_cap = std::min(4, mw_cap);
     sh.mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
+    sh.interp = a.interpolation;
+    if (sh.interp != PSTM_INTERP_LINEAR) sh.mw = 2;  // the interpolators' one instantiation
     // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
     sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * sizeof(double);
     sh.lat_nodes = sh.learned ? 256 * sh.mw / a.mlp_node_spacing + 1 : 0;
@@ -1421,6 +1585,7 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.tot_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * sh.batch * sh.pair_vec * 16, 16);
     kp.taper_off = a.taper_fraction > 0.0 ? (float)(-(1.0 - a.taper_fraction) / a.taper_fraction) : -1e30f;
     kp.wpow = (a.apply_spreading ? 1 : 0) + (a.apply_obliquity ? 1 : 0);
+    kp.interp = a.interpolation;
     const double dt_s = a.dt_in_ms / 1000.0;
     kp.idt = (float)(1.0 / dt_s);
     kp.uoff = (float)(-(a.t_start_in_ms / 1000.0) / dt_s);
@@ -1550,7 +1715,24 @@ long tile_items(const PstmMigrateArgs& a, const Shape& sh) {
     return (long)((a.nx * a.ny + kNPil - 1) / kNPil) * sh.nwin;
 }
 
+// straight ray, no anti-aliasing, no counter, MW = 2 (launch_shape), every
+// lower-bound check: the one instantiation per interpolator
+template <int INTERP>
+int launch_interp(const KParams& kp, int grid, size_t lds, hipStream_t st) {
+    return launch_migrate<2, PSTM_TT_STRAIGHT, false, false, true, false, INTERP>(kp, grid, lds, st);
+}
+
 int dispatch(const Shape& sh, bool count, bool lob, const KParams& kp, int grid, hipStream_t st) {
+    switch (sh.interp) {
+        case PSTM_INTERP_NEAREST: return launch_interp<PSTM_INTERP_NEAREST>(kp, grid, sh.lds, st);
+        case PSTM_INTERP_CUBIC: return launch_interp<PSTM_INTERP_CUBIC>(kp, grid, sh.lds, st);
+        case PSTM_INTERP_SINC4: return launch_interp<PSTM_INTERP_SINC4>(kp, grid, sh.lds, st);
+        case PSTM_INTERP_SINC8: return launch_interp<PSTM_INTERP_SINC8>(kp, grid, sh.lds, st);
+        case PSTM_INTERP_SINC16: return launch_interp<PSTM_INTERP_SINC16>(kp, grid, sh.lds, st);
+        case PSTM_INTERP_LANCZOS3: return launch_interp<PSTM_INTERP_LANCZOS3>(kp, grid, sh.lds, st);
+        case PSTM_INTERP_LANCZOS5: return launch_interp<PSTM_INTERP_LANCZOS5>(kp, grid, sh.lds, st);
+        default: break;
+    }
     if (sh.tt == PSTM_TT_STRAIGHT && sh.aa) return dispatch_mw<PSTM_TT_STRAIGHT, true>(sh.mw, count, lob, kp, grid, sh.lds, st);
     if (sh.tt == PSTM_TT_STRAIGHT) return dispatch_mw<PSTM_TT_STRAIGHT, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
     if (sh.tt == PSTM_TT_CURVED) return dispatch_mw<PSTM_TT_CURVED, false>(sh.mw, count, lob, kp, grid, sh.lds, st);

@@ -13,12 +13,30 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 TT_STRAIGHT, TT_CURVED, TT_VTI, TT_VTI_LEARNED = 0, 1, 2, 3
 AA_NONE, AA_TRIANGLE = 0, 1
 VEL_1D, VEL_3D = 0, 1
 ETA_CONST, ETA_1D, ETA_3D = 0, 1, 2
+#: interpolation codes; names and aliases as the reference's get_method_code
+#: (pstm/kernels/interpolation.py:322-356), unknown names -> linear
+INTERP_LINEAR, INTERP_NEAREST, INTERP_CUBIC = 0, 1, 2
+INTERP_SINC4, INTERP_SINC8, INTERP_SINC16, INTERP_LANCZOS3, INTERP_LANCZOS5 = 3, 4, 5, 6, 7
+INTERP_CODES = {"nearest": INTERP_NEAREST, "linear": INTERP_LINEAR, "cubic": INTERP_CUBIC, "spline": INTERP_CUBIC,
+                "sinc4": INTERP_SINC4, "sinc8": INTERP_SINC8, "sinc": INTERP_SINC8, "sinc16": INTERP_SINC16,
+                "lanczos3": INTERP_LANCZOS3, "lanczos": INTERP_LANCZOS3, "lanczos5": INTERP_LANCZOS5}
+
+
+def interp_code(method) -> int:
+    """Code of an interpolation method name (or enum with a .value), linear
+    for None or an unknown name, as the reference maps them."""
+    if method is None:
+        return INTERP_LINEAR
+    name = getattr(method, "value", method)
+    return INTERP_CODES.get(str(name).lower().strip(), INTERP_LINEAR)
+
+
 SEL_OFFSET_MIN, SEL_OFFSET_MAX = 1, 2
 SEL_OVT_X_MIN, SEL_OVT_X_MAX, SEL_OVT_Y_MIN, SEL_OVT_Y_MAX = 4, 8, 16, 32
 
@@ -50,7 +68,7 @@ class PstmMigrateArgs(ctypes.Structure):
         ("grid_dx", ctypes.c_double), ("grid_dy", ctypes.c_double),
         ("curved_v0", ctypes.c_double), ("curved_k", ctypes.c_double),
         ("eta_constant", ctypes.c_double),
-        ("mlp_params", _vp), ("mlp_node_spacing", ctypes.c_int32), ("mlp_reserved", ctypes.c_int32),
+        ("mlp_params", _vp), ("mlp_node_spacing", ctypes.c_int32), ("interpolation", ctypes.c_int32),
         ("mlp_tau_max", ctypes.c_double), ("mlp_v_lo", ctypes.c_double), ("mlp_v_hi", ctypes.c_double),
         ("mlp_eta_max", ctypes.c_double), ("mlp_out_scale", ctypes.c_double),
     ]

@@ -398,6 +398,9 @@ class HipKernel:
             raise ValueError(f"unknown antialias method {config.antialias!r}")
         a.aa_mode = _abi.AA_TRIANGLE if aa == "triangle" else _abi.AA_NONE
         a.aa_max_half_width = int(config.aa_max_half_width)
+        # KernelConfig.interpolation_method, as the reference's NumbaKernel reads
+        # it (pstm/kernels/numba_cpu.py:632); straight ray without AA only
+        a.interpolation = _abi.interp_code(config.interpolation_method)
         a.apply_spreading = int(bool(config.apply_spreading))
         a.apply_obliquity = int(bool(config.apply_obliquity))
         a.dt_in_ms, a.t_start_in_ms = dt.sample_rate_ms, dt.start_time_ms

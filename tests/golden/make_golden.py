@@ -119,9 +119,35 @@ def _case(nco, nc, name, seed, *, n_traces=64, n_samples=101, dt_in=4.0,
           f"(numba_cpu kernel diff {np.abs(image - image_b).max():.3g})")
 
 
+def _interp_vectors() -> None:
+    """interpolate_sample (pstm/kernels/interpolation.py:276-318) for every
+    method code on an f32-representable trace, at random positions inside the
+    migration's sample bounds and at the edges of each method's linear
+    fallback, integer and half-integer positions."""
+    from pstm.kernels.interpolation import interpolate_sample
+
+    rng = np.random.default_rng(31)
+    ns = 41
+    trace = rng.standard_normal(ns).astype(np.float32).astype(np.float64)
+    special = [0.0, 0.5, 0.999, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0, 4.5, 5.0, 7.999, 8.0, 8.5,
+               20.0, 20.25, 20.5, 20.75, ns - 9.0, ns - 8.5, ns - 8.001, ns - 6.0, ns - 5.5,
+               ns - 5.0, ns - 4.5, ns - 4.0, ns - 3.5, ns - 3.0, ns - 2.5, ns - 2.0, ns - 1.5,
+               ns - 1.001]
+    pos = np.concatenate([rng.uniform(0.0, ns - 1.0, 300), np.array(special)])
+    codes = np.arange(8)
+    value = np.array([[interpolate_sample(trace, float(t), int(c)) for t in pos] for c in codes])
+    np.savez_compressed(OUT / "interp_methods.npz", trace=trace, pos=pos, reference_code=codes,
+                        value=value)
+    print("interpolation: ", value[:, :2])
+
+
 def main() -> None:
     _install_numba_standin()
     sys.path.insert(0, str(REF))
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "interp":
+        _interp_vectors()
+        return
+    _interp_vectors()
     import pstm.kernels.numba_cpu as nc
     import pstm.kernels.numba_cpu_optimized as nco
     from pstm.algorithm import anisotropy_vti as vti
