@@ -392,6 +392,46 @@ def time_variant_section(kernel, job, uniform_image, uniform_ms: float, uniform_
             "rel_l2_vs_uniform_image": round(rel, 5)}
 
 
+def interpolation_section(ex, methods=("linear", "cubic", "sinc8", "sinc16", "lanczos3")) -> dict:
+    """Cost of the trace interpolators: the job without anti-aliasing (the
+    interpolators other than linear need it off), one plan over its 4 busiest
+    tiles per method, timed with events around one eager run after a warm-up
+    run."""
+    from dataclasses import replace
+
+    import torch
+
+    from pm_amd.kernels import _abi
+
+    lib = _abi.load_library()
+    lib.pstm_hip_set_tuning(0, 0)       # each method at its built-in launch shape
+    lib.pstm_hip_set_pipeline(-1)
+    busiest = sorted(range(len(ex.tiles)), key=lambda i: -int(ex.tiles[i].rows.shape[0]))[:4]
+    saved = ex.config
+    out = {}
+    try:
+        for m in methods:
+            ex.config = replace(saved, antialias="none", interpolation_method=m)
+            plan = ex.plan(tiles=busiest)
+            ex.reset()
+            plan.run(use_graph=False)
+            torch.cuda.synchronize()
+            ex.reset()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            plan.run(use_graph=False)
+            b.record()
+            b.synchronize()
+            out[m] = round(a.elapsed_time(b), 1)
+            plan.destroy()
+    finally:
+        ex.config = saved
+        ex.reset()
+    log(f"interpolation (4 tiles, no AA, ms): {out}")
+    return {"ms_4_busiest_tiles_no_aa": out,
+            "relative_to_linear": {m: round(v / out["linear"], 3) for m, v in out.items()}}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -415,6 +455,8 @@ def main() -> None:
     ap.add_argument("--train-steps", type=int, default=80_000)
     ap.add_argument("--train-batch", type=int, default=16_384)
     ap.add_argument("--node-spacing", type=int, default=16)
+    ap.add_argument("--no-interpolation", dest="interpolation", action="store_false",
+                    help="skip the interpolator timing section")
     ap.add_argument("--no-time-variant", dest="time_variant", action="store_false",
                     help="skip the time-variant sampling section")
     ap.add_argument("--no-output", dest="output", action="store_false",
@@ -537,6 +579,7 @@ def main() -> None:
         img_u, _ = ex.assemble()
         tvar = time_variant_section(kernel, job, img_u, ms_step, e_total)
         del img_u
+    interp = interpolation_section(ex) if args.interpolation else None
     # the VTI job has its own launch shape: back to the built-in choice
     lib.pstm_hip_set_tuning(0, 0)
     lib.pstm_hip_set_pipeline(-1)
@@ -566,6 +609,8 @@ def main() -> None:
         out["output"] = output
     if tvar is not None:
         out["time_variant"] = tvar
+    if interp is not None:
+        out["interpolation"] = interp
     if tune is not None:
         out["autotune"] = tune.as_dict()
     if cpu:
