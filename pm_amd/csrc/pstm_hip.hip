@@ -138,6 +138,7 @@ struct KParams {
     int tt_req;                  // traveltime asked for (PSTM_TT_*)
     float lo_eps;                // eps_u where u >= 0 can be decided differently from f64, else -inf
     float eps_u, eps_x;          // f32 bands around the sample bounds and the aperture edge
+    float eps_half;              // relative band around fraction 0.5 (nearest / sinc)
     double tan_dip, min_ap64, max_ap64, t_start_s, dt_s, cv0, ck, eta_const;
     // learned VTI traveltime: network operand images and the t0 lattice
     const void* mlpA;            // bf16 weight operands (k_prep_mlp)
@@ -682,7 +683,7 @@ __device__ __forceinline__ float sinc_at(const float* row, int i0, float f0, int
         const int i = ic + k;                   // in [1, ns]
         const float x = fr - (float)k;
         const float2 cs = tap_trig<H>(j);
-        const float sinc = fabsf(x) < 1e-10f ? 1.0f : ((k & 1) ? -sp : sp) / (kPiF * x);
+        const float sinc = fabsf(x) < 1e-10f ? 1.0f : ((k & 1) ? -sp : sp) * __builtin_amdgcn_rcpf(kPiF * x);
         const float win = fabsf(x) >= (float)H ? 0.0f : 0.5f * (1.0f + fmaf(ca, cs.x, sa * cs.y));
         const float w = i < ns ? sinc * win : 0.0f;
         res = fmaf(row[min(i, ns - 1)], w, res);
@@ -709,13 +710,19 @@ __device__ __forceinline__ float lanczos_at(const float* row, int i0, float f0, 
         const float snx = (k & 1) ? -sp : sp;               // sin(pi x)
         const float sna = fmaf(sa, cs.x, -ca * cs.y);       // sin(pi x / A)
         const float px = kPiF * x;
-        float w = (float)A * snx * sna / (px * px);
+        float w = (float)A * snx * sna * __builtin_amdgcn_rcpf(px * px);
         w = fabsf(x) < 1e-10f ? 1.0f : (fabsf(x) >= (float)A ? 0.0f : w);
         res = fmaf(row[i0 + k], w, res);
         norm += w;
     }
     return norm > 1e-10f ? res / norm : 0.0f;
 }
+
+// sample blocks per wave of each interpolator's one instantiation (16 sinc
+// taps leave room for one block of accumulators)
+template <int INTERP>
+constexpr int kInterpMW = INTERP == PSTM_INTERP_SINC16 ? 1 : 2;
+inline int interp_mw(int interp) { return interp == PSTM_INTERP_SINC16 ? 1 : 2; }
 
 // interpolators whose value jumps where the fraction crosses 0.5
 template <int INTERP>
@@ -1188,7 +1195,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             // nearest sample and windowed sinc jump at fraction 0.5 (the
                             // sample / the tap window moves by one): decided in f64 there
                             if (kHalfJump<INTERP>)
-                                near = near || (in_ap && below && fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < P.eps_u);
+                                near = near || (in_ap && below &&
+                                                 fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < fmaf(u, P.eps_half, 1e-5f));
                             // the learned time has no f64 counterpart to decide against
                             if (TT == PSTM_TT_VTI_LEARNED) near = false;
                             any_near |= near;
@@ -1236,7 +1244,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                 if (COUNT) near = near || (fabsf(xt - 1.0f) < P.eps_x);
                                 if (kHalfJump<INTERP>)
                                     near = near || (in_ap && u < P.umax_lo &&
-                                                    fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < P.eps_u);
+                                                    fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < fmaf(u, P.eps_half, 1e-5f));
                                 if (!near) continue;
                                 const int pil = pil0 + pg * kPW + p;
                                 const int k = kbase + 64 * (tw + kTW * m) + lane;
@@ -1532,7 +1540,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     mw_cap = std::min(4, mw_cap);
     sh.mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
     sh.interp = a.interpolation;
-    if (sh.interp != PSTM_INTERP_LINEAR) sh.mw = 2;  // the interpolators' one instantiation
+    if (sh.interp != PSTM_INTERP_LINEAR) sh.mw = interp_mw(sh.interp);  // their one instantiation
     // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
     sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * sizeof(double);
     sh.lat_nodes = sh.learned ? 256 * sh.mw / a.mlp_node_spacing + 1 : 0;
@@ -1610,6 +1618,10 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     // at or below t_start: t_start > 0, a curved ray with k < 0, or VTI
     // (t = t0 for t0 <= 0, and negative output times give u < 0)
     kp.eps_u = (float)(2e-3 + 1e-5 * (double)(ns - 1));
+    // the straight-ray f32 position is within ~2.6e-7 u of the f64 one (CPU
+    // emulation of this arithmetic at 800 m tiles, 2500 m apertures, 3 km
+    // offsets); the fraction-0.5 band keeps a 4x margin
+    kp.eps_half = 1e-6f;
     kp.lo_eps = lo_check(a, sh) ? kp.eps_u : -INFINITY;
     kp.umax_lo = kp.umax - kp.eps_u;
     kp.umax_hi = kp.umax + kp.eps_u;
@@ -1715,11 +1727,12 @@ long tile_items(const PstmMigrateArgs& a, const Shape& sh) {
     return (long)((a.nx * a.ny + kNPil - 1) / kNPil) * sh.nwin;
 }
 
-// straight ray, no anti-aliasing, no counter, MW = 2 (launch_shape), every
-// lower-bound check: the one instantiation per interpolator
+// straight ray, no anti-aliasing, no counter, MW = kInterpMW (launch_shape),
+// every lower-bound check: the one instantiation per interpolator
 template <int INTERP>
 int launch_interp(const KParams& kp, int grid, size_t lds, hipStream_t st) {
-    return launch_migrate<2, PSTM_TT_STRAIGHT, false, false, true, false, INTERP>(kp, grid, lds, st);
+    return launch_migrate<kInterpMW<INTERP>, PSTM_TT_STRAIGHT, false, false, true, false, INTERP>(kp, grid, lds,
+                                                                                                  st);
 }
 
 int dispatch(const Shape& sh, bool count, bool lob, const KParams& kp, int grid, hipStream_t st) {
