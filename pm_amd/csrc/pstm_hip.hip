@@ -1921,13 +1921,15 @@ int plan_enqueue(PstmPlan* pl, hipStream_t st) {
 
 void plan_free(PstmPlan* pl) {
     if (!pl) return;
+    // A run may still be in flight on the caller's stream (HipPlan.run returns
+    // at once): wait before destroying its executable graph, which holds the
+    // gather's kernel arguments the running kernel still reads, and before
+    // freeing the plan's memory.
+    (void)hipDeviceSynchronize();
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     if (pl->cap) (void)hipStreamDestroy(pl->cap);
-    if (pl->mem) {
-        (void)hipDeviceSynchronize();
-        (void)hipFree(pl->mem);
-    }
+    if (pl->mem) (void)hipFree(pl->mem);
     delete pl;
 }
 

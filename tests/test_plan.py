@@ -191,3 +191,29 @@ def test_plan_other_traveltimes_equal_tile_runs(kernel, kind):
     img, fold = _state(ex)
     assert torch.equal(img, ref_img) and torch.equal(fold, ref_fold)
     plan.destroy()
+
+
+def test_temporary_plan_equals_kept_plan(kernel):
+    """``ex.plan().run()`` frees the plan as soon as run() returns, while its
+    graph may still be executing: destruction waits for it, so the image
+    equals that of a plan kept until after synchronisation, also when other
+    device allocations follow at once."""
+    import torch
+
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.synthetic import reflector_job
+
+    job = reflector_job(seed=9, n_traces=6000, n_samples=601, n_out=24, nt=500, d_out=25.0, tile=12,
+                        n_planes=6, n_diffractors=10, antialias="none", device="cuda")
+    ex = TileExecutor(kernel, job)
+    plan = ex.plan()
+    plan.run()
+    torch.cuda.synchronize()
+    ref, _ = _state(ex)
+    plan.destroy()
+    ex.reset()
+    ex.plan().run()                       # temporary: destroyed on return
+    scratch = [torch.full((1 << 24,), float(i), device="cuda") for i in range(4)]
+    img, _ = _state(ex)
+    del scratch
+    assert torch.equal(img, ref)

@@ -244,3 +244,33 @@ def test_drop_in_time_variant_tile(kernel):
              x_.sample_end, x_.f_max_hz) for x_ in w]
     np.testing.assert_array_equal(back, otv.resample(out.image.astype(np.float32), wins, 2.0,
                                                      len(job.t_axis_ms)).astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.gpu
+def test_time_variant_vti_equals_uniform_at_window_samples(kernel):
+    """A VTI job with 3-D velocity and eta: the window samples take the eta of
+    their uniform samples, so the time-variant image is the uniform image
+    there."""
+    import torch
+
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.synthetic import vti_diffractor_job
+
+    job = vti_diffractor_job(seed=6, n_traces=4000, n_samples=801, n_out=20, nt=600, d_out=25.0, tile=10,
+                             n_diffractors=12, device="cuda")
+    w = tv.compute_time_windows(0.0, float(job.t_axis_ms[-1]), 2.0, tv.FrequencyTimeTable.default())
+    k = torch.as_tensor(tv.uniform_index(tv.sample_times(w), job.t_axis_ms), device="cuda")
+    uni = TileExecutor(kernel, job)
+    uni.plan().run()
+    u_img, u_fold = uni.assemble()
+    ex = TileExecutor(kernel, job, time_windows=w)
+    assert ex.tiles[0].eta is not None and ex.tiles[0].eta.shape[-1] == len(k)
+    ex.plan().run()
+    t_img, t_fold = ex.assemble()
+    assert torch.equal(t_fold, u_fold)
+    ref = u_img[..., k]
+    rel = float((t_img.double() - ref.double()).norm() / ref.double().norm())
+    print("VTI time-variant vs uniform at window samples", rel)
+    assert rel < 1e-6
+    with pytest.raises(ValueError, match="uniform output axis"):
+        ex.run(surrogate=object())
