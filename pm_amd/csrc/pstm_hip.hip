@@ -1204,7 +1204,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             if (LOB) ok = ok && u >= 0.0f;
                             float w = ok ? e.w : 0.0f;
                             if (__any(ok && xt[m] > 0.0f)) {  // taper zone (wave-uniform test)
-                                if (xt[m] > 0.0f) w *= taper_weight(fminf(xt[m], 1.0f));
+                                // a select, not a branch: the polynomial runs for the
+                                // whole wave either way, without exec-mask juggling
+                                const float tw = taper_weight(__builtin_amdgcn_fmed3f(xt[m], 0.0f, 1.0f));
+                                w = xt[m] > 0.0f ? w * tw : w;
                             }
                             const float us = ok ? u : 0.0f;
                             const int i0 = (int)us;
