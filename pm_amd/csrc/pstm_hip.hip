@@ -819,7 +819,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    // the wave index is wave-uniform; say so, so that everything derived from
+    // it (pair indices, the pair data read from LDS, their branches) is scalar
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tw = wave % kTW;
     const int pg = wave / kTW;
     const int win = 64 * kTW * MW;
