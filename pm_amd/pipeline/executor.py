@@ -250,13 +250,17 @@ class TileExecutor:
         return sums
 
     def finalize(self, out_dir, headers=None, **kw):
-        """Write the stacked image, fold volume and bin headers
+        """Write the stacked image (on the uniform axis, also with time
+        windows), fold volume and bin headers
         (pm_amd.pipeline.output.finalize_stack); computes the header sums when
         none are given.  Returns the WriteStats."""
         from pm_amd.pipeline.output import finalize_stack
 
         headers = headers if headers is not None else self.header_sums()
         image, fold = self.assemble()
+        # with time windows: the window samples back on the uniform axis, as
+        # the reference resamples each time-variant tile (executor.py:1720-1726)
+        image = self.to_uniform(image)
         return finalize_stack(out_dir, image, fold, self.output_grid(), headers, **kw)
 
     def gathers(self, bins) -> "GatherSet":
@@ -286,13 +290,19 @@ class TileExecutor:
         window samples resampled as the reference does after a time-variant
         tile (pstm/pipeline/executor.py:1720-1726; np.interp onto
         k * dt, k < nt), or ``assemble()``'s image without time windows."""
+        img, _ = self.assemble()
+        return self.to_uniform(img)
+
+    def to_uniform(self, image):
+        """An (nx, ny, n) image of this executor's compute axis on the job's
+        uniform axis (unchanged without time windows)."""
         from pm_amd.time_variant import sample_times, uniform_times
 
-        img, _ = self.assemble()
         if not self.time_windows:
-            return img
+            return image
         t = np.asarray(self.job.t_axis_ms, dtype=np.float64)
-        return self.kernel.resample_uniform(img, sample_times(self.time_windows), uniform_times(t[1] - t[0], len(t)))
+        return self.kernel.resample_uniform(image.contiguous(), sample_times(self.time_windows),
+                                            uniform_times(t[1] - t[0], len(t)))
 
     def host_traces(self, rows: np.ndarray | None = None) -> dict:
         """Host copies (in the executor's row order) for the CPU oracle."""
@@ -371,6 +381,6 @@ class GatherSet:
 
         def volumes(b):
             img, fold = self.assemble(b)
-            return img, fold, self.headers[b]
+            return self.ex.to_uniform(img), fold, self.headers[b]
 
         return finalize_gathers(out_dir, self.ex.kernel, self.bins, volumes, self.ex.output_grid(), **kw)
