@@ -36,15 +36,36 @@
 
 #include "../../include/pstm_hip.h"
 
+// Shared by the library's translation units (hidden, not exported): the
+// last-error slot, the attribute-only and launch-profile switches the gather
+// launches read, and the gather launchers of each kernel group.  The groups
+// (straight ray with / without anti-aliasing, curved ray, VTI, learned VTI,
+// interpolators) are compiled from this file in translation units of their
+// own (gather_part1..6.hip, PSTM_GATHER_PART = 1..6) so that the kernel
+// variants build in parallel; without PSTM_GATHER_PART this file is the rest
+// of the library.
+void pstm_set_last_error(const std::string& msg);
+bool pstm_gather_attr_only();
+bool pstm_gather_prof_on();
+void pstm_gather_prof_push(hipEvent_t ev0, hipEvent_t ev1);
+int pstm_gather_part1(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
+int pstm_gather_part2(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
+int pstm_gather_part3(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
+int pstm_gather_part4(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
+int pstm_gather_part5(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
+int pstm_gather_part6(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
+
 namespace {
 
 // ---------------------------------------------------------------------------
 // error handling
 // ---------------------------------------------------------------------------
+#ifndef PSTM_GATHER_PART
 thread_local std::string g_last_error;
+#endif
 
 int fail(int code, const std::string& msg) {
-    g_last_error = msg;
+    pstm_set_last_error(msg);
     return code;
 }
 
@@ -173,6 +194,7 @@ __device__ __forceinline__ float taper_weight(float x) {
 // preparation kernels (f64)
 // ---------------------------------------------------------------------------
 
+#ifndef PSTM_GATHER_PART
 // absolute f64 survey coordinates -> tile-relative f32 offsets
 __global__ void k_prep_geometry(PstmMigrateArgs a, float4* geo) {
     const int jj = blockIdx.x * blockDim.x + threadIdx.x;
@@ -239,6 +261,7 @@ __global__ void k_prep_eta(PstmMigrateArgs a, int nev, float2* ek) {
     const double eta = a.eta_mode == PSTM_ETA_CONST ? a.eta_constant : a.eta[e];
     ek[e] = make_float2((float)(2.0 * eta), (float)(1.0 + 2.0 * eta));
 }
+#endif  // PSTM_GATHER_PART
 
 // f64 distance rounded operation by operation, as the oracle's unfused
 // arithmetic: the __d*_rn intrinsics alone do not keep the compiler from
@@ -248,6 +271,7 @@ __device__ __forceinline__ double dist64(double dx, double dy) {
     return __dsqrt_rn(dx * dx + dy * dy);
 }
 
+#ifndef PSTM_GATHER_PART
 // Fold: count of traces with f64 midpoint distance <= max aperture, one
 // workgroup per pillar (numba_cpu_optimized.py:167-175,231).  The products and
 // sums are kept unfused so the test is the oracle's, bit for bit.
@@ -331,6 +355,8 @@ __global__ __launch_bounds__(256) void k_integrate(PstmMigrateArgs a, int rowlen
     for (int n = ns + 1 + (int)threadIdx.x; n + lo < rowlen; n += blockDim.x)
         out[n + lo] = pend + (double)(n - ns) * send;
 }
+
+#endif  // PSTM_GATHER_PART
 
 // ---------------------------------------------------------------------------
 // exact decisions
@@ -445,6 +471,7 @@ __host__ __device__ inline unsigned short bf16_bits(float f) {
     return (unsigned short)(u >> 16);
 }
 
+#ifndef PSTM_GATHER_PART
 __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restrict__ A, float* __restrict__ B) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < kANum * 4) {
@@ -473,6 +500,7 @@ __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restri
         B[f] = o < mlp_out(l) ? P[mlp_w_off(l) + mlp_in(l) * mlp_out(l) + o] : 0.0f;
     }
 }
+#endif  // PSTM_GATHER_PART
 
 __device__ __forceinline__ void split_bf16(float a0, float a1, float a2, float a3, bf16x4_t& hi, bf16x4_t& lo) {
     bf16v4_t h;
@@ -536,6 +564,7 @@ __device__ __forceinline__ void mlp_eval32(const bf16x4_t* __restrict__ WA, cons
     y1 = o[1];
 }
 
+#ifndef PSTM_GATHER_PART
 // Batched inference through the fused path's device code: one wavefront per
 // 32 rows of x (n, 5), outputs y (n, 2).
 __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x4_t* __restrict__ WA,
@@ -556,6 +585,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x4_t* __rest
         y[2 * r + 1] = y1;
     }
 }
+#endif  // PSTM_GATHER_PART
 
 // Traveltime of one contribution from the pair terms e/f (LDS) and the
 // (pillar, sample) terms.  y1/y2 return 1/t_src, 1/t_rec for anti-aliasing.
@@ -1312,6 +1342,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
 }
 
 
+#ifndef PSTM_GATHER_PART
 __global__ void k_sum_counts(const unsigned long long* part, int n, unsigned long long* out) {
     __shared__ unsigned long long red[256];
     unsigned long long s = 0;
@@ -1439,6 +1470,7 @@ int validate(const PstmMigrateArgs* a) {
 // set by plan creation: configure the kernel's dynamic LDS limit only, so that
 // nothing but stream work happens while a plan's launch sequence is captured
 bool g_attr_only = false;
+#endif  // PSTM_GATHER_PART
 
 template <int MW, int TT, bool AA, bool COUNT, bool LOB, bool W2, int INTERP = PSTM_INTERP_LINEAR>
 int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
@@ -1449,18 +1481,19 @@ int launch_migrate(const KParams& kp, int grid, size_t lds, hipStream_t st) {
                                     (int)lds));
         lds_set = lds;
     }
-    if (g_attr_only) return 0;
+    if (pstm_gather_attr_only()) return 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (g_prof.on) {
+    const bool prof = pstm_gather_prof_on();
+    if (prof) {
         HIP_TRY(hipEventCreate(&ev0));
         HIP_TRY(hipEventCreate(&ev1));
         HIP_TRY(hipEventRecord(ev0, st));
     }
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), lds, st, kp);
     HIP_TRY(hipGetLastError());
-    if (g_prof.on) {
+    if (prof) {
         HIP_TRY(hipEventRecord(ev1, st));
-        g_prof.events.emplace_back(ev0, ev1);
+        pstm_gather_prof_push(ev0, ev1);
     }
     return 0;
 }
@@ -1491,6 +1524,7 @@ int dispatch_mw(int mw, bool count, bool lob, const KParams& kp, int grid, size_
                : dispatch_mw3<TT, AA, false, false>(mw, kp, grid, lds, st);
 }
 
+#ifndef PSTM_GATHER_PART
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 __global__ void k_reset_queue(unsigned* q) {
@@ -1732,6 +1766,8 @@ long tile_items(const PstmMigrateArgs& a, const Shape& sh) {
     return (long)((a.nx * a.ny + kNPil - 1) / kNPil) * sh.nwin;
 }
 
+#endif  // PSTM_GATHER_PART
+
 // straight ray, no anti-aliasing, no counter, MW = kInterpMW (launch_shape),
 // every lower-bound check: the one instantiation per interpolator
 template <int INTERP>
@@ -1740,22 +1776,16 @@ int launch_interp(const KParams& kp, int grid, size_t lds, hipStream_t st) {
                                                                                                   st);
 }
 
+#ifndef PSTM_GATHER_PART
+// the gather launch of a call or plan, through the launcher of its kernel group
 int dispatch(const Shape& sh, bool count, bool lob, const KParams& kp, int grid, hipStream_t st) {
-    switch (sh.interp) {
-        case PSTM_INTERP_NEAREST: return launch_interp<PSTM_INTERP_NEAREST>(kp, grid, sh.lds, st);
-        case PSTM_INTERP_CUBIC: return launch_interp<PSTM_INTERP_CUBIC>(kp, grid, sh.lds, st);
-        case PSTM_INTERP_SINC4: return launch_interp<PSTM_INTERP_SINC4>(kp, grid, sh.lds, st);
-        case PSTM_INTERP_SINC8: return launch_interp<PSTM_INTERP_SINC8>(kp, grid, sh.lds, st);
-        case PSTM_INTERP_SINC16: return launch_interp<PSTM_INTERP_SINC16>(kp, grid, sh.lds, st);
-        case PSTM_INTERP_LANCZOS3: return launch_interp<PSTM_INTERP_LANCZOS3>(kp, grid, sh.lds, st);
-        case PSTM_INTERP_LANCZOS5: return launch_interp<PSTM_INTERP_LANCZOS5>(kp, grid, sh.lds, st);
-        default: break;
-    }
-    if (sh.tt == PSTM_TT_STRAIGHT && sh.aa) return dispatch_mw<PSTM_TT_STRAIGHT, true>(sh.mw, count, lob, kp, grid, sh.lds, st);
-    if (sh.tt == PSTM_TT_STRAIGHT) return dispatch_mw<PSTM_TT_STRAIGHT, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
-    if (sh.tt == PSTM_TT_CURVED) return dispatch_mw<PSTM_TT_CURVED, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
-    if (sh.tt == PSTM_TT_VTI) return dispatch_mw<PSTM_TT_VTI, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
-    return dispatch_mw<PSTM_TT_VTI_LEARNED, false>(sh.mw, count, lob, kp, grid, sh.lds, st);
+    int (*part)(int, bool, bool, int, const void*, int, size_t, hipStream_t);
+    if (sh.interp != PSTM_INTERP_LINEAR) part = pstm_gather_part6;
+    else if (sh.tt == PSTM_TT_STRAIGHT) part = sh.aa ? pstm_gather_part1 : pstm_gather_part2;
+    else if (sh.tt == PSTM_TT_CURVED) part = pstm_gather_part3;
+    else if (sh.tt == PSTM_TT_VTI) part = pstm_gather_part4;
+    else part = pstm_gather_part5;
+    return part(sh.mw, count, lob, sh.interp, &kp, grid, sh.lds, st);
 }
 
 // shared by migrate_tile and migrate_bins: `ntr` rows starting at row
@@ -1862,6 +1892,11 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
 // Failure message setter for the library's other translation units (the
 // output kernels), so that pstm_hip_last_error() reports their failures too.
 __attribute__((visibility("hidden"))) void pstm_set_last_error(const std::string& msg) { g_last_error = msg; }
+__attribute__((visibility("hidden"))) bool pstm_gather_attr_only() { return g_attr_only; }
+__attribute__((visibility("hidden"))) bool pstm_gather_prof_on() { return g_prof.on; }
+__attribute__((visibility("hidden"))) void pstm_gather_prof_push(hipEvent_t ev0, hipEvent_t ev1) {
+    g_prof.events.emplace_back(ev0, ev1);
+}
 
 // A device-resident work queue over many output tiles (common-offset bins
 // included: a tile's trace list may hold rows of every bin): per-tile
@@ -2283,3 +2318,47 @@ int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_star
 }
 
 }  // extern "C"
+
+#else  // PSTM_GATHER_PART: one kernel group
+
+}  // namespace
+
+#define PSTM_GATHER_CAT2(a, b) a##b
+#define PSTM_GATHER_CAT(a, b) PSTM_GATHER_CAT2(a, b)
+
+__attribute__((visibility("hidden"))) int PSTM_GATHER_CAT(pstm_gather_part, PSTM_GATHER_PART)(
+    int mw, bool count, bool lob, int interp, const void* kpv, int grid, size_t lds, hipStream_t st) {
+    const KParams& kp = *static_cast<const KParams*>(kpv);
+#if PSTM_GATHER_PART == 1
+    (void)interp;
+    return dispatch_mw<PSTM_TT_STRAIGHT, true>(mw, count, lob, kp, grid, lds, st);
+#elif PSTM_GATHER_PART == 2
+    (void)interp;
+    return dispatch_mw<PSTM_TT_STRAIGHT, false>(mw, count, lob, kp, grid, lds, st);
+#elif PSTM_GATHER_PART == 3
+    (void)interp;
+    return dispatch_mw<PSTM_TT_CURVED, false>(mw, count, lob, kp, grid, lds, st);
+#elif PSTM_GATHER_PART == 4
+    (void)interp;
+    return dispatch_mw<PSTM_TT_VTI, false>(mw, count, lob, kp, grid, lds, st);
+#elif PSTM_GATHER_PART == 5
+    (void)interp;
+    return dispatch_mw<PSTM_TT_VTI_LEARNED, false>(mw, count, lob, kp, grid, lds, st);
+#else
+    (void)mw;
+    (void)count;
+    (void)lob;
+    switch (interp) {
+        case PSTM_INTERP_NEAREST: return launch_interp<PSTM_INTERP_NEAREST>(kp, grid, lds, st);
+        case PSTM_INTERP_CUBIC: return launch_interp<PSTM_INTERP_CUBIC>(kp, grid, lds, st);
+        case PSTM_INTERP_SINC4: return launch_interp<PSTM_INTERP_SINC4>(kp, grid, lds, st);
+        case PSTM_INTERP_SINC8: return launch_interp<PSTM_INTERP_SINC8>(kp, grid, lds, st);
+        case PSTM_INTERP_SINC16: return launch_interp<PSTM_INTERP_SINC16>(kp, grid, lds, st);
+        case PSTM_INTERP_LANCZOS3: return launch_interp<PSTM_INTERP_LANCZOS3>(kp, grid, lds, st);
+        case PSTM_INTERP_LANCZOS5: return launch_interp<PSTM_INTERP_LANCZOS5>(kp, grid, lds, st);
+        default: return fail(1, "no gather kernel for this interpolation");
+    }
+#endif
+}
+
+#endif  // PSTM_GATHER_PART
