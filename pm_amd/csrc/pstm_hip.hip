@@ -784,44 +784,95 @@ __device__ __forceinline__ float interp_at(const float* row, int i0, float f0, i
     }
 }
 
+// some lane of the wave holds p: the ballot of the predicate itself, so the
+// mask stays in scalar registers (__any takes an int, which costs a select and
+// a compare per call)
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+__device__ __forceinline__ unsigned long long wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // Trace amplitude at sample position i0 + f0: linear interpolation
 // (interpolation.py:68-92) or another interpolator (interp_at), or, with
 // anti-aliasing, the triangle filter of half width L (from the operator time
 // dip) on the f64 double running sum: a blend of the integer half widths
 // floor(L) and floor(L)+1, each a second difference interpolated between i0
 // and i0+1.
+//
+// The amplitude is split into a fetch (every LDS read it needs) and a finish
+// (the arithmetic on what was read), so that the gather can issue the reads
+// first and work out the weight while they are in flight.
+struct LinTaps {
+    float s0, s1;
+};
+struct AaTaps {
+    double am1, am0, am1p, c0, c1, b0, b1, b2;  // running sum at i0 - lf - 1 .. i0 + lf + 2
+    float2 iw;                                  // {1 / lf^2, 1 / (lf + 1)^2}
+    float lam;                                  // fraction of L between lf and lf + 1
+};
+
+__device__ __forceinline__ LinTaps lin_fetch(const float* row, int i0) {
+    const float* pc = row + i0;
+    return LinTaps{pc[0], pc[1]};
+}
+
+__device__ __forceinline__ float lin_finish(const LinTaps& s, float f0) { return fmaf(f0, s.s1 - s.s0, s.s0); }
+
+// half width L = operator time dip x bin size in samples (offsets in f carry
+// the bin size / dt), clamped to [1, lmax]; the taps of floor(L) and
+// floor(L) + 1 around i0 and i0 + 1, addressed from one centre pointer
+__device__ __forceinline__ AaTaps aa_fetch(const KParams& P, const double* row, int i0, const float4& f, float cs,
+                                           float y1, float y2, const float2* invsq) {
+    const float gx = fmaf(f.x, y1, f.z * y2);
+    const float gy = fmaf(f.y, y1, f.w * y2);
+    const float L = __builtin_amdgcn_fmed3f(cs * fmaxf(fabsf(gx), fabsf(gy)), 1.0f, P.lmax);
+    const int lf = (int)L;
+    AaTaps a;
+    a.lam = __builtin_amdgcn_fractf(L);
+    // pc points one element before the centre, so that every tap is at a
+    // non-negative offset of pc, pc - lf or pc + lf (LDS instruction offsets
+    // are unsigned); the address is hidden from the optimiser, which would
+    // otherwise expand pc - lf and pc + lf from i0 and lf separately (six
+    // integer operations instead of four)
+    using LdsDouble = __attribute__((address_space(3))) const double;
+    LdsDouble* pc = (LdsDouble*)(row + P.lo - 1) + i0;
+    asm("" : "+v"(pc));
+    LdsDouble* pa = pc - lf;
+    LdsDouble* pb = pc + lf;
+    a.am1 = pa[0];
+    a.am0 = pa[1];
+    a.am1p = pa[2];
+    a.c0 = pc[1];
+    a.c1 = pc[2];
+    a.b0 = pb[1];
+    a.b1 = pb[2];
+    a.b2 = pb[3];
+    a.iw = invsq[lf];
+    return a;
+}
+
+// triangle filters of half widths lf and lf + 1 at i0 and i0 + 1, blended by
+// lam across L and by f0 across the sample
+__device__ __forceinline__ float aa_finish(const AaTaps& a, float f0) {
+    // second differences P[n + L] - 2 P[n] + P[n - L] (one add, one fma in f64)
+    const float dl0 = (float)fma(-2.0, a.c0, a.b0 + a.am0);   // T_lf[i0] * lf^2
+    const float dh0 = (float)fma(-2.0, a.c0, a.b1 + a.am1);   // T_lf+1[i0] * (lf+1)^2
+    const float dl1 = (float)fma(-2.0, a.c1, a.b1 + a.am1p);  // T_lf[i0+1] * lf^2
+    const float dh1 = (float)fma(-2.0, a.c1, a.b2 + a.am0);   // T_lf+1[i0+1] * (lf+1)^2
+    const float wl = fmaf(-a.lam, a.iw.x, a.iw.x);
+    const float wh = a.lam * a.iw.y;
+    const float tr0 = fmaf(dl0, wl, dh0 * wh);
+    const float tr1 = fmaf(dl1, wl, dh1 * wh);
+    return fmaf(f0, tr1 - tr0, tr0);
+}
+
 template <bool AA, int INTERP, typename Row>
 __device__ __forceinline__ float contrib_amp(const KParams& P, const Row* row, int i0, float f0, const float4& f,
                                              float cs, float y1, float y2, const float2* invsq) {
-    if (!AA && INTERP != PSTM_INTERP_LINEAR) {
-        return interp_at<INTERP>(reinterpret_cast<const float*>(row), i0, f0, P.ns);
-    } else if (!AA) {
-        const float s0 = row[i0];
-        const float s1 = row[i0 + 1];
-        return fmaf(f0, s1 - s0, s0);
+    if constexpr (AA) {
+        return aa_finish(aa_fetch(P, row, i0, f, cs, y1, y2, invsq), f0);
+    } else if constexpr (INTERP != PSTM_INTERP_LINEAR) {
+        return interp_at<INTERP>(row, i0, f0, P.ns);
     } else {
-        // half width L = operator time dip x bin size in samples (offsets in f
-        // carry the bin size / dt), clamped to [1, lmax]
-        const float gx = fmaf(f.x, y1, f.z * y2);
-        const float gy = fmaf(f.y, y1, f.w * y2);
-        const float L = __builtin_amdgcn_fmed3f(cs * fmaxf(fabsf(gx), fabsf(gy)), 1.0f, P.lmax);
-        const int lf = (int)L;
-        const float lam = __builtin_amdgcn_fractf(L);
-        const Row* r0 = row + P.lo;
-        const double am1 = r0[i0 - lf - 1], am0 = r0[i0 - lf], am1p = r0[i0 - lf + 1];
-        const double c0 = r0[i0], c1 = r0[i0 + 1];
-        const double b0 = r0[i0 + lf], b1 = r0[i0 + lf + 1], b2 = r0[i0 + lf + 2];
-        // second differences P[n + L] - 2 P[n] + P[n - L] (one add, one fma in f64)
-        const float dl0 = (float)fma(-2.0, c0, b0 + am0);   // T_lf[i0] * lf^2
-        const float dh0 = (float)fma(-2.0, c0, b1 + am1);   // T_lf+1[i0] * (lf+1)^2
-        const float dl1 = (float)fma(-2.0, c1, b1 + am1p);  // T_lf[i0+1] * lf^2
-        const float dh1 = (float)fma(-2.0, c1, b2 + am0);   // T_lf+1[i0+1] * (lf+1)^2
-        const float2 iw = invsq[lf];                        // {1 / lf^2, 1 / (lf + 1)^2}
-        const float wl = (1.0f - lam) * iw.x;
-        const float wh = lam * iw.y;
-        const float tr0 = fmaf(dl0, wl, dh0 * wh);
-        const float tr1 = fmaf(dl1, wl, dh1 * wh);
-        return fmaf(f0, tr1 - tr0, tr0);
+        return lin_finish(lin_fetch(row, i0), f0);
     }
 }
 
@@ -1176,10 +1227,14 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 for (int jj = 0; jj < nb; ++jj) {
                     const Row* row = rows + (size_t)jj * P.rowlen;
                     bool any_near = false;  // some slot within eps of a threshold (rare)
+                    // both pairs' records read at once: one LDS round trip per trace
+                    float4 ep[kPW];
+    #pragma unroll
+                    for (int p = 0; p < kPW; ++p) ep[p] = pairs[((pg * kPW + p) * P.batch + jj) * kPairVec];
     #pragma unroll
                     for (int p = 0; p < kPW; ++p) {
                         const int pi = (pg * kPW + p) * P.batch + jj;
-                        const float4 e = pairs[pi * kPairVec];
+                        const float4 e = ep[p];
                         if (e.z == INFINITY) continue;  // pair outside max aperture (wave-uniform)
                         float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
                         if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
@@ -1196,7 +1251,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     #pragma unroll
                         for (int m = 0; m < MW; ++m) {
                             xt[m] = fmaf(e.z, ci[p][m], P.taper_off);
-                            blk[m] = __any(xt[m] <= 1.0f + P.eps_x);
+                            blk[m] = wave_any(xt[m] <= 1.0f + P.eps_x);
                             any_in |= blk[m];
                         }
                         if (!any_in) continue;  // time-dependent aperture excludes every slot
@@ -1234,17 +1289,27 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             any_near |= near;
                             bool ok = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u < P.umax : below) && !near;
                             if (LOB) ok = ok && u >= 0.0f;
+                            const float us = ok ? u : 0.0f;
+                            const int i0 = (int)us;
+                            const float f0 = __builtin_amdgcn_fractf(us);
+                            // the row reads are issued before the weight is worked out
+                            AaTaps ta;
+                            LinTaps tl;
+                            if constexpr (AA) ta = aa_fetch(P, row, i0, f, cs[p][m], y1, y2, s_invsq);
+                            else if constexpr (INTERP == PSTM_INTERP_LINEAR) tl = lin_fetch(row, i0);
                             float w = ok ? e.w : 0.0f;
-                            if (__any(ok && xt[m] > 0.0f)) {  // taper zone (wave-uniform test)
+                            // taper zone 0 < x_t <= 1 on some lane (wave-uniform test: the
+                            // ballots of two plain compares, ANDed as masks, stay scalar)
+                            if ((wave_mask(xt[m] > 0.0f) & wave_mask(xt[m] <= 1.0f)) != 0) {
                                 // a select, not a branch: the polynomial runs for the
                                 // whole wave either way, without exec-mask juggling
                                 const float tw = taper_weight(__builtin_amdgcn_fmed3f(xt[m], 0.0f, 1.0f));
                                 w = xt[m] > 0.0f ? w * tw : w;
                             }
-                            const float us = ok ? u : 0.0f;
-                            const int i0 = (int)us;
-                            const float f0 = __builtin_amdgcn_fractf(us);
-                            const float amp = contrib_amp<AA, INTERP>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq);
+                            float amp;
+                            if constexpr (AA) amp = aa_finish(ta, f0);
+                            else if constexpr (INTERP == PSTM_INTERP_LINEAR) amp = lin_finish(tl, f0);
+                            else amp = interp_at<INTERP>(row, i0, f0, P.ns);
                             if (W2 || P.wpow != 0) {
                                 const float r = frcp(t);
                                 float wf = cw[p][m] * ((W2 || P.wpow == 2) ? r * r : r);
