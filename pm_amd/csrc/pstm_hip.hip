@@ -784,19 +784,18 @@ __device__ __forceinline__ float interp_at(const float* row, int i0, float f0, i
     }
 }
 
-// some lane of the wave holds p: the ballot of the predicate itself, so the
-// mask stays in scalar registers (__any takes an int, which costs a select and
-// a compare per call)
-__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
-__device__ __forceinline__ unsigned long long wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
-
 // Trace amplitude at sample position i0 + f0: linear interpolation
 // (interpolation.py:68-92) or another interpolator (interp_at), or, with
 // anti-aliasing, the triangle filter of half width L (from the operator time
 // dip) on the f64 double running sum: a blend of the integer half widths
 // floor(L) and floor(L)+1, each a second difference interpolated between i0
 // and i0+1.
-//
+// some lane of the wave holds p: the ballot of the predicate itself, so the
+// mask stays in scalar registers (__any takes an int, which costs a select and
+// a compare per call)
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+__device__ __forceinline__ unsigned long long wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // The amplitude is split into a fetch (every LDS read it needs) and a finish
 // (the arithmetic on what was read), so that the gather can issue the reads
 // first and work out the weight while they are in flight.
@@ -1224,6 +1223,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             // (wave-uniform, rare: one copy of the loop without that test)
             auto trace_loop = [&](auto early_c) {
                 constexpr bool EARLY = decltype(early_c)::value;
+                // the sample offset held in a vector register: the sample position
+                // fmaf(t, idt, uoff) then needs no copy of a scalar operand per slot
+                float uoff_v = P.uoff;
+                asm("" : "+v"(uoff_v));
                 for (int jj = 0; jj < nb; ++jj) {
                     const Row* row = rows + (size_t)jj * P.rowlen;
                     bool any_near = false;  // some slot within eps of a threshold (rare)
@@ -1272,7 +1275,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             float y1, y2;
                             const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m],
                                                                  y1, y2);
-                            const float u = fmaf(t, P.idt, P.uoff);
+                            const float u = fmaf(t, P.idt, uoff_v);
                             // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
                             const bool in_ap = xt[m] <= ((COUNT && TT != PSTM_TT_VTI_LEARNED) ? 1.0f - P.eps_x : 1.0f);
                             const bool below = u < P.umax_lo;
