@@ -784,18 +784,19 @@ __device__ __forceinline__ float interp_at(const float* row, int i0, float f0, i
     }
 }
 
-// Trace amplitude at sample position i0 + f0: linear interpolation
-// (interpolation.py:68-92) or another interpolator (interp_at), or, with
-// anti-aliasing, the triangle filter of half width L (from the operator time
-// dip) on the f64 double running sum: a blend of the integer half widths
-// floor(L) and floor(L)+1, each a second difference interpolated between i0
-// and i0+1.
 // some lane of the wave holds p: the ballot of the predicate itself, so the
 // mask stays in scalar registers (__any takes an int, which costs a select and
 // a compare per call)
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 __device__ __forceinline__ unsigned long long wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// Trace amplitude at sample position i0 + f0: linear interpolation
+// (interpolation.py:68-92) or another interpolator (interp_at), or, with
+// anti-aliasing, the triangle filter of half width L (from the operator time
+// dip) on the f64 double running sum: a blend of the integer half widths
+// floor(L) and floor(L)+1, each a second difference interpolated between i0
+// and i0+1.
+//
 // The amplitude is split into a fetch (every LDS read it needs) and a finish
 // (the arithmetic on what was read), so that the gather can issue the reads
 // first and work out the weight while they are in flight.
