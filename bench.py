@@ -110,8 +110,9 @@ def cpu_baseline(ex, job, budget_s: float) -> dict:
 
 # HBM bytes per gather launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per
 # dispatch scaled by true bytes / counter of the calibration kernel), per workload
-# (profiles/k_migrate_reflectors_aa.md: FETCH x 2.000 + WRITE x 1.000, one plan launch)
-TRAFFIC: dict = {"reflectors": (6_328_551_838 * 2.000 + 266_505 * 1.000) * 1024}
+# (the committed rocprofv3 summary of the reflector gather: FETCH x 2.000 + WRITE
+# x 1.000, one plan launch)
+TRAFFIC: dict = {"reflectors": (6_202_182_409 * 2.000 + 260_362 * 1.000) * 1024}
 
 
 def learned_vs_exact(kernel, args) -> dict:
