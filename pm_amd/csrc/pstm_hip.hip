@@ -1231,6 +1231,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 for (int jj = 0; jj < nb; ++jj) {
                     const Row* row = rows + (size_t)jj * P.rowlen;
                     bool any_near = false;  // some slot within eps of a threshold (rare)
+                    // interpolators that jump at fraction 0.5 have a near lane in
+                    // most traces: the slots (bit p * MW + m) holding one, so
+                    // that only those are recomputed
+                    unsigned near_slots = 0;
                     // both pairs' records read at once: one LDS round trip per trace
                     float4 ep[kPW];
     #pragma unroll
@@ -1291,6 +1295,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             // the learned time has no f64 counterpart to decide against
                             if (TT == PSTM_TT_VTI_LEARNED) near = false;
                             any_near |= near;
+                            if constexpr (kHalfJump<INTERP>) near_slots |= wave_any(near) ? 1u << (p * MW + m) : 0u;
                             bool ok = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u < P.umax : below) && !near;
                             if (LOB) ok = ok && u >= 0.0f;
                             const float us = ok ? u : 0.0f;
@@ -1337,6 +1342,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             if (kPairVec == 2) f = pairs[pi * kPairVec + 1];
     #pragma unroll
                             for (int m = 0; m < MW; ++m) {
+                                if (kHalfJump<INTERP> && !(near_slots & (1u << (p * MW + m)))) continue;
                                 const float xt = fmaf(e.z, ci[p][m], P.taper_off);
                                 float y1, y2;
                                 const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m],
