@@ -909,7 +909,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     unsigned long long cnt = 0;
     __shared__ int s_item;
     __shared__ int s_row[kMaxBatch];  // block rows of the current batch
-    __shared__ int s_rownext[kMaxBatch];  // and of the next one (pipelined staging)
+    // and of the next one (pipelined staging), in two copies by batch parity: a
+    // skipped batch has one barrier only, after which the waves read the copy
+    // of the batch while a wave that runs ahead writes the other one for the
+    // batch after; the copy is written again only after the next barrier
+    __shared__ int s_rownext[2][kMaxBatch];
     __shared__ float2 s_invsq[kMaxHalfWidth + 2];  // AA: {1 / l^2, 1 / (l + 1)^2}
     if (AA) {
         for (int i = tid; i < kMaxHalfWidth + 2; i += kThreads)
@@ -1002,13 +1006,23 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // are written to LDS after the batch's last barrier.
         uint32_t pf[kPF];
         const int rl_dw = P.rowlen * (int)(sizeof(Row) / 4);
-        auto row_dw = [&](int jrow, bool from_lds, int r) -> const uint32_t* {
+        // row jrow of the tile's list (r-th of its batch); its block row from
+        // (wave-uniform) rsrc: kRowsGlobal the tile's list in global memory,
+        // kRowsThis s_row, kRowsNext + parity s_rownext.  The LDS arrays are
+        // indexed by name so their reads stay LDS reads: through a plain
+        // pointer they would be flat loads, whose wait also waits for the row
+        // loads in flight
+        constexpr int kRowsGlobal = 0, kRowsThis = 1, kRowsNext = 2;
+        auto row_dw = [&](int jrow, int rsrc, int r) -> const uint32_t* {
             if (AA && !P.prow_global) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)jrow * P.rowlen);
-            const int rw = from_lds ? s_rownext[r] : (T.tidx ? T.tidx[jrow] : jrow);
+            int rw;
+            if (rsrc == kRowsThis) rw = s_row[r];
+            else if (rsrc >= kRowsNext) rw = s_rownext[rsrc - kRowsNext][r];
+            else rw = T.tidx ? T.tidx[jrow] : jrow;
             if constexpr (AA) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)rw * P.rowlen);
             else return reinterpret_cast<const uint32_t*>(P.amp + (size_t)rw * P.ns);
         };
-        auto prefetch = [&](int jb, bool from_lds) {
+        auto prefetch = [&](int jb, int rsrc) {
             const int nbb = min(P.batch, T.ntr - jb);
             if (nbb <= 0) return;
             const int n = nbb * rl_dw;
@@ -1017,7 +1031,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 c -= rl_dw;
                 ++r;
             }
-            const uint32_t* src = row_dw(jb + min(r, nbb - 1), from_lds, min(r, nbb - 1));
+            const uint32_t* src = row_dw(jb + min(r, nbb - 1), rsrc, min(r, nbb - 1));
 #pragma unroll
             for (int i = 0; i < kPF; ++i) {
                 if (tid + i * kThreads < n) pf[i] = src[c];
@@ -1028,13 +1042,14 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     ++r;
                     moved = true;
                 }
-                if (moved && r < nbb) src = row_dw(jb + r, from_lds, r);
+                if (moved && r < nbb) src = row_dw(jb + r, rsrc, r);
             }
         };
-        // AA rows: double-buffered direct global -> LDS loads, 64 dwords per
-        // wave instruction (wave-uniform LDS destination, per-lane source)
+        // AA rows: direct global -> LDS loads, 64 dwords per wave instruction
+        // (wave-uniform LDS destination, per-lane source), into the second of
+        // two row buffers when pipelined, else into the one buffer
         int cur = 0;
-        auto dma = [&](int jb, int buf, bool from_lds) {
+        auto dma = [&](int jb, int buf, int rsrc) {
             if constexpr (AA) {
                 const int nbb = min(P.batch, T.ntr - jb);
                 if (nbb <= 0) return;
@@ -1046,7 +1061,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     col -= rl_dw;
                     ++r;
                 }
-                const uint32_t* src = row_dw(jb + min(r, nbb - 1), from_lds, min(r, nbb - 1));
+                const uint32_t* src = row_dw(jb + min(r, nbb - 1), rsrc, min(r, nbb - 1));
                 for (int c = wave; c * 64 < n; c += kWaves) {
                     if (e < n) __builtin_amdgcn_global_load_lds(src + col, dst0 + c * 64, 4, 0, 0);
                     e += kWaves * 64;
@@ -1057,16 +1072,16 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         ++r;
                         moved = true;
                     }
-                    if (moved && r < nbb) src = row_dw(jb + r, from_lds, r);
+                    if (moved && r < nbb) src = row_dw(jb + r, rsrc, r);
                 }
             }
         };
         if (P.pipe) {
-            if (AA) dma(0, 0, false);
-            else prefetch(0, false);
+            if (AA) dma(0, 0, kRowsGlobal);
+            else prefetch(0, kRowsGlobal);
         }
 
-        for (int j0 = 0; j0 < T.ntr; j0 += P.batch) {
+        for (int j0 = 0, par = 0; j0 < T.ntr; j0 += P.batch, par ^= 1) {
             const int nb = min(P.batch, T.ntr - j0);
             // (pillar, trace) pair table; the max-aperture test is the
             // oracle's f64 one (numba_cpu_optimized.py:167-173)
@@ -1074,7 +1089,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
             if (P.pipe && tid >= 64 && tid < 64 + P.batch) {
                 const int jn = j0 + P.batch + (tid - 64);
-                s_rownext[tid - 64] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
+                s_rownext[par][tid - 64] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
             }
             if (pt_thread) {
                 const int pil = pil0 + pt_p;
@@ -1117,10 +1132,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             if (!__syncthreads_or(inap)) {
                 if (P.pipe) {  // this batch's rows are not needed
                     if (AA) {
-                        dma(j0 + P.batch, cur ^ 1, true);
+                        dma(j0 + P.batch, cur ^ 1, kRowsNext + par);
                         cur ^= 1;
                     } else {
-                        prefetch(j0 + P.batch, true);
+                        prefetch(j0 + P.batch, kRowsNext + par);
                     }
                 }
                 continue;
@@ -1179,11 +1194,13 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
 #pragma unroll
                 for (int i = 0; i < kPF; ++i)
                     if (tid + i * kThreads < n) dst[tid + i * kThreads] = pf[i];
-            } else if (AA && !P.prow_global) {
-                const double* src = P.prow + (size_t)j0 * P.rowlen;
-                double* dst = reinterpret_cast<double*>(rows);
-                const int n = nb * P.rowlen;
-                for (int e = tid; e < n; e += kThreads) dst[e] = src[e];
+            } else if (AA) {
+                // the batch's f64 rows by direct global -> LDS loads into the one
+                // row buffer: every load of a wave in flight at once and no
+                // registers held (a loop of load-then-store through registers
+                // waits out the memory latency once per element)
+                dma(j0, 0, kRowsThis);
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
             } else {
                 // rows picked through the trace list: one flat loop over the
                 // batch's nb x rowlen elements, every thread busy, each row
@@ -1215,8 +1232,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
             __syncthreads();
             if (P.pipe) {  // in flight during the batch
-                if (AA) dma(j0 + P.batch, cur ^ 1, true);
-                else prefetch(j0 + P.batch, true);
+                if (AA) dma(j0 + P.batch, cur ^ 1, kRowsNext + par);
+                else prefetch(j0 + P.batch, kRowsNext + par);
             }
 
             // the traces of the batch; EARLY: this wave holds t0 <= 2 ms samples,
