@@ -9,7 +9,7 @@ float32, chunked by rows, with sample_rate_ms / start_time_ms attributes
 holding the chunk's C-order bytes (full chunk shape, the edge chunk padded).
 This module reads and writes the uncompressed form (``compressor: null``).
 The reference's own files use Blosc/zstd compression, whose decoder is not
-part of this build: opening such an array raises NotImplementedError.
+part of this package: opening such an array raises NotImplementedError.
 """
 
 from __future__ import annotations

@@ -219,7 +219,7 @@ def load_library() -> ctypes.CDLL:
     if not path.exists():
         raise RuntimeError(
             f"HIP backend library not built: {path} is missing "
-            "(run __graft_entry__.build() or python -m pm_amd.build)")
+            "(build it with python -m pm_amd.build)")
     lib = declare(ctypes.CDLL(str(path)))
     check_layout(lib)
     _LIB = lib

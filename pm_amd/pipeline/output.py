@@ -21,7 +21,7 @@ Reference behaviour restated here:
     ``gathers_index.parquet`` — ``finalize_gathers``.
 
 Differences, on purpose: chunks are stored uncompressed (``compressor:
-null``; any Zarr v2 reader reads them, and this build has no Blosc codec);
+null``; any Zarr v2 reader reads them, and this package has no Blosc codec);
 the fold volumes are chunked like the images (the reference leaves their
 chunking to zarr); the image is accumulated in f32 where the reference's
 memmap is f64 that it casts to f32 on write.
