@@ -1008,17 +1008,13 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         const int rl_dw = P.rowlen * (int)(sizeof(Row) / 4);
         // row jrow of the tile's list (r-th of its batch); its block row from
         // (wave-uniform) rsrc: kRowsGlobal the tile's list in global memory,
-        // kRowsThis s_row, kRowsNext + parity s_rownext.  The LDS arrays are
-        // indexed by name so their reads stay LDS reads: through a plain
-        // pointer they would be flat loads, whose wait also waits for the row
-        // loads in flight
-        constexpr int kRowsGlobal = 0, kRowsThis = 1, kRowsNext = 2;
+        // kRowsNext + parity s_rownext.  The LDS array is indexed by name so
+        // its reads stay LDS reads: through a plain pointer they would be flat
+        // loads, whose wait also waits for the row loads in flight
+        constexpr int kRowsGlobal = 0, kRowsNext = 1;
         auto row_dw = [&](int jrow, int rsrc, int r) -> const uint32_t* {
             if (AA && !P.prow_global) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)jrow * P.rowlen);
-            int rw;
-            if (rsrc == kRowsThis) rw = s_row[r];
-            else if (rsrc >= kRowsNext) rw = s_rownext[rsrc - kRowsNext][r];
-            else rw = T.tidx ? T.tidx[jrow] : jrow;
+            const int rw = rsrc >= kRowsNext ? s_rownext[rsrc - kRowsNext][r] : (T.tidx ? T.tidx[jrow] : jrow);
             if constexpr (AA) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)rw * P.rowlen);
             else return reinterpret_cast<const uint32_t*>(P.amp + (size_t)rw * P.ns);
         };
@@ -1049,23 +1045,26 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // (wave-uniform LDS destination, per-lane source), into the second of
         // two row buffers when pipelined, else into the one buffer
         int cur = 0;
-        auto dma = [&](int jb, int buf, int rsrc) {
+        // waves w0.. issue the loads (chunk c of 64 dwords by wave w0 + c % (kWaves - w0),
+        // the same wave for a given LDS chunk at every call)
+        auto dma = [&](int jb, int buf, int rsrc, int w0) {
             if constexpr (AA) {
                 const int nbb = min(P.batch, T.ntr - jb);
-                if (nbb <= 0) return;
+                if (nbb <= 0 || wave < w0) return;
+                const int nw = kWaves - w0;
                 const int n = nbb * rl_dw;
                 uint32_t* dst0 = reinterpret_cast<uint32_t*>(smem + (size_t)buf * P.rows_bytes);
-                int e = wave * 64 + lane;
+                int e = (wave - w0) * 64 + lane;
                 int r = 0, col = e;
                 while (col >= rl_dw) {
                     col -= rl_dw;
                     ++r;
                 }
                 const uint32_t* src = row_dw(jb + min(r, nbb - 1), rsrc, min(r, nbb - 1));
-                for (int c = wave; c * 64 < n; c += kWaves) {
+                for (int c = wave - w0; c * 64 < n; c += nw) {
                     if (e < n) __builtin_amdgcn_global_load_lds(src + col, dst0 + c * 64, 4, 0, 0);
-                    e += kWaves * 64;
-                    col += kWaves * 64;
+                    e += nw * 64;
+                    col += nw * 64;
                     bool moved = false;
                     while (col >= rl_dw) {
                         col -= rl_dw;
@@ -1077,19 +1076,32 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
         };
         if (P.pipe) {
-            if (AA) dma(0, 0, kRowsGlobal);
+            if (AA) dma(0, 0, kRowsGlobal, 0);
             else prefetch(0, kRowsGlobal);
         }
 
+        // Unpipelined AA staging (early_dma): the batch's f64 rows go by direct
+        // global -> LDS loads into the one row buffer, issued at the batch start
+        // so that they are in flight while the pair table is built: every load
+        // of a wave in flight at once and no registers held (a loop of
+        // load-then-store through registers waits out the memory latency once
+        // per element).  The row indices come from s_rownext, written by the
+        // batch before.  The waves that build the pair table issue none: their
+        // own global reads would wait behind the row loads.
+        const bool early_dma = AA && !P.pipe;
+        const int w_pairs = (kNPil * P.batch + 63) / 64;  // waves holding pair-table threads
         for (int j0 = 0, par = 0; j0 < T.ntr; j0 += P.batch, par ^= 1) {
             const int nb = min(P.batch, T.ntr - j0);
+            if (early_dma) dma(j0, 0, j0 == 0 ? kRowsGlobal : kRowsNext + (par ^ 1), w_pairs);
             // (pillar, trace) pair table; the max-aperture test is the
             // oracle's f64 one (numba_cpu_optimized.py:167-173)
             int inap = 0;
-            if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
-            if (P.pipe && tid >= 64 && tid < 64 + P.batch) {
-                const int jn = j0 + P.batch + (tid - 64);
-                s_rownext[par][tid - 64] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
+            if (tid < P.batch) {  // wave 0, which builds pairs and issues no row loads
+                if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
+                if (P.pipe || early_dma) {
+                    const int jn = j0 + P.batch + tid;
+                    s_rownext[par][tid] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
+                }
             }
             if (pt_thread) {
                 const int pil = pil0 + pt_p;
@@ -1132,7 +1144,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             if (!__syncthreads_or(inap)) {
                 if (P.pipe) {  // this batch's rows are not needed
                     if (AA) {
-                        dma(j0 + P.batch, cur ^ 1, kRowsNext + par);
+                        dma(j0 + P.batch, cur ^ 1, kRowsNext + par, 0);
                         cur ^= 1;
                     } else {
                         prefetch(j0 + P.batch, kRowsNext + par);
@@ -1195,11 +1207,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 for (int i = 0; i < kPF; ++i)
                     if (tid + i * kThreads < n) dst[tid + i * kThreads] = pf[i];
             } else if (AA) {
-                // the batch's f64 rows by direct global -> LDS loads into the one
-                // row buffer: every load of a wave in flight at once and no
-                // registers held (a loop of load-then-store through registers
-                // waits out the memory latency once per element)
-                dma(j0, 0, kRowsThis);
+                // the batch's rows, loaded at the batch start (early_dma)
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
             } else {
                 // rows picked through the trace list: one flat loop over the
@@ -1232,7 +1240,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
             __syncthreads();
             if (P.pipe) {  // in flight during the batch
-                if (AA) dma(j0 + P.batch, cur ^ 1, kRowsNext + par);
+                if (AA) dma(j0 + P.batch, cur ^ 1, kRowsNext + par, 0);
                 else prefetch(j0 + P.batch, kRowsNext + par);
             }
 
@@ -1402,6 +1410,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             __syncthreads();
             if (AA && P.pipe) cur ^= 1;
         }
+        // a skipped last batch leaves its row loads in flight: none may outlive
+        // the item (nor the workgroup, whose LDS is then reassigned)
+        if (early_dma) __builtin_amdgcn_s_waitcnt(0x0F70);
 
         // the workgroup is the only writer of these cells
 #pragma unroll
