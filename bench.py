@@ -112,7 +112,7 @@ def cpu_baseline(ex, job, budget_s: float) -> dict:
 # dispatch scaled by true bytes / counter of the calibration kernel), per workload
 # (the committed rocprofv3 summary of the reflector gather: FETCH x 2.000 + WRITE
 # x 1.000, one plan launch)
-TRAFFIC: dict = {"reflectors": (6_949_227_349 * 2.000 + 256_268 * 1.000) * 1024}
+TRAFFIC: dict = {"reflectors": (8_834_774_528 * 2.000 + 256_297 * 1.000) * 1024}
 
 
 def learned_vs_exact(kernel, args) -> dict:
