@@ -230,7 +230,13 @@ typedef struct PstmPlan PstmPlan;
  * vrms, eta, image, fold and trace_index are ignored (per-tile values come
  * from `tiles`).  Geometry, velocity and fold terms are computed here, once;
  * amplitudes and (for the learned traveltime) mlp_params are read at every
- * run.  If common->n_effective is set, every run adds E to it. */
+ * run.  If common->n_effective is set, every run adds E to it.
+ * Each tile must own its image and fold memory: the gather adds into image
+ * cells and the fold counts into fold cells without atomics, so the byte
+ * ranges of all tiles' images and folds must be pairwise disjoint (bins that
+ * share an output tile go into ONE descriptor whose trace_index lists the
+ * rows of every bin).  Overlapping ranges are rejected with status 1 before
+ * the device is touched. */
 int pstm_hip_plan_create(const PstmMigrateArgs* common, const PstmTileDesc* tiles, int32_t n_tiles,
                          PstmPlan** plan);
 

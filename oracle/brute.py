@@ -15,8 +15,17 @@ import numpy as np
 
 def migrate_plain(amp, sx, sy, rx, ry, mx, my, w, x_axis, y_axis, t_axis_ms, vrms,
                   dt_in_ms, t_start_in_ms, max_ap, min_ap, dip_deg, taper,
-                  spreading=True, obliquity=True):
-    """Returns (image (nx,ny,nt) f64, fold2d (nx,ny) i64). vrms (nt,) or (nx,ny,nt)."""
+                  spreading=True, obliquity=True, aa=None):
+    """Returns (image (nx,ny,nt) f64, fold2d (nx,ny) i64). vrms (nt,) or (nx,ny,nt).
+
+    aa = (grid_dx, grid_dy, max_half_width): triangle anti-aliasing, written
+    without the oracle's integrated traces: the operator dip dt/dx, dt/dy by
+    central finite differences of the DSR time at the output point, half
+    width L = max(|dt/dx| dx, |dt/dy| dy) / dt clamped to [1, max_half_width],
+    and the triangles of the integer half widths floor(L) and floor(L) + 1 as
+    explicit convolutions of the zero-extended trace, blended with weight
+    L - floor(L) and interpolated linearly between the samples around the
+    position."""
     amp = np.asarray(amp, dtype=np.float64)
     ntr, ns = amp.shape
     X, Y = np.meshgrid(np.asarray(x_axis, float), np.asarray(y_axis, float), indexing="ij")
@@ -41,7 +50,10 @@ def migrate_plain(amp, sx, sy, rx, ry, mx, my, w, x_axis, y_axis, t_axis_ms, vrm
         ok = inside & (dm <= ap) & (u >= 0) & (u < ns - 1)
         i0 = np.where(ok, np.floor(u), 0).astype(np.int64)
         f = np.where(ok, u - i0, 0.0)
-        a = amp[rows, i0] * (1 - f) + amp[rows, np.minimum(i0 + 1, ns - 1)] * f
+        if aa is None:
+            a = amp[rows, i0] * (1 - f) + amp[rows, np.minimum(i0 + 1, ns - 1)] * f
+        else:
+            a = _aa_amplitude(amp, rows, ox, oy, sx, sy, rx, ry, t0, v, dt_in_ms / 1000.0, i0, f, aa)
         wt = np.broadcast_to(w[None, None, :], ok.shape).copy()
         ts = ap * (1 - taper)
         in_taper = dm > ts
@@ -66,3 +78,35 @@ def triangle_filter(trace: np.ndarray, n: int, L: int) -> float:
         if 0 <= j < len(trace):
             acc += (L - abs(m)) * float(trace[j])
     return acc / (L * L)
+
+
+def _aa_amplitude(amp, rows, ox, oy, sx, sy, rx, ry, t0, v, dt_s, i0, f, aa):
+    gdx, gdy, lmax = aa
+    h = 1e-3
+
+    def t_at(px, py):
+        return (np.sqrt((t0 / 2) ** 2 + ((px - sx) ** 2 + (py - sy) ** 2) / v ** 2)
+                + np.sqrt((t0 / 2) ** 2 + ((px - rx) ** 2 + (py - ry) ** 2) / v ** 2))
+
+    gx = (t_at(ox + h, oy) - t_at(ox - h, oy)) / (2 * h)
+    gy = (t_at(ox, oy + h) - t_at(ox, oy - h)) / (2 * h)
+    L = np.maximum(np.abs(gx) * gdx, np.abs(gy) * gdy) / dt_s
+    L = np.minimum(np.where(L > 1.0, L, 1.0), float(lmax))
+    lf = np.floor(L).astype(np.int64)
+    lam = L - lf
+    ntr, ns = amp.shape
+    pad = int(lmax) + 2
+    ext = np.zeros((ntr, ns + 2 * pad))
+    ext[:, pad:pad + ns] = amp
+
+    def tri(n, lint):
+        acc = np.zeros(n.shape)
+        for m in range(-int(lmax) - 1, int(lmax) + 2):
+            wgt = np.maximum(lint - abs(m), 0) / (lint * lint)
+            acc += wgt * ext[rows, n + m + pad]
+        return acc
+
+    def blended(n):
+        return (1 - lam) * tri(n, lf) + lam * tri(n, lf + 1)
+
+    return blended(i0) * (1 - f) + blended(i0 + 1) * f

@@ -48,6 +48,8 @@ def _load(fast: bool = False) -> ctypes.CDLL:
     lib.oracle_vti_leg.argtypes = [ctypes.c_double] * 4
     lib.oracle_interp_sample.restype = ctypes.c_double
     lib.oracle_interp_sample.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.oracle_triangle_at.restype = ctypes.c_double
+    lib.oracle_triangle_at.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     lib.oracle_integrate_traces.restype = None
     lib.oracle_integrate_traces.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_void_p]
@@ -194,6 +196,13 @@ def integrate_traces(amp: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
 
 __all__ = ["MigrateInputs", "OracleResult", "migrate", "build", "curved_dsr", "vti_dsr",
            "vti_leg", "integrate_traces", "AA_TRIANGLE", "ETA_3D"]
+
+
+def triangle_at(trace, u: float, L: float) -> float:
+    """Anti-aliased sample of ``trace`` (rounded to f32): triangle of half
+    width L >= 1 at fractional position u, FP64."""
+    tr = np.ascontiguousarray(trace, dtype=np.float32)
+    return float(_load().oracle_triangle_at(tr.ctypes.data, int(tr.size), float(u), float(L)))
 
 
 def interp_sample(trace, t: float, method: int) -> float:

@@ -376,6 +376,14 @@ void oracle_integrate_traces(const float* amp, int n_traces, int ns, double* p, 
     }
 }
 
+// Triangle filter of half width L at position u on one trace (the
+// anti-aliased sample of a contribution), for the filter checks.
+double oracle_triangle_at(const float* amp, int ns, double u, double L) {
+    Integrated I;
+    integrate_trace(amp, ns, I);
+    return triangle_at(I, u, L, ns);
+}
+
 // Scalar traveltimes, for the closed-form and golden checks.
 double oracle_curved_dsr(double t0_s, double h_s, double h_r, double v0, double k) {
     double z = time_to_depth_curved(t0_s / 2, v0, k);

@@ -2076,6 +2076,32 @@ void plan_free(PstmPlan* pl) {
     delete pl;
 }
 
+// Byte ranges [image, image + nx ny nt f32) and [fold, fold + nx ny i32) of
+// the tiles of a plan must not overlap one another (an image range may not
+// overlap a fold range either): sort the ranges by start; a range that starts
+// before the furthest end seen so far overlaps an earlier one.
+int check_disjoint(const std::vector<PstmMigrateArgs>& targs) {
+    struct Range { uintptr_t lo, hi; int tile; const char* what; };
+    std::vector<Range> r;
+    r.reserve(targs.size() * 2);
+    for (size_t i = 0; i < targs.size(); ++i) {
+        const PstmMigrateArgs& t = targs[i];
+        const uintptr_t im = reinterpret_cast<uintptr_t>(t.image), fo = reinterpret_cast<uintptr_t>(t.fold);
+        r.push_back({im, im + (uintptr_t)t.nx * t.ny * t.nt * sizeof(float), (int)i, "image"});
+        r.push_back({fo, fo + (uintptr_t)t.nx * t.ny * sizeof(int32_t), (int)i, "fold"});
+    }
+    std::sort(r.begin(), r.end(), [](const Range& a, const Range& b) { return a.lo < b.lo; });
+    size_t far = 0;  // range with the furthest end so far
+    for (size_t k = 1; k < r.size(); ++k) {
+        if (r[k].lo < r[far].hi)
+            return fail(1, std::string("pstm_hip_plan_create: the ") + r[far].what + " of tile " +
+                               std::to_string(r[far].tile) + " overlaps the " + r[k].what + " of tile " +
+                               std::to_string(r[k].tile) + " (every tile needs its own image and fold memory)");
+        if (r[k].hi > r[far].hi) far = k;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2085,9 +2111,9 @@ int pstm_hip_plan_create(const PstmMigrateArgs* common, const PstmTileDesc* tile
     g_last_error.clear();
     if (!common || !tiles || !out || n_tiles <= 0) return fail(1, "pstm_hip_plan_create: bad arguments");
     *out = nullptr;
-    int rc = ensure_init();
-    if (rc) return rc;
-    // per-tile argument sets, validated like single-tile calls
+    // per-tile argument sets, validated like single-tile calls (host checks
+    // only, before the device is touched)
+    int rc = 0;
     std::vector<PstmMigrateArgs> targs(n_tiles);
     for (int i = 0; i < n_tiles; ++i) {
         PstmMigrateArgs t = *common;
@@ -2108,6 +2134,12 @@ int pstm_hip_plan_create(const PstmMigrateArgs* common, const PstmTileDesc* tile
         targs[i] = t;
     }
     if (common->n_traces <= 0) return fail(1, "pstm_hip_plan_create: common.n_traces must be the rows of the block");
+    // the gather adds into image cells and k_fold_add into fold cells without
+    // atomics: each tile must own its image and fold memory
+    rc = check_disjoint(targs);
+    if (rc) return rc;
+    rc = ensure_init();
+    if (rc) return rc;
     const Shape sh = launch_shape(targs[0]);
     if (sh.lds > kLdsBudget) return fail(5, "trace too long for one LDS row (n_samples too large)");
     // heaviest tiles first

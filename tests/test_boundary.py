@@ -176,3 +176,55 @@ def test_roofline_constants_frozen():
     assert roofline.FLOPS_PER_CONTRIBUTION["straight"] == 19
     assert roofline.FLOPS_PER_CONTRIBUTION["straight_noweights"] == 16
     assert roofline.MI355X_FP32_VALU_TFLOPS == 157.3
+
+
+def _plan_common():
+    a = _abi.PstmMigrateArgs()
+    a.n_traces, a.n_samples, a.nt = 4, 10, 5
+    # fake, never dereferenced: the overlap test runs before any device work
+    a.amplitudes = a.source_x = a.source_y = a.receiver_x = a.receiver_y = 8
+    a.midpoint_x = a.midpoint_y = a.t_axis_ms = 8
+    a.dt_in_ms, a.max_aperture_m = 2.0, 100.0
+    return a
+
+
+def _desc(image: int, fold: int, nx: int = 2, ny: int = 3):
+    d = _abi.PstmTileDesc()
+    d.x_axis = d.y_axis = d.vrms = 8
+    d.image, d.fold = image, fold
+    d.trace_index, d.n_traces, d.nx, d.ny = 8, 4, nx, ny
+    return d
+
+
+def test_plan_rejects_tiles_that_share_image_or_fold_memory():
+    """The gather adds into image cells and the fold counts into fold cells
+    without atomics, so two descriptors over one buffer would lose updates:
+    plan creation refuses them before touching the device."""
+    lib = _abi.load_library()
+    a = _plan_common()
+    img_bytes = 2 * 3 * 5 * 4          # nx ny nt f32
+    h = ctypes.c_void_p()
+    cases = {
+        "same image": [_desc(0x10000, 0x90000), _desc(0x10000, 0xA0000)],
+        "image ranges overlap by one cell": [_desc(0x10000, 0x90000),
+                                             _desc(0x10000 + img_bytes - 4, 0xA0000)],
+        "fold ranges overlap": [_desc(0x10000, 0x90000), _desc(0x20000, 0x90000 + 8)],
+        "fold inside an image": [_desc(0x10000, 0x90000), _desc(0x20000, 0x10000 + 16)],
+        "third tile overlaps the first": [_desc(0x10000, 0x90000, nx=20), _desc(0x10100, 0xA0000),
+                                          _desc(0x30000, 0xB0000)],
+    }
+    for name, descs in cases.items():
+        arr = (_abi.PstmTileDesc * len(descs))(*descs)
+        rc = lib.pstm_hip_plan_create(ctypes.byref(a), arr, len(descs), ctypes.byref(h))
+        assert rc == 1, name
+        assert "overlaps" in _abi.last_error(lib), name
+    # adjacent, disjoint ranges pass the check (and then fail only for want of
+    # a device here)
+    descs = [_desc(0x10000, 0x90000), _desc(0x10000 + img_bytes, 0x90000 + 2 * 3 * 4)]
+    arr = (_abi.PstmTileDesc * 2)(*descs)
+    rc = lib.pstm_hip_plan_create(ctypes.byref(a), arr, 2, ctypes.byref(h))
+    assert "overlaps" not in _abi.last_error(lib)
+    import torch
+
+    if torch.cuda.device_count() == 0:
+        assert rc != 0
