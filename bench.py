@@ -1,20 +1,36 @@
-"""Benchmark: Gcontrib/s of the MI355X PSTM backend on one GPU.
+"""Benchmark: Gcontrib/s of learned-traveltime VTI migration on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload reflectors]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-A step is one migration pass over the whole synthetic job with every input
-already resident in HBM: for the default workload, 16 output tiles (64 x 64
-pillars x 1000 samples each) over 200,000 traces of 1501 samples with 3-D
-v_rms and triangle anti-aliasing, run as one plan (one persistent gather
-launch fed by a device work queue, replayed from a captured hipGraph).  A
-contribution is one (trace, output sample) pair inside the maximum aperture,
-C = sum of the 3-D fold; the rate is C / step time.  The roofline entry uses E
-(contributions that pass the time-dependent aperture and the sample bounds) x
-the frozen FLOPs per contribution over the gather launch's own duration,
-measured with HIP events on its stream.  The CPU baseline is the C++ oracle
-(built with -O3 -ffast-math, OpenMP) on a bounded pillar sample of the same
-job.  The learned_vs_exact entry times VTI migration with the learned and the
-exact traveltime on a reduced VTI diffractor job.
+Workload (the learned-vs-exact traveltime job): 500,000 traces x 2001
+samples (2 ms) with 300 random diffractors at their exact VTI times, 3-D
+v_rms (1500-4500 m/s) and eta (0-0.2) fields on 128 x 128 x 201 grids, output
+512 x 512 bins at 12.5 m x 1500 samples, aperture 2500 m, max dip 60 deg,
+spreading and obliquity weights, straight-ray VTI traveltime.  Every input is
+resident in HBM and the whole job is built (all traces, the full velocity and
+eta volumes, the full image and fold, every tile's trace list), but a step is
+one migration pass over a FIXED SUBSET of its output tiles (the tiles nearest
+the grid centre, ``--subset-tiles`` of edge ``--tile``): a pass over all
+512 x 512 pillars takes minutes, and the driver's 20 + 5 steps must fit in a
+few minutes.  C counts exactly the subset's contributions (sum of its 3-D
+fold), so the rate is that of the subset.
+
+The traveltime surrogate (5-64-64-64-64-2 SiLU MLP) is trained by the
+library's HIP kernels on 10M samples of the job's own v_rms / eta model
+(training time reported, not part of a step), checked per tile against the
+exact VTI time (0.25-sample held-out bar; failing tiles run the exact
+traveltime), and a step is one persistent, hipGraph-captured gather launch
+per plan (learned tiles, fallback tiles).  The same subset is migrated with
+the exact VTI traveltime (``exact_ms``, ``learned_speedup``) and the learned
+image is compared with the exact one (relative L2; target <= 1e-3).
+
+Also reported: the roofline of the learned gather (VALU FLOPs per effective
+contribution x E over the launch time, measured with HIP events on the
+launch stream), the plain straight-ray gather on the same subset and on the
+4096-trace point-diffractor job (both against the FP32 VALU peak), and the
+CPU baseline (the FP64 oracle, -O3 -ffast-math, OpenMP, on a pillar sample of
+the subset).  Side measurements on the reflector job (output step,
+time-variant sampling, interpolators, streaming) are opt-in: ``--sections``.
 
 Prints one JSON line.
 """
@@ -40,179 +56,338 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_job(name: str, device: str):
-    from pm_amd.synthetic import point_diffractor_job, reflector_job
+def subset_tiles(ex, n: int) -> list[int]:
+    """The n tiles whose centres are nearest the output grid's centre (ties by
+    tile index): a fixed, interior subset with full aperture coverage."""
+    import numpy as np
 
-    if name == "reflectors":
-        return reflector_job(device=device), (
-            "reflectors: 200000 traces x 1501 samples (2 ms), 40 dipping planes + 200 "
-            "diffractors, 3-D v_rms cube 64x64x151, output 256x256 bins at 12.5 m x 1000 "
-            "samples in 16 tiles of 64x64, aperture 2500 m, dip 60, triangle AA, f32")
-    if name == "bins":
-        from pm_amd.synthetic import bins_job
+    nx, ny, _ = ex.shape
+    cx, cy = (nx - 1) / 2.0, (ny - 1) / 2.0
 
-        return bins_job(device=device), (
-            "bins: 40 common-offset bins (50 m spacing), lognormal counts (median 30k, 5k-120k), "
-            "1.6M traces x 1501 samples, reflector model, output 256x256 x 1000 samples in 16 tiles "
-            "of 64x64, one shared image, no AA, f32")
-    if name == "point_diffractor":
-        return point_diffractor_job(device=device), (
-            "point diffractor: 4096 traces x 1001 samples, output 64x64 x 500, aperture "
-            "1000 m, weights on, no AA")
-    raise SystemExit(f"unknown workload {name}")
+    def dist(i):
+        t = ex.tiles[i]
+        return (np.hypot((t.x0 + t.x1 - 1) / 2.0 - cx, (t.y0 + t.y1 - 1) / 2.0 - cy), i)
+
+    return sorted(range(len(ex.tiles)), key=dist)[:n]
 
 
-def cpu_baseline(ex, job, budget_s: float) -> dict:
-    """Oracle (-O3 -ffast-math, OpenMP) on a pillar sample of the busiest
-    tile, all of that tile's traces, all output samples."""
+def reset_tiles(ex, tiles) -> None:
+    for i in tiles:
+        ex.tiles[i].image.zero_()
+        ex.tiles[i].fold.zero_()
+
+
+def subset_image(ex, tiles):
+    import torch
+
+    return torch.cat([ex.tiles[i].image.reshape(-1) for i in tiles]).double()
+
+
+def subset_contributions(ex, tiles) -> int:
+    return int(sum(int(ex.tiles[i].fold.sum()) for i in tiles)) * ex.nt
+
+
+def plan_launch_ms(lib, plans, reps: int = 1) -> float:
+    """Average duration of one pass over the plans (summed launch times, HIP
+    events on the launch stream, recorded by the library)."""
+    import torch
+
+    torch.cuda.synchronize()
+    lib.pstm_hip_profile_reset(1)
+    for _ in range(reps):
+        for p in plans:
+            p.run()
+    kms, nl = ctypes.c_double(), ctypes.c_longlong()
+    lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
+    lib.pstm_hip_profile_reset(0)
+    return kms.value / reps
+
+
+def count_effective(ex, tiles, **kw) -> int:
+    import torch
+
+    reset_tiles(ex, tiles)
+    p = ex.plan(tiles=tiles, count_effective=True, **kw)
+    p.run(use_graph=False)
+    torch.cuda.synchronize()
+    e = int(p.counter.item())
+    p.destroy()
+    reset_tiles(ex, tiles)
+    return e
+
+
+def roofline_entry(f: int, e: int, launch_ms: float, kernel: str) -> dict:
+    from pm_amd import roofline
+
+    achieved = f * e / (launch_ms / 1000.0) / 1e12
+    peak = roofline.MI355X_FP32_VALU_TFLOPS
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None, "kernel": kernel,
+            "flops_per_contribution": f, "effective_contributions_per_launch": int(e),
+            "avg_launch_ms": round(launch_ms, 3),
+            "frac_of_unpacked_fp32": round(achieved / roofline.MI355X_FP32_VALU_UNPACKED_TFLOPS, 4)}
+
+
+def cpu_baseline(ex, tile_index: int, budget_s: float, traveltime: int) -> dict:
+    """The FP64 oracle (-O3 -ffast-math, OpenMP on every host core) on whole
+    inlines of one tile: all of that tile's traces, all output samples, the
+    same settings and traveltime; the inline count is sized to about
+    budget_s of CPU time."""
     import numpy as np
 
     from oracle import oracle as orc
     from pm_amd.kernels import _abi
 
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    tile = max(ex.tiles, key=lambda t: int(t.rows.shape[0]))
+    tile = ex.tiles[tile_index]
     rows = tile.rows.cpu().numpy().astype(np.int64)
     h = ex.host_traces(rows)
     cfg = ex.config
-    ix = (tile.x1 - tile.x0) // 2
+    job = ex.job
     v3 = tile.vrms.cpu().numpy() if tile.vrms.dim() == 3 else None
     v1 = tile.vrms.cpu().numpy() if tile.vrms.dim() == 1 else None
+    e3 = tile.eta.cpu().numpy() if tile.eta is not None else None
     xa, ya = tile.x_axis.cpu().numpy(), tile.y_axis.cpu().numpy()
+    ny = tile.y1 - tile.y0
+    ix0 = (tile.x1 - tile.x0) // 2
 
     def run(n_p: int, n_x: int = 1):
+        sl = (slice(ix0, ix0 + n_x), slice(0, n_p))
         inp = orc.MigrateInputs(
             amplitudes=h["amplitudes"], source_x=h["source_x"], source_y=h["source_y"],
             receiver_x=h["receiver_x"], receiver_y=h["receiver_y"], midpoint_x=h["midpoint_x"],
-            midpoint_y=h["midpoint_y"], x_axis=xa[ix:ix + n_x], y_axis=ya[:n_p],
-            t_axis_ms=job.t_axis_ms, vrms=(v3[ix:ix + n_x, :n_p] if v3 is not None else v1),
-            dt_in_ms=job.dt_ms, max_aperture_m=cfg.max_aperture_m,
-            min_aperture_m=cfg.min_aperture_m, max_dip_deg=cfg.max_dip_degrees,
-            taper_fraction=cfg.taper_fraction, apply_spreading=cfg.apply_spreading,
-            apply_obliquity=cfg.apply_obliquity, grid_dx=cfg.grid_dx, grid_dy=cfg.grid_dy,
+            midpoint_y=h["midpoint_y"], x_axis=xa[sl[0]], y_axis=ya[sl[1]],
+            t_axis_ms=np.asarray(job.t_axis_ms, float), vrms=(v3[sl] if v3 is not None else v1),
+            eta=(e3[sl] if e3 is not None else None), dt_in_ms=job.dt_ms,
+            max_aperture_m=cfg.max_aperture_m, min_aperture_m=cfg.min_aperture_m,
+            max_dip_deg=cfg.max_dip_degrees, taper_fraction=cfg.taper_fraction,
+            apply_spreading=cfg.apply_spreading, apply_obliquity=cfg.apply_obliquity,
+            grid_dx=cfg.grid_dx, grid_dy=cfg.grid_dy, traveltime=traveltime,
             aa_mode=_abi.AA_TRIANGLE if cfg.antialias == "triangle" else _abi.AA_NONE,
-            aa_max_half_width=cfg.aa_max_half_width)
+            aa_max_half_width=cfg.aa_max_half_width, eta_constant=cfg.vti_eta_constant)
         t0 = time.perf_counter()
         res = orc.migrate(inp, threads=threads, fast=True)
-        return time.perf_counter() - t0, int(res.fold2d.sum()) * len(job.t_axis_ms), res
+        return time.perf_counter() - t0, int(res.fold2d.sum()) * len(job.t_axis_ms)
 
-    ny = tile.y1 - tile.y0
     run(1)                      # warm-up: library load, OpenMP thread start
-    dt1, _, _ = run(ny)         # one inline: every thread busy
-    n_x = int(max(1, min(tile.x1 - tile.x0 - ix, round(budget_s / max(dt1, 1e-3)))))
-    dt, c, _ = run(ny, n_x)
+    dt1, _ = run(ny)            # one inline: every thread busy
+    n_x = int(max(1, min(tile.x1 - tile.x0 - ix0, round(budget_s / max(dt1, 1e-3)))))
+    dt, c = run(ny, n_x)
     log(f"cpu baseline: {n_x} x {ny} pillars x {len(rows)} traces in {dt:.1f} s on {threads} threads")
-    return {"value": c / dt / 1e9, "unit": "Gcontrib/s", "cores": threads, "kind": "port",
-            "sample": (f"{n_x} x {ny} pillars (whole inlines of the busiest tile) x all {len(rows)} "
-                       f"traces of that tile x {len(job.t_axis_ms)} samples, same settings; "
-                       f"C_sample={c}, {dt:.2f} s; oracle C++ FP64 -O3 -ffast-math OpenMP")}
+    return {"value": round(c / dt / 1e9, 4), "unit": "Gcontrib/s", "cores": threads, "kind": "port",
+            "sample": (f"{n_x} x {ny} pillars (whole inlines of subset tile {tile_index}) x all {len(rows)} "
+                       f"traces of that tile x {len(job.t_axis_ms)} samples, same settings and exact "
+                       f"traveltime; C_sample={c}, {dt:.2f} s; oracle C++ FP64 -O3 -ffast-math OpenMP")}
 
 
-# HBM bytes per gather launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs (KiB per
-# dispatch scaled by true bytes / counter of the calibration kernel), per workload
-# (the committed rocprofv3 summary of the reflector gather: FETCH x 2.000 + WRITE
-# x 1.000, one plan launch)
-TRAFFIC: dict = {"reflectors": (8_834_774_528 * 2.000 + 256_297 * 1.000) * 1024}
+def point_diffractor_plain(kernel, reps: int) -> dict:
+    """The plain straight-ray gather on the 4096-trace point-diffractor job
+    (64 x 64 x 500 output, one tile, aperture 1000 m, weights on, no AA):
+    one plan replayed ``reps`` times, HIP events on the launch stream."""
+    import torch
+
+    from pm_amd import roofline
+    from pm_amd.kernels import _abi
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.synthetic import point_diffractor_job
+
+    lib = _abi.load_library()
+    job = point_diffractor_job(device="cuda")
+    ex = TileExecutor(kernel, job)
+    e = count_effective(ex, [0])
+    plan = ex.plan()
+    plan.run()                   # capture
+    torch.cuda.synchronize()
+    ms = plan_launch_ms(lib, [plan], reps)
+    ex.reset()
+    plan.run()
+    torch.cuda.synchronize()
+    c = ex.contributions()
+    info = plan.info()
+    plan.destroy()
+    f = roofline.flops_per_contribution("straight_ray", "none", True)
+    out = roofline_entry(f, e, ms, "k_migrate (straight ray, weights, linear; one plan launch)")
+    out.update({"workload": "point diffractor: 4096 traces x 1001 samples, 64x64 x 500 output, aperture 1000 m",
+                "contributions": c, "gcontrib_s": round(c / (ms / 1e3) / 1e9, 2), "launch": info,
+                "reps": reps})
+    return out
 
 
-def learned_vs_exact(kernel, args) -> dict:
-    """Config-4 family, reduced to fit a bench run: VTI diffractor job, the
-    surrogate trained on the job's own v_rms / eta model by the library's
-    kernels, per-tile held-out check, then one pass of every tile with the
-    exact VTI traveltime and one with the learned traveltime (each a plan:
-    one persistent graph launch), timed with HIP events; image error of the
-    learned image against the exact one, and of the isotropic image (the
-    method without the network) against it."""
+def vti_headline(kernel, args) -> dict:
+    """Config-4 job, full size; the learned-vs-exact comparison on the tile
+    subset.  Returns the bench line's fields."""
     import torch
     from dataclasses import replace
 
+    from pm_amd import roofline
     from pm_amd.kernels import _abi
     from pm_amd.pipeline.executor import TileExecutor
     from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set
     from pm_amd.synthetic import vti_diffractor_job
 
+    lib = _abi.load_library()
     t0 = time.perf_counter()
-    job = vti_diffractor_job(n_traces=args.vti_traces, n_samples=2001, n_out=args.vti_out, nt=1500, tile=64,
-                             device="cuda")
+    job = vti_diffractor_job(n_traces=args.vti_traces, n_samples=2001, n_out=args.vti_out, nt=1500,
+                             tile=args.tile, device="cuda")
+    torch.cuda.synchronize()
+    log(f"vti job built in {time.perf_counter() - t0:.1f} s: {job.n_traces} traces x {job.n_samples}")
+    t0 = time.perf_counter()
     ex = TileExecutor(kernel, job)
+    torch.cuda.synchronize()
+    sub = subset_tiles(ex, args.subset_tiles)
+    log(f"executor ready in {time.perf_counter() - t0:.1f} s: {len(ex.tiles)} tiles; subset {sub} "
+        f"rows {[int(ex.tiles[i].rows.shape[0]) for i in sub]}")
     vc, ec = job.cube, job.meta["eta_cube"]
     d_max = job.config.max_aperture_m + 1500.0
     norm = Normalizer(tau_max=1.5, d_max=d_max, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
-    torch.cuda.synchronize()
-    log(f"vti job ready in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     tr = make_training_set(vc, ec, args.train_samples, d_max, norm, seed=1, t_max_ms=3000.0)
+    torch.cuda.synchronize()
+    set_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
     sur = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=args.train_steps, batch=args.train_batch,
-                                                          qat_fraction=0.25, log_every=5000, verbose=True)
+                                                          qat_fraction=0.25, log_every=max(args.train_steps // 8, 1),
+                                                          verbose=True)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    held = sur.error_samples(tr, job.dt_ms, bf16=True) if args.train_samples <= 2_000_000 else None
     del tr
-    log(f"surrogate trained in {train_s:.1f} s: {sur.history}")
+    log(f"surrogate: training set {set_s:.1f} s, training {train_s:.1f} s")
     K = args.node_spacing
-    errs = ex.check_surrogate(sur, node_spacing=K)
-    learned_tiles = [i for i, t in enumerate(ex.tiles) if t.learned_ok]
-    exact_tiles = [i for i, t in enumerate(ex.tiles) if not t.learned_ok]
-    lib = _abi.load_library()
+    errs = ex.check_surrogate(sur, node_spacing=K, tiles=sub)
+    if args.force_learned:
+        for i in sub:
+            ex.tiles[i].learned_ok = True
+    learned = [i for i in sub if ex.tiles[i].learned_ok]
+    fallback = [i for i in sub if not ex.tiles[i].learned_ok]
+    log(f"held-out max error per subset tile (samples): {[round(e, 4) for e in errs]}; learned {learned}, "
+        f"exact fallback {fallback}")
 
-    def timed(plan_list):
-        ex.reset()
-        for p in plan_list:
-            p.run(use_graph=True)       # capture + warm
-        torch.cuda.synchronize()
-        ex.reset()
-        torch.cuda.synchronize()
-        lib.pstm_hip_profile_reset(1)
-        for p in plan_list:
-            p.run(use_graph=True)
-        kms, nl = ctypes.c_double(), ctypes.c_longlong()
-        lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
-        lib.pstm_hip_profile_reset(0)
-        img, _ = ex.assemble()
-        torch.cuda.synchronize()
-        return kms.value, img.double()
+    # exact VTI traveltime on the subset: E, one captured run, one timed run
+    e_exact = count_effective(ex, sub)
+    exact_plan = ex.plan(tiles=sub)
+    exact_plan.run()
+    torch.cuda.synchronize()
+    reset_tiles(ex, sub)
+    exact_ms = plan_launch_ms(lib, [exact_plan])
+    img_exact = subset_image(ex, sub)
+    c_sub = subset_contributions(ex, sub)
+    exact_info = exact_plan.info()
+    exact_plan.destroy()
+    log(f"exact VTI: {exact_ms:.1f} ms, C = {c_sub:.4e}, E = {e_exact:.4e}")
 
-    from pm_amd.autotune import autotune
-
-    vt = autotune(kernel, ex, candidates=[(2, 10, 0), (2, 16, 1), (4, 8, 0), (4, 16, 1)])
-    log(f"vti autotune: mw={vt.mw} batch={vt.batch} pipe={vt.pipe}")
-    exact_plan = ex.plan()
-    ms_exact, img_exact = timed([exact_plan])
-    plans = []
-    if learned_tiles:
-        plans.append(ex.plan(tiles=learned_tiles, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=K))
-    if exact_tiles:
-        plans.append(ex.plan(tiles=exact_tiles))
-    ms_learned, img_learned = timed(plans)
-    c_total = ex.contributions()
+    # the method without the network: isotropic straight-ray time in the same
+    # v_rms (also the plain straight-ray gather's measurement on this subset)
     saved = ex.config
     ex.config = replace(saved, kernel_type="straight_ray", vti_enabled=False)
-    iso_plan = ex.plan()
-    _, img_iso = timed([iso_plan])
+    e_plain = count_effective(ex, sub)
+    iso_plan = ex.plan(tiles=sub)
+    iso_plan.run()
+    torch.cuda.synchronize()
+    reset_tiles(ex, sub)
+    plain_ms = plan_launch_ms(lib, [iso_plan])
+    img_iso = subset_image(ex, sub)
+    plain_info = iso_plan.info()
+    iso_plan.destroy()
     ex.config = saved
+    reset_tiles(ex, sub)
+
+    # learned traveltime: the timed steps
+    e_learned = count_effective(ex, learned, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur,
+                                node_spacing=K) if learned else 0
+    plans = []
+    if learned:
+        plans.append(ex.plan(tiles=learned, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=K))
+    if fallback:
+        plans.append(ex.plan(tiles=fallback))
+    use_graph = not args.profile
+    for _ in range(args.warmup):
+        reset_tiles(ex, sub)
+        for p in plans:
+            p.run(use_graph=use_graph)   # the first run captures the graph
+    torch.cuda.synchronize()
+    lib.pstm_hip_profile_reset(1)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        reset_tiles(ex, sub)
+        for p in plans:
+            p.run(use_graph=use_graph)
+        torch.cuda.synchronize()
+        log(f"step {s} done")
+    elapsed = time.perf_counter() - t0
+    kms, nl = ctypes.c_double(), ctypes.c_longlong()
+    lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
+    lib.pstm_hip_profile_reset(0)
+    learned_step_ms = kms.value / max(args.steps, 1)
+    img_learned = subset_image(ex, sub)
+    learned_info = plans[0].info()
+    for p in plans:
+        p.destroy()
+    ms_step = elapsed * 1000.0 / max(args.steps, 1)
     nrm = float(torch.linalg.norm(img_exact))
     rel = float(torch.linalg.norm(img_learned - img_exact)) / nrm
     rel_iso = float(torch.linalg.norm(img_iso - img_exact)) / nrm
-    for p in [exact_plan, iso_plan, *plans]:
-        p.destroy()
-    out = {
-        "workload": (f"VTI diffractors: {job.n_traces} traces x 2001 samples, 3-D v_rms 1500-4500 and eta "
-                     f"0-0.2 on 128x128x201 grids, output {args.vti_out}x{args.vti_out} x 1500 samples in "
-                     f"{len(ex.tiles)} tiles, aperture 2500 m"),
-        "surrogate": {"train_samples": args.train_samples, "steps": args.train_steps, "batch": args.train_batch,
-                      "train_s": round(train_s, 1), "node_spacing": K,
-                      "heldout_max_samples_per_tile": [round(e, 4) for e in errs],
-                      "tiles_learned": len(learned_tiles), "tiles_exact_fallback": len(exact_tiles)},
-        "contributions": c_total,
-        "exact_gcontrib_s": round(c_total / (ms_exact / 1e3) / 1e9, 3),
-        "learned_gcontrib_s": round(c_total / (ms_learned / 1e3) / 1e9, 3),
-        "exact_ms": round(ms_exact, 2), "learned_ms": round(ms_learned, 2),
-        "launch_shape": {"mw": vt.mw, "batch": vt.batch, "pipe": vt.pipe},
-        "learned_speedup": round(ms_exact / ms_learned, 3),
-        "image_rel_l2_learned_vs_exact": rel,
-        "image_rel_l2_without_network_vs_exact": rel_iso,
+
+    # roofline of the learned gather (the learned plan's launches alone when
+    # some tiles fell back: timed separately)
+    f_learned = roofline.FLOPS_PER_CONTRIBUTION["vti_learned"]
+    if learned and not fallback:
+        rl = roofline_entry(f_learned, e_learned, learned_step_ms,
+                            "k_migrate<VTI_LEARNED> (fused BF16-MFMA lattice inference + VALU gather)")
+    elif learned:
+        lp = ex.plan(tiles=learned, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=K)
+        lp.run()
+        torch.cuda.synchronize()
+        rl = roofline_entry(f_learned, e_learned, plan_launch_ms(lib, [lp]),
+                            "k_migrate<VTI_LEARNED> (learned tiles only)")
+        lp.destroy()
+    else:
+        rl = roofline_entry(roofline.FLOPS_PER_CONTRIBUTION["vti"], e_exact, learned_step_ms,
+                            "k_migrate<VTI> (every subset tile fell back to the exact traveltime)")
+    rl["launches_timed"] = int(nl.value)
+    reset_tiles(ex, sub)
+    tiles_desc = ", ".join(f"x{ex.tiles[i].x0}-{ex.tiles[i].x1 - 1} y{ex.tiles[i].y0}-{ex.tiles[i].y1 - 1}"
+                           for i in sub)
+    res = {
+        "value": c_sub / (elapsed / max(args.steps, 1)) / 1e9, "ms_per_step": ms_step,
+        "config": {
+            "workload": (f"VTI diffractors (learned vs exact traveltime): {job.n_traces} traces x 2001 samples "
+                         f"(2 ms), 300 diffractors, 3-D v_rms 1500-4500 m/s and eta 0-0.2 on 128x128x201 grids, "
+                         f"output {args.vti_out}x{args.vti_out} bins at 12.5 m x 1500 samples in tiles of "
+                         f"{args.tile}x{args.tile}, aperture 2500 m, dip 60, weights on; the whole job resident "
+                         f"in HBM, a step migrates the {len(sub)} tile(s) nearest the grid centre "
+                         f"({tiles_desc})"),
+            "traces": job.n_traces, "samples": job.n_samples, "output": list(ex.shape),
+            "tiles_total": len(ex.tiles), "tiles_per_step": len(sub), "tile": args.tile,
+            "contributions_per_step": c_sub, "effective_contributions_per_step_learned": e_learned,
+            "traveltime": "learned VTI (MLP 5-64x4-2 SiLU, BF16 MFMA, FP32 accumulation) with per-tile exact "
+                          "fallback", "parallelism": "1 GPU",
+            "launch": "one persistent hipGraph-captured gather per plan per step"},
+        "roofline": rl,
+        "learned_vs_exact": {
+            "exact_ms": round(exact_ms, 2), "learned_ms": round(learned_step_ms, 2),
+            "learned_speedup": round(exact_ms / learned_step_ms, 3),
+            "exact_gcontrib_s": round(c_sub / (exact_ms / 1e3) / 1e9, 2),
+            "learned_gcontrib_s": round(c_sub / (learned_step_ms / 1e3) / 1e9, 2),
+            "image_rel_l2_learned_vs_exact": rel, "image_rel_l2_without_network_vs_exact": rel_iso,
+            "effective_contributions": {"exact": e_exact, "learned": e_learned},
+            "exact_roofline": roofline_entry(roofline.FLOPS_PER_CONTRIBUTION["vti"], e_exact, exact_ms,
+                                             "k_migrate<VTI> (exact VTI traveltime)"),
+            "launch_shape": {"learned": learned_info, "exact": exact_info},
+            "surrogate": {"train_samples": args.train_samples, "steps": args.train_steps,
+                          "batch": args.train_batch, "train_s": round(train_s, 2),
+                          "training_set_s": round(set_s, 2), "node_spacing": K,
+                          "heldout_max_samples_per_tile": [round(e, 4) for e in errs],
+                          "tiles_learned": len(learned), "tiles_exact_fallback": len(fallback),
+                          "training_set_error_samples": held, "forced": bool(args.force_learned)},
+        },
+        "plain_straight_ray": dict(roofline_entry(roofline.FLOPS_PER_CONTRIBUTION["straight"], e_plain, plain_ms,
+                                                  "k_migrate (straight ray, weights, linear) on the same subset"),
+                                   gcontrib_s=round(c_sub / (plain_ms / 1e3) / 1e9, 2), launch=plain_info),
     }
-    log(f"learned vs exact: {out}")
-    return out
+    res["_ex"] = ex
+    res["_sub"] = sub
+    return res
 
 
 def streamed_section(kernel, args) -> dict:
@@ -430,37 +605,83 @@ def interpolation_section(ex, methods=("linear", "cubic", "sinc8", "sinc16", "la
             "relative_to_linear": {m: round(v / out["linear"], 3) for m, v in out.items()}}
 
 
+def reflector_section(kernel, args, sections) -> dict:
+    """The anti-aliased reflector job (200,000 traces x 1501 samples, 3-D
+    v_rms, 256 x 256 x 1000 output in 16 tiles, triangle anti-aliasing):
+    one pass over every tile as one plan, timed with HIP events, and the
+    side sections that run on it."""
+    import torch
+
+    from pm_amd import roofline
+    from pm_amd.kernels import _abi
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.synthetic import reflector_job
+
+    lib = _abi.load_library()
+    lib.pstm_hip_set_tuning(0, 0)
+    lib.pstm_hip_set_pipeline(-1)
+    job = reflector_job(device="cuda")
+    ex = TileExecutor(kernel, job)
+    all_tiles = list(range(len(ex.tiles)))
+    e = count_effective(ex, all_tiles)
+    plan = ex.plan()
+    ex.reset()
+    plan.run()
+    torch.cuda.synchronize()
+    ex.reset()
+    ms = plan_launch_ms(lib, [plan])
+    c = ex.contributions()
+    out = roofline_entry(roofline.FLOPS_PER_CONTRIBUTION["triangle_aa"], e, ms,
+                         "k_migrate (straight ray + triangle AA, one plan over 16 tiles)")
+    out.update({"workload": ("reflectors: 200000 traces x 1501 samples, 40 planes + 200 diffractors, 3-D v_rms "
+                             "64x64x151, output 256x256 x 1000 in 16 tiles of 64x64, aperture 2500 m, dip 60, "
+                             "triangle AA"), "contributions": c, "gcontrib_s": round(c / (ms / 1e3) / 1e9, 2),
+                "launch": plan.info()})
+    plan.destroy()
+    log(f"reflectors: {ms:.0f} ms, {out['gcontrib_s']} Gcontrib/s")
+    if "output" in sections:
+        out["output"] = output_section(kernel, ex)
+    if "time_variant" in sections:
+        img_u, _ = ex.assemble()
+        out["time_variant"] = time_variant_section(kernel, job, img_u, ms, e)
+        del img_u
+    if "interpolation" in sections:
+        out["interpolation"] = interpolation_section(ex)
+    lib.pstm_hip_set_tuning(0, 0)
+    lib.pstm_hip_set_pipeline(-1)
+    del ex
+    torch.cuda.empty_cache()
+    return out
+
+
+SECTIONS = ("reflectors", "output", "time_variant", "interpolation", "stream")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="reflectors", choices=["reflectors", "point_diffractor", "bins"])
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="skip timing candidate launch shapes on the busiest tile")
-    ap.add_argument("--candidates", default="4x8x0,4x16x1,2x10x0,4x3x1",
-                    help="autotune candidates as MWxBATCHxPIPE,... (PIPE: 0 off, 1 pipelined staging)")
-    ap.add_argument("--profile", action="store_true",
-                    help="profiler run: one elementwise kernel over 256 MiB (traffic calibration), "
-                         "then the steps as eager launches; no counting pass, no CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="target CPU time of the oracle baseline sample (0 = skip)")
-    ap.add_argument("--no-learned", dest="learned", action="store_false",
-                    help="skip the learned-vs-exact VTI traveltime comparison")
-    ap.add_argument("--vti-traces", type=int, default=100_000)
-    ap.add_argument("--vti-out", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--tile", type=int, default=32, help="output tile edge of the VTI job (bins)")
+    ap.add_argument("--subset-tiles", type=int, default=1,
+                    help="tiles per step: the ones nearest the output grid's centre")
+    ap.add_argument("--vti-traces", type=int, default=500_000)
+    ap.add_argument("--vti-out", type=int, default=512)
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=80_000)
     ap.add_argument("--train-batch", type=int, default=16_384)
     ap.add_argument("--node-spacing", type=int, default=16)
-    ap.add_argument("--no-interpolation", dest="interpolation", action="store_false",
-                    help="skip the interpolator timing section")
-    ap.add_argument("--no-time-variant", dest="time_variant", action="store_false",
-                    help="skip the time-variant sampling section")
-    ap.add_argument("--no-output", dest="output", action="store_false",
-                    help="skip the output-step section (header sums, normalisation, finalize write-out)")
-    ap.add_argument("--stream", action="store_true",
-                    help="also measure migration streamed from disk (scaled config-5 bins job)")
+    ap.add_argument("--force-learned", action="store_true",
+                    help="use the learned traveltime on every subset tile whatever its held-out error "
+                         "(profiling runs with a briefly trained network)")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: the learned plans launched directly (no graph), no CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the oracle baseline sample (0 = skip)")
+    ap.add_argument("--plain-reps", type=int, default=20,
+                    help="replays of the point-diffractor plan for the plain-kernel roofline (0 = skip)")
+    ap.add_argument("--sections", default="",
+                    help="comma list of side measurements on the reflector job: " + ", ".join(SECTIONS))
     ap.add_argument("--stream-bins", type=int, default=60)
     ap.add_argument("--stream-traces", type=int, default=240_000)
     args = ap.parse_args()
@@ -469,150 +690,51 @@ def main() -> None:
               "by output tile with no collective, so N GPUs run N independent tile sets",
               file=sys.stderr)
         sys.exit(2)
+    sections = {s for s in args.sections.split(",") if s}
+    unknown = sections - set(SECTIONS)
+    if unknown:
+        raise SystemExit(f"unknown sections {sorted(unknown)}; choose from {SECTIONS}")
 
-    import numpy as np
     import torch
 
-    from pm_amd import roofline
     from pm_amd.kernels import HipKernel
+    from pm_amd.kernels import KernelConfig
     from pm_amd.kernels import _abi
-    from pm_amd.pipeline.executor import TileExecutor
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = "cuda"
-    t0 = time.perf_counter()
-    job, desc = make_job(args.workload, dev)
-    torch.cuda.synchronize()
-    log(f"job built in {time.perf_counter() - t0:.1f} s: {job.n_traces} traces x {job.n_samples}")
     kernel = HipKernel()
-    kernel.initialize(job.config)
-    ex = TileExecutor(kernel, job)
-    torch.cuda.synchronize()
-    log(f"executor ready: {len(ex.tiles)} tiles, rows/tile {[int(t.rows.shape[0]) for t in ex.tiles][:4]}...")
-    tune = None
-    if args.autotune and not args.profile:
-        from pm_amd.autotune import autotune
-
-        cands = None
-        if args.candidates:
-            cands = [tuple(int(v) for v in c.split("x")) for c in args.candidates.split(",")]
-        tune = autotune(kernel, ex, candidates=cands)
-        log(f"autotune: mw={tune.mw} batch={tune.batch} pipe={tune.pipe} ({tune.ms:.1f} ms on the 4 busiest tiles)")
-    lib = _abi.load_library()
-
+    kernel.initialize(KernelConfig())
+    t_all = time.perf_counter()
+    head = vti_headline(kernel, args)
+    ex, sub = head.pop("_ex"), head.pop("_sub")
     if args.profile:
-        # elementwise kernel of known size (a name nothing else here launches):
-        # reference point for the HBM byte counters
-        a = torch.full((64 * 2**20,), 7, dtype=torch.int32, device=dev)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        torch.bitwise_not(a, out=b)
-        torch.cuda.synchronize()
-        log("calibration bitwise_not: 268435456 bytes read, 268435456 bytes written")
-        del a, b
-        plan = ex.plan()
-        for _ in range(args.steps):
-            ex.reset()
-            plan.run(use_graph=False)
-            torch.cuda.synchronize()
-            log("profile step done")
-        plan.destroy()
+        log("profile run done")
         kernel.cleanup()
         return
-
-    # E (counting plan, not timed) and C
-    cplan = ex.plan(count_effective=True)
-    ex.reset()
-    cplan.run(use_graph=False)
-    torch.cuda.synchronize()
-    e_total = int(cplan.counter.item())
-    cplan.destroy()
-    c_total = ex.contributions()
-    img, _ = ex.assemble()
-    assert bool(torch.isfinite(img).all()) and float(img.abs().max()) > 0, "empty or non-finite image"
-    log(f"C = {c_total:.4e} contributions, E = {e_total:.4e}")
-
-    plan = ex.plan()
-    for _ in range(args.warmup):
-        ex.reset()
-        plan.run()           # the first run captures the graph
-    torch.cuda.synchronize()
-    # HIP events on the launch stream around every graph launch (the gather
-    # plus a few small kernels: fold add, queue reset, AA row integration)
-    lib.pstm_hip_profile_reset(1)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ex.reset()
-        plan.run()
-        torch.cuda.synchronize()
-        log("step done")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_step = elapsed * 1000.0 / args.steps
-    value = c_total / (elapsed / args.steps) / 1e9
-    kms, nl = ctypes.c_double(), ctypes.c_longlong()
-    lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
-    lib.pstm_hip_profile_reset(0)
-    launch = plan.info()
-    plan.destroy()
-
-    cfg = job.config
-    f = roofline.flops_per_contribution(cfg.kernel_type, cfg.antialias,
-                                        bool(cfg.apply_spreading or cfg.apply_obliquity))
-    launches = int(nl.value)
-    launch_ms = kms.value / max(launches, 1)
-    e_per_launch = e_total          # one gather launch covers every tile
-    achieved = f * e_per_launch / (launch_ms / 1000.0) / 1e12
-    peak = roofline.MI355X_FP32_VALU_TFLOPS
-    rl = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-          "frac": round(achieved / peak, 4), "traffic": TRAFFIC.get(args.workload),
-          "kernel": "k_migrate (gather, all tiles in one persistent launch; timed as its graph)",
-          "flops_per_contribution": f,
-          "effective_contributions_per_launch": int(e_per_launch),
-          "avg_launch_ms": round(launch_ms, 3), "launches_timed": launches,
-          "frac_of_unpacked_fp32": round(achieved / roofline.MI355X_FP32_VALU_UNPACKED_TFLOPS, 4)}
-    output = output_section(kernel, ex) if args.output else None
-    tvar = None
-    if args.time_variant:
-        img_u, _ = ex.assemble()
-        tvar = time_variant_section(kernel, job, img_u, ms_step, e_total)
-        del img_u
-    interp = interpolation_section(ex) if args.interpolation else None
-    # the VTI job has its own launch shape: back to the built-in choice
-    lib.pstm_hip_set_tuning(0, 0)
-    lib.pstm_hip_set_pipeline(-1)
-    learned = learned_vs_exact(kernel, args) if args.learned else None
-    streamed = streamed_section(kernel, args) if args.stream else None
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(ex, job, args.cpu_seconds)
+        busiest = max(sub, key=lambda i: int(ex.tiles[i].rows.shape[0]))
+        cpu = cpu_baseline(ex, busiest, args.cpu_seconds, _abi.TT_VTI)
+    del ex
+    torch.cuda.empty_cache()
+    plain_pd = point_diffractor_plain(kernel, args.plain_reps) if args.plain_reps > 0 else None
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Gcontrib/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 2),
+        "metric": METRIC, "value": round(head["value"], 3), "unit": "Gcontrib/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded, generated on the GPU)",
-        "config": {"workload": desc, "traces": job.n_traces, "samples": job.n_samples,
-                   "output": list(ex.shape), "tiles": len(ex.tiles),
-                   "contributions": c_total, "effective_contributions": e_total,
-                   "traveltime": "exact straight-ray DSR", "parallelism": "1 GPU",
-                   "launch": "one persistent hipGraph-captured gather per step over every tile"},
-        "roofline": rl, "cpu_baseline": cpu,
-        "launch": launch,
+        "config": head["config"], "roofline": head["roofline"], "cpu_baseline": cpu,
+        "learned_vs_exact": head["learned_vs_exact"],
+        "plain_straight_ray": {"vti_subset": head["plain_straight_ray"], "point_diffractor": plain_pd},
     }
-    if learned is not None:
-        out["learned_vs_exact"] = learned
-    if streamed is not None:
-        out["streamed"] = streamed
-    if output is not None:
-        out["output"] = output
-    if tvar is not None:
-        out["time_variant"] = tvar
-    if interp is not None:
-        out["interpolation"] = interp
-    if tune is not None:
-        out["autotune"] = tune.as_dict()
     if cpu:
-        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        out["gpu_over_cpu"] = {"learned": round(head["value"] / cpu["value"], 1),
+                               "exact": round(head["learned_vs_exact"]["exact_gcontrib_s"] / cpu["value"], 1)}
+    if sections & {"reflectors", "output", "time_variant", "interpolation"}:
+        out["reflectors"] = reflector_section(kernel, args, sections)
+    if "stream" in sections:
+        out["streamed"] = streamed_section(kernel, args)
+    out["wall_s"] = round(time.perf_counter() - t_all, 1)
     print(json.dumps(out), flush=True)
     kernel.cleanup()
 

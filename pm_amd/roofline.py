@@ -11,6 +11,12 @@ FMA = 2, add/sub/mul = 1, sqrt/rsq/rcp/log/exp = 1, compares/floor/convert = 0.
   curved ray 25, VTI 31, triangle anti-aliasing 47, MLP 5-64-64-64-64-2
     25,472 per evaluation: fixed before their first measurement and kept
     regardless of how the kernels evaluate them.
+  learned VTI traveltime, VALU part of a contribution: the anisotropic
+    factor of each leg interpolated between lattice samples (sub + FMA, 3
+    per leg: 6) + per leg h s (1), 1 + c (1), FMA radicand (2) (8) + 2 sqrt +
+    add (3) + the straight-ray tail (sample index FMA 2, frac 1, lerp 3,
+    rcp 1, t0/(v t^2) 2, pair weight 1, accumulate FMA 2: 12) = 29; the
+    network evaluations on the lattice are MFMA work, counted apart.
 """
 
 from __future__ import annotations
@@ -21,6 +27,7 @@ FLOPS_PER_CONTRIBUTION = {
     "curved": 25,
     "vti": 31,
     "triangle_aa": 47,
+    "vti_learned": 29,
 }
 MLP_FLOPS_PER_EVAL = 2 * (5 * 64 + 3 * 64 * 64 + 64 * 2)  # 25,472
 
