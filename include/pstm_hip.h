@@ -192,6 +192,21 @@ int pstm_hip_last_launch(int* mw, int* batch, int* grid, int* items);
  * of _migrate_tile_kernel_optimized at numba_cpu_optimized.py:593-618. */
 int pstm_hip_migrate_tile(const PstmMigrateArgs* args, void* stream);
 
+/* pstm_hip_migrate_tile without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as pstm_hip_migrate_tile, but the call's
+ * temporaries (tile-relative geometry, per-sample terms, queue counter,
+ * anti-aliasing rows, network operands) live in a fixed 64 MiB device array
+ * of the library, one copy per GPU held until the process ends, and the call
+ * makes no hipMalloc / hipFree.  Calls whose temporaries exceed 64 MiB are
+ * rejected with status 1 (about 32 B per trace, 16 B per (pillar, sample)
+ * with 3-D velocity, 8 B per (pillar, sample) with 3-D eta, 8 B per
+ * anti-aliasing row sample).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads) race.  Launch
+ * profiling (pstm_hip_profile_reset(1)) must be off. */
+int pstm_hip_migrate_tile_noalloc(const PstmMigrateArgs* args, void* stream);
+
 /* Fold only: per-pillar count of traces with midpoint distance <=
  * max_aperture_m (numba_cpu_optimized.py:167-175,231). */
 int pstm_hip_fold_tile(const PstmMigrateArgs* args, void* stream);

@@ -55,6 +55,10 @@ int pstm_gather_part4(int mw, bool count, bool lob, int interp, const void* kp, 
 int pstm_gather_part5(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
 int pstm_gather_part6(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
 
+#ifndef PSTM_GATHER_PART
+static __device__ __align__(256) unsigned char scratch[(size_t)64 << 20];
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -1462,6 +1466,14 @@ __global__ void k_sum_counts(const unsigned long long* part, int n, unsigned lon
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Fixed scratch of the allocation-free entry point
+// (pstm_hip_migrate_tile_noalloc): every GPU the library runs on holds one
+// copy for the life of the process.  In host code `scratch` is only a handle
+// for hipGetSymbolAddress; its device address, per device, is g_scratch[dev].
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+char* g_scratch[kMaxDevices] = {};
+
 struct DeviceState {
     int device = -1;
     int n_cu = 0;
@@ -1892,8 +1904,12 @@ int dispatch(const Shape& sh, bool count, bool lob, const KParams& kp, int grid,
 }
 
 // shared by migrate_tile and migrate_bins: `ntr` rows starting at row
-// offset `row0` of args->trace_index (or of the block).
-int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hipStream_t st) {
+// offset `row0` of args->trace_index (or of the block).  The temporaries
+// live in the library's workspace (grown with hipMalloc when too small), or,
+// when fixed_ws is set, in that buffer of fixed_ws_size bytes and nowhere
+// else (status 1 when they do not fit).
+int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hipStream_t st,
+                  char* fixed_ws = nullptr, size_t fixed_ws_size = 0) {
     int rc = validate(in);
     if (rc) return rc;
     rc = ensure_init();
@@ -1932,9 +1948,17 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     const size_t o_q = off; off = align_up(off + 8, 256);
     const size_t o_view = off; off = align_up(off + sizeof(TileView), 256);
     const size_t o_prow = off; off = align_up(off + (sh.aa ? (size_t)ntr * sh.rowlen * 8 : 0), 256);
-    rc = ensure_workspace(off);
-    if (rc) return rc;
-    char* ws = static_cast<char*>(g_state.ws);
+    char* ws = fixed_ws;
+    if (fixed_ws) {
+        if (off > fixed_ws_size)
+            return fail(1, "this migration needs " + std::to_string(off) + " bytes of temporaries; the "
+                           "allocation-free entry point holds at most " + std::to_string(fixed_ws_size) +
+                           " (fewer traces or pillars per call, or 1-D velocity)");
+    } else {
+        rc = ensure_workspace(off);
+        if (rc) return rc;
+        ws = static_cast<char*>(g_state.ws);
+    }
     const TileBufs tb = carve_tile(ws + o_tile, a, sh, ntr);
     float* qk = reinterpret_cast<float*>(ws + o_qk);
     float* t0k = reinterpret_cast<float*>(ws + o_t0);
@@ -2401,6 +2425,32 @@ int pstm_hip_migrate_tile(const PstmMigrateArgs* args, void* stream) {
     g_last_error.clear();
     if (!args) return fail(1, "args is NULL");
     return run_migration(args, 0, args->n_traces, 1, static_cast<hipStream_t>(stream));
+}
+
+int pstm_hip_migrate_tile_noalloc(const PstmMigrateArgs* args, void* stream) {
+    g_last_error.clear();
+    if (!args) return fail(1, "args is NULL");
+    int rc = validate(args);
+    if (rc) return rc;
+    if (g_prof.on)
+        return fail(1, "pstm_hip_migrate_tile_noalloc: launch profiling (pstm_hip_profile_reset(1)) creates HIP "
+                       "events per launch; switch it off for allocation-free calls");
+    rc = ensure_init();
+    if (rc) return rc;
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(4, "pstm_hip_migrate_tile_noalloc: device index out of range");
+    if (!g_scratch[dev]) {
+        // once per device, with that device current: the address of this
+        // device's copy of the scratch array
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(3, "pstm_hip_migrate_tile_noalloc: no device address for the scratch array");
+        g_scratch[dev] = static_cast<char*>(ptr);
+    }
+    static_assert(sizeof(scratch) == kScratchBytes, "scratch size");
+    return run_migration(args, 0, args->n_traces, 1, static_cast<hipStream_t>(stream), g_scratch[dev],
+                         kScratchBytes);
 }
 
 int pstm_hip_fold_tile(const PstmMigrateArgs* args, void* stream) {

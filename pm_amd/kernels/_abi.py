@@ -89,7 +89,7 @@ class PstmTileDesc(ctypes.Structure):
 EXPORTED_SYMBOLS = (
     "pstm_hip_abi_version", "pstm_hip_args_size", "pstm_hip_last_error",
     "pstm_hip_init", "pstm_hip_release", "pstm_hip_stream_sync",
-    "pstm_hip_migrate_tile", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
+    "pstm_hip_migrate_tile", "pstm_hip_migrate_tile_noalloc", "pstm_hip_fold_tile", "pstm_hip_migrate_bins",
     "pstm_hip_profile_reset", "pstm_hip_profile_read",
     "pstm_hip_set_tuning", "pstm_hip_set_pipeline", "pstm_hip_last_launch",
     "pstm_hip_plan_create", "pstm_hip_plan_run", "pstm_hip_plan_info", "pstm_hip_plan_destroy",
@@ -126,6 +126,8 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_hip_stream_sync.argtypes = [_vp]
     lib.pstm_hip_migrate_tile.restype = ctypes.c_int
     lib.pstm_hip_migrate_tile.argtypes = [P, _vp]
+    lib.pstm_hip_migrate_tile_noalloc.restype = ctypes.c_int
+    lib.pstm_hip_migrate_tile_noalloc.argtypes = [P, _vp]
     lib.pstm_hip_fold_tile.restype = ctypes.c_int
     lib.pstm_hip_fold_tile.argtypes = [P, _vp]
     lib.pstm_hip_migrate_bins.restype = ctypes.c_int
