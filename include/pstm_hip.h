@@ -373,6 +373,30 @@ int pstm_mlp_train_step(float* params, float* m, float* v, float* grads, const f
                         int n, float lr, float beta1, float beta2, float eps, int step, float* loss,
                         void* stream);
 
+/* The whole training run on `stream`, synchronous: `steps` Adam iterations
+ * (bias-corrected step counts step0 + 1 .. step0 + steps) of the mean squared
+ * error over minibatches of `batch` rows of (x (n, 5), target (n, 2)), drawn
+ * on the device by a counter-based hash of (seed, iteration, sample); the
+ * learning rate decays from lr to lr_final on a cosine over the run;
+ * iterations >= q_from train through the fused inference's bf16 numerics
+ * (quantisation-aware, straight-through gradients).  One fused kernel per
+ * iteration computes forward, loss and backward of the minibatch in LDS (no
+ * per-layer launches), a second reduces the gradient and applies Adam; runs
+ * of iterations are captured once into a hipGraph and replayed.  If loss_log
+ * (device f32) is set, it receives the loss of iterations 0, log_every,
+ * 2 log_every, ... and of the last one (ceil(steps / log_every) + 1 slots).
+ * The training loop the reference would own has no counterpart there (it has
+ * no learned traveltime). */
+int pstm_mlp_train(float* params, float* m, float* v, const float* x, const float* target, long long n, int steps,
+                   int batch, float lr, float lr_final, float beta1, float beta2, float eps, int step0, int q_from,
+                   unsigned long long seed, int log_every, float* loss_log, void* stream);
+
+/* Loss and gradient of rows [0, n) by the fused training kernel (the
+ * gradient check of pstm_mlp_train), in the precision of
+ * pstm_mlp_set_precision. */
+int pstm_mlp_loss_grad_fused(const float* params, const float* x, const float* target, int n, float* grads,
+                             float* loss, void* stream);
+
 /* Exact one-way VTI times in f64 (pstm/algorithm/anisotropy_vti.py:181-220):
  * t_out[i] = t(tau[i], h[i], v[i], eta[i]); the surrogate's training targets. */
 int pstm_vti_leg_times(const double* tau, const double* h, const double* v, const double* eta, long long n,

@@ -96,7 +96,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_bin_headers", "pstm_hip_select_rows", "pstm_hip_fold_normalize", "pstm_hip_resample_uniform",
     "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_set_precision", "pstm_mlp_forward",
     "pstm_mlp_forward_mfma", "pstm_mlp_loss_grad",
-    "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_vti_leg_times",
+    "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_mlp_train", "pstm_mlp_loss_grad_fused", "pstm_vti_leg_times",
 )
 
 MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 5, 64, 2, 12994
@@ -180,6 +180,11 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_mlp_train_step.restype = ctypes.c_int
     lib.pstm_mlp_train_step.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, f, f, f, f,
                                         ctypes.c_int, _vp, _vp]
+    lib.pstm_mlp_train.restype = ctypes.c_int
+    lib.pstm_mlp_train.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, f, f,
+                                   f, f, f, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_int, _vp, _vp]
+    lib.pstm_mlp_loss_grad_fused.restype = ctypes.c_int
+    lib.pstm_mlp_loss_grad_fused.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
     lib.pstm_vti_leg_times.restype = ctypes.c_int
     lib.pstm_vti_leg_times.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_longlong, _vp, _vp]
     return lib

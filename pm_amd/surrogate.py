@@ -245,8 +245,45 @@ class TraveltimeSurrogate:
               verbose: bool = False):
         """Adam with a cosine-decayed learning rate over `steps` minibatches;
         the last qat_fraction of the steps train through the bf16 numerics of
-        the fused inference (quantisation-aware).  Every log_every steps the
-        loss is recorded in history (and printed to stderr when verbose)."""
+        the fused inference (quantisation-aware).  The whole run is one
+        library call (pstm_mlp_train): one fused forward/backward kernel and
+        one Adam kernel per step, minibatches drawn on the device, replayed
+        from a captured hipGraph.  Every log_every steps the loss is recorded
+        in history (and printed to stderr when verbose)."""
+        import sys
+
+        import torch
+
+        lib = _abi.load_library()
+        q_from = steps - int(round(qat_fraction * steps))
+        n = int(ts.x.shape[0])
+        x = ts.x.float().contiguous()
+        y = ts.y.float().contiguous()
+        le = int(log_every) if log_every else 0
+        slots = (steps + le - 1) // le + 1 if le else 0
+        log = torch.zeros(max(slots, 1), dtype=torch.float32, device=x.device)
+        _check(lib, lib.pstm_mlp_train(_p(self.params), _p(self.m), _p(self.v), _p(x), _p(y), n, int(steps),
+                                       int(batch), float(lr), float(lr_final), 0.9, 0.999, 1e-8, int(self.step),
+                                       int(q_from), int(seed) & (2**64 - 1), le, _p(log) if le else None,
+                                       _stream()), "pstm_mlp_train")
+        self.step += steps
+        if le:
+            vals = log.cpu().numpy()
+            its = list(range(0, steps, le))
+            if its[-1] != steps - 1:
+                its.append(steps - 1)
+            for j, i in enumerate(its):
+                self.history.append((i, float(vals[j])))
+                if verbose:
+                    print(f"[surrogate] step {i}: loss {vals[j]:.3e}", file=sys.stderr, flush=True)
+        return self
+
+    def train_stepwise(self, ts: TrainingSet, steps: int, batch: int = 8192, lr: float = 2e-3,
+                       lr_final: float = 5e-5, seed: int = 0, log_every: int = 0, qat_fraction: float = 0.0,
+                       verbose: bool = False):
+        """The same schedule as ``train`` with one host-driven library call
+        per step (pstm_mlp_train_step: per-layer kernels, minibatches drawn
+        by torch): the reference for the fused training."""
         import sys
 
         import torch

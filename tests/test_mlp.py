@@ -171,6 +171,102 @@ class TestHipSurrogate:
             print(mode, "max abs error / max |y|", err)
             assert err < 2e-5, mode
 
+    def test_fused_gradient_matches_fp64(self):
+        """The fused training kernel's loss and gradient (forward, loss and
+        backward of 32-sample chunks in LDS, per-workgroup partials) against
+        the FP64 NumPy network; n = 9,999 is not a multiple of 32 and spans
+        more chunks than workgroups."""
+        import torch
+
+        from pm_amd.surrogate import _stream
+
+        lib = _abi.load_library()
+        rng = np.random.default_rng(15)
+        p = _rand_params(15)
+        n = 9999
+        x = rng.uniform(-1, 1, (n, 5))
+        t = rng.uniform(-0.1, 0.05, (n, 2))
+        pd = torch.as_tensor(p.astype(np.float32), device="cuda")
+        xd = torch.as_tensor(x, dtype=torch.float32, device="cuda")
+        td = torch.as_tensor(t, dtype=torch.float32, device="cuda")
+        g = torch.full((_abi.MLP_PARAMS,), float("nan"), device="cuda")
+        loss = torch.zeros(1, device="cuda")
+        assert lib.pstm_mlp_loss_grad_fused(pd.data_ptr(), xd.data_ptr(), td.data_ptr(), n, g.data_ptr(),
+                                            loss.data_ptr(), _stream()) == 0
+        lr_, gr = omlp.loss_grad(p.astype(np.float32).astype(np.float64), x.astype(np.float32), t.astype(np.float32))
+        g = g.cpu().numpy()
+        err_g = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+        # per layer too: a dropped term in one small layer would hide in the total
+        offs = np.cumsum([0] + [a * b + b for a, b in [(5, 64), (64, 64), (64, 64), (64, 64), (64, 2)]])
+        per = [np.linalg.norm(g[a:b] - gr[a:b]) / np.linalg.norm(gr[a:b]) for a, b in zip(offs[:-1], offs[1:])]
+        print(f"fused gradient rel-L2 {err_g:.2e}, per layer {[f'{e:.1e}' for e in per]}, loss "
+              f"{float(loss.item()):.6e} vs {lr_:.6e}")
+        assert err_g < 1e-4 and max(per) < 1e-4
+        assert abs(float(loss.item()) - lr_) <= 1e-5 * lr_
+
+    def test_fused_gradient_bf16_mode_matches_stepwise(self):
+        """Quantisation-aware numerics: the fused gradient against the
+        per-layer kernels' gradient in the same bf16 mode (straight-through;
+        the fused kernel's layer-0 weight gradient uses the input as the bf16
+        forward saw it, so the two agree to bf16 input rounding there)."""
+        import torch
+
+        from pm_amd.surrogate import _stream
+
+        lib = _abi.load_library()
+        rng = np.random.default_rng(16)
+        p = _rand_params(16).astype(np.float32)
+        n = 4096
+        pd = torch.as_tensor(p, device="cuda")
+        xd = torch.as_tensor(rng.uniform(-1, 1, (n, 5)).astype(np.float32), device="cuda")
+        td = torch.as_tensor(rng.uniform(-0.1, 0.05, (n, 2)).astype(np.float32), device="cuda")
+        outs = []
+        assert lib.pstm_mlp_set_precision(1) == 0
+        try:
+            for fn in (lib.pstm_mlp_loss_grad_fused, lib.pstm_mlp_loss_grad):
+                g = torch.zeros(_abi.MLP_PARAMS, device="cuda")
+                loss = torch.zeros(1, device="cuda")
+                assert fn(pd.data_ptr(), xd.data_ptr(), td.data_ptr(), n, g.data_ptr(), loss.data_ptr(),
+                          _stream()) == 0
+                outs.append((g.cpu().numpy(), float(loss.item())))
+        finally:
+            lib.pstm_mlp_set_precision(0)
+        (gf, lf), (gs, ls) = outs
+        rel = np.linalg.norm(gf - gs) / np.linalg.norm(gs)
+        print(f"bf16-mode fused vs per-layer gradient rel-L2 {rel:.2e}, loss {lf:.6e} vs {ls:.6e}")
+        assert rel < 1e-2 and abs(lf - ls) <= 1e-5 * ls
+
+    def test_fused_training_step_is_adam_on_the_gradient(self):
+        """One iteration of pstm_mlp_train on a one-row set (every drawn
+        sample is row 0, so the minibatch gradient is that row's): the
+        parameters move by the Adam closed form with the schedule's first
+        learning rate."""
+        import torch
+
+        from pm_amd.surrogate import _stream
+
+        lib = _abi.load_library()
+        rng = np.random.default_rng(17)
+        p = _rand_params(17).astype(np.float32)
+        x = rng.uniform(-1, 1, (1, 5))
+        t = rng.uniform(-0.1, 0.05, (1, 2))
+        _, gr = omlp.loss_grad(p.astype(np.float64), x.astype(np.float32), t.astype(np.float32))
+        pd = torch.as_tensor(p, device="cuda")
+        m = torch.zeros_like(pd)
+        v = torch.zeros_like(pd)
+        xd = torch.as_tensor(x.astype(np.float32), device="cuda")
+        td = torch.as_tensor(t.astype(np.float32), device="cuda")
+        log = torch.full((2,), float("nan"), device="cuda")
+        assert lib.pstm_mlp_train(pd.data_ptr(), m.data_ptr(), v.data_ptr(), xd.data_ptr(), td.data_ptr(), 1, 1,
+                                  256, 1e-3, 1e-5, 0.9, 0.999, 1e-8, 0, 1 << 30, 7, 1, log.data_ptr(),
+                                  _stream()) == 0
+        pr, _, _ = omlp.adam(p.astype(np.float64), gr, np.zeros_like(gr), np.zeros_like(gr), 1e-3, 0.9, 0.999,
+                             1e-8, 1)
+        got = pd.cpu().numpy()
+        moved = np.abs(gr) > 1e-6        # where |g| >> eps the first step is lr * sign(g)
+        np.testing.assert_allclose(got[moved], pr[moved], rtol=0, atol=2e-6)
+        assert np.isfinite(float(log[0].item()))
+
     def test_training_reaches_quarter_sample(self):
         """Training on the VTI model (the last quarter quantisation-aware)
         reaches held-out errors below 0.25 sample (dt 2 ms) under the fused
