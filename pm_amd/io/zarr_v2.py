@@ -7,9 +7,9 @@ float32, chunked by rows, with sample_rate_ms / start_time_ms attributes
 (shape, chunks, dtype, compressor, fill_value, order, filters), an optional
 ``.zattrs`` JSON document, and one file per chunk named ``"<i>.<j>"``
 holding the chunk's C-order bytes (full chunk shape, the edge chunk padded).
-This module reads and writes the uncompressed form (``compressor: null``).
-The reference's own files use Blosc/zstd compression, whose decoder is not
-part of this package: opening such an array raises NotImplementedError.
+This module writes the uncompressed form (``compressor: null``) and reads
+it and the Blosc-compressed form the reference writes (numcodecs Blosc,
+zstd, bit-shuffle: pm_amd.io.blosc).
 """
 
 from __future__ import annotations
@@ -32,6 +32,7 @@ class ZarrV2Array:
     chunk_rows: int
     dtype: np.dtype
     attrs: dict
+    compressor: dict | None = None   # numcodecs-style {"id": "blosc", ...}, or None
 
     @classmethod
     def open(cls, path) -> "ZarrV2Array":
@@ -42,10 +43,11 @@ class ZarrV2Array:
         meta = json.loads(meta_path.read_text())
         if meta.get("zarr_format") != 2:
             raise ValueError(f"{path}: zarr_format {meta.get('zarr_format')} (expected 2)")
-        if meta.get("compressor") is not None or meta.get("filters"):
-            raise NotImplementedError(
-                f"{path}: compressed or filtered chunks ({meta.get('compressor')}); only uncompressed "
-                "Zarr v2 arrays are read here")
+        comp = meta.get("compressor")
+        if meta.get("filters"):
+            raise NotImplementedError(f"{path}: filters {meta.get('filters')} are not supported")
+        if comp is not None and comp.get("id") != "blosc":
+            raise NotImplementedError(f"{path}: compressor {comp.get('id')!r}; uncompressed or blosc chunks only")
         if meta.get("order", "C") != "C":
             raise NotImplementedError(f"{path}: order {meta.get('order')} (expected C)")
         dt = meta.get("dtype")
@@ -61,7 +63,7 @@ class ZarrV2Array:
             raise NotImplementedError(f"{path}: dimension_separator {meta['dimension_separator']!r}")
         attrs_path = path / ".zattrs"
         attrs = json.loads(attrs_path.read_text()) if attrs_path.exists() else {}
-        return cls(path, shape, chunks[0], np.dtype(_DTYPES[dt]), attrs)
+        return cls(path, shape, chunks[0], np.dtype(_DTYPES[dt]), attrs, comp)
 
     @property
     def n_traces(self) -> int:
@@ -95,6 +97,15 @@ class ZarrV2Array:
         f = self.chunk_file(i)
         if not f.exists():
             flat[:] = 0
+            return r1 - r0
+        if self.compressor is not None:
+            from pm_amd.io import blosc
+
+            raw = blosc.decompress(f.read_bytes())
+            full = self.chunk_rows * self.shape[1] * self.dtype.itemsize
+            if raw.nbytes != full:
+                raise IOError(f"{f}: decoded chunk holds {raw.nbytes} bytes, expected {full}")
+            flat.view(np.uint8)[:] = raw[:flat.nbytes]
             return r1 - r0
         view = memoryview(flat.view(np.uint8))
         with open(f, "rb", buffering=0) as fh:
