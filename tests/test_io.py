@@ -36,7 +36,8 @@ def test_missing_chunk_reads_fill_value(tmp_path):
 
 
 @pytest.mark.parametrize("change,exc", [
-    ({"compressor": {"id": "blosc", "cname": "zstd"}}, NotImplementedError),
+    ({"compressor": {"id": "lz4"}}, NotImplementedError),
+    ({"filters": [{"id": "delta"}]}, NotImplementedError),
     ({"chunks": [1000, 10]}, NotImplementedError),
     ({"dtype": "<i2"}, NotImplementedError),
     ({"order": "F"}, NotImplementedError),
