@@ -9,11 +9,14 @@ v_rms (1500-4500 m/s) and eta (0-0.2) fields on 128 x 128 x 201 grids, output
 spreading and obliquity weights, straight-ray VTI traveltime.  Every input is
 resident in HBM and the whole job is built (all traces, the full velocity and
 eta volumes, the full image and fold, every tile's trace list), but a step is
-one migration pass over a FIXED SUBSET of its output tiles (the tiles nearest
-the grid centre, ``--subset-tiles`` of edge ``--tile``): a pass over all
-512 x 512 pillars takes minutes, and the driver's 20 + 5 steps must fit in a
-few minutes.  C counts exactly the subset's contributions (sum of its 3-D
-fold), so the rate is that of the subset.
+one migration pass over a FIXED REGION of its output: the central
+``--region`` x ``--region`` pillars (32 x 32 by default), in the tiles of the
+edge the autotuner picks: a pass over all 512 x 512 pillars takes minutes,
+and the driver's 20 + 5 steps must fit in a few minutes.  C counts exactly
+the region's contributions (sum of its 3-D fold), so the rate is that of the
+region.  The autotuner times tile edges (output bins per tile) x launch
+shapes (samples per wave, traces per batch, pipelining) with the exact
+traveltime over twice the region and reports its table.
 
 The traveltime surrogate (5-64-64-64-64-2 SiLU MLP) is trained by the
 library's HIP kernels on 10M samples of the job's own v_rms / eta model
@@ -56,19 +59,14 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def subset_tiles(ex, n: int) -> list[int]:
-    """The n tiles whose centres are nearest the output grid's centre (ties by
-    tile index): a fixed, interior subset with full aperture coverage."""
-    import numpy as np
-
-    nx, ny, _ = ex.shape
-    cx, cy = (nx - 1) / 2.0, (ny - 1) / 2.0
-
-    def dist(i):
-        t = ex.tiles[i]
-        return (np.hypot((t.x0 + t.x1 - 1) / 2.0 - cx, (t.y0 + t.y1 - 1) / 2.0 - cy), i)
-
-    return sorted(range(len(ex.tiles)), key=dist)[:n]
+def central_region(nx: int, ny: int, region: int, align: int) -> tuple[int, int, int, int]:
+    """The region x region pillar block nearest the grid centre whose corner
+    sits on a multiple of ``align`` (so that tiles of every edge dividing
+    ``align`` cover it exactly): (x0, x1, y0, y1)."""
+    x0, y0 = (nx - region) // 2, (ny - region) // 2
+    x0 -= x0 % align
+    y0 -= y0 % align
+    return x0, x0 + region, y0, y0 + region
 
 
 def reset_tiles(ex, tiles) -> None:
@@ -228,15 +226,31 @@ def vti_headline(kernel, args) -> dict:
     lib = _abi.load_library()
     t0 = time.perf_counter()
     job = vti_diffractor_job(n_traces=args.vti_traces, n_samples=2001, n_out=args.vti_out, nt=1500,
-                             tile=args.tile, device="cuda")
+                             tile=args.region, device="cuda")
     torch.cuda.synchronize()
     log(f"vti job built in {time.perf_counter() - t0:.1f} s: {job.n_traces} traces x {job.n_samples}")
+    tune = None
+    if args.autotune:
+        # output bins per tile (tile edge) x launch shape, on the exact
+        # traveltime over a fixed region of 2 x region pillars per side
+        from pm_amd.autotune import autotune_tiles
+
+        edges = tuple(e for e in (16, 32, 64) if e <= 2 * args.region)
+        tune = autotune_tiles(kernel, job, edges=edges, region=2 * args.region,
+                              candidates=[(3, 16, 1), (2, 16, 1), (4, 8, 0)], max_edge=args.region)
+        log(f"autotune: edge {tune.edge}, mw {tune.mw}, batch {tune.batch}, pipe {tune.pipe}")
+    edge = tune.edge if tune else args.region
+    job.tile = edge
     t0 = time.perf_counter()
     ex = TileExecutor(kernel, job)
     torch.cuda.synchronize()
-    sub = subset_tiles(ex, args.subset_tiles)
-    log(f"executor ready in {time.perf_counter() - t0:.1f} s: {len(ex.tiles)} tiles; subset {sub} "
-        f"rows {[int(ex.tiles[i].rows.shape[0]) for i in sub]}")
+    from pm_amd.autotune import region_tiles
+
+    nx_, ny_, _ = ex.shape
+    rx0, rx1, ry0, ry1 = central_region(nx_, ny_, args.region, args.region)
+    sub = region_tiles(ex, rx0, rx1, ry0, ry1)
+    log(f"executor ready in {time.perf_counter() - t0:.1f} s: {len(ex.tiles)} tiles of {edge}; step region "
+        f"x{rx0}-{rx1 - 1} y{ry0}-{ry1 - 1}: tiles {sub} rows {[int(ex.tiles[i].rows.shape[0]) for i in sub]}")
     vc, ec = job.cube, job.meta["eta_cube"]
     d_max = job.config.max_aperture_m + 1500.0
     norm = Normalizer(tau_max=1.5, d_max=d_max, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
@@ -245,9 +259,12 @@ def vti_headline(kernel, args) -> dict:
     torch.cuda.synchronize()
     set_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    sur = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=args.train_steps, batch=args.train_batch,
-                                                          qat_fraction=0.25, log_every=max(args.train_steps // 8, 1),
-                                                          verbose=True)
+    sur = TraveltimeSurrogate.create(norm, seed=0)
+    sur.weight_terms = args.weight_terms
+    # bf16-rounded weights train through that rounding for the last quarter;
+    # split weights keep an f32-trained network
+    sur.train(tr, steps=args.train_steps, batch=args.train_batch, qat_fraction=0.25 if args.weight_terms == 1 else 0.0,
+              log_every=max(args.train_steps // 8, 1), verbose=True)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     held = sur.error_samples(tr, job.dt_ms, bf16=True) if args.train_samples <= 2_000_000 else None
@@ -346,19 +363,18 @@ def vti_headline(kernel, args) -> dict:
                             "k_migrate<VTI> (every subset tile fell back to the exact traveltime)")
     rl["launches_timed"] = int(nl.value)
     reset_tiles(ex, sub)
-    tiles_desc = ", ".join(f"x{ex.tiles[i].x0}-{ex.tiles[i].x1 - 1} y{ex.tiles[i].y0}-{ex.tiles[i].y1 - 1}"
-                           for i in sub)
+    tiles_desc = f"x{rx0}-{rx1 - 1} y{ry0}-{ry1 - 1} in {len(sub)} tile(s) of {edge}x{edge}"
     res = {
         "value": c_sub / (elapsed / max(args.steps, 1)) / 1e9, "ms_per_step": ms_step,
         "config": {
             "workload": (f"VTI diffractors (learned vs exact traveltime): {job.n_traces} traces x 2001 samples "
                          f"(2 ms), 300 diffractors, 3-D v_rms 1500-4500 m/s and eta 0-0.2 on 128x128x201 grids, "
-                         f"output {args.vti_out}x{args.vti_out} bins at 12.5 m x 1500 samples in tiles of "
-                         f"{args.tile}x{args.tile}, aperture 2500 m, dip 60, weights on; the whole job resident "
-                         f"in HBM, a step migrates the {len(sub)} tile(s) nearest the grid centre "
-                         f"({tiles_desc})"),
+                         f"output {args.vti_out}x{args.vti_out} bins at 12.5 m x 1500 samples, aperture 2500 m, "
+                         f"dip 60, weights on; the whole job resident in HBM, a step migrates the central "
+                         f"{args.region}x{args.region} pillars ({tiles_desc})"),
             "traces": job.n_traces, "samples": job.n_samples, "output": list(ex.shape),
-            "tiles_total": len(ex.tiles), "tiles_per_step": len(sub), "tile": args.tile,
+            "tiles_total": len(ex.tiles), "tiles_per_step": len(sub), "tile": edge,
+            "pillars_per_step": args.region * args.region,
             "contributions_per_step": c_sub, "effective_contributions_per_step_learned": e_learned,
             "traveltime": "learned VTI (MLP 5-64x4-2 SiLU, BF16 MFMA, FP32 accumulation) with per-tile exact "
                           "fallback", "parallelism": "1 GPU",
@@ -379,12 +395,15 @@ def vti_headline(kernel, args) -> dict:
                           "training_set_s": round(set_s, 2), "node_spacing": K,
                           "heldout_max_samples_per_tile": [round(e, 4) for e in errs],
                           "tiles_learned": len(learned), "tiles_exact_fallback": len(fallback),
-                          "training_set_error_samples": held, "forced": bool(args.force_learned)},
+                          "training_set_error_samples": held, "forced": bool(args.force_learned),
+                          "weight_terms": args.weight_terms},
         },
         "plain_straight_ray": dict(roofline_entry(roofline.FLOPS_PER_CONTRIBUTION["straight"], e_plain, plain_ms,
                                                   "k_migrate (straight ray, weights, linear) on the same subset"),
                                    gcontrib_s=round(c_sub / (plain_ms / 1e3) / 1e9, 2), launch=plain_info),
     }
+    if tune is not None:
+        res["autotune"] = tune.as_dict()
     res["_ex"] = ex
     res["_sub"] = sub
     return res
@@ -662,14 +681,18 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--tile", type=int, default=32, help="output tile edge of the VTI job (bins)")
-    ap.add_argument("--subset-tiles", type=int, default=1,
-                    help="tiles per step: the ones nearest the output grid's centre")
+    ap.add_argument("--region", type=int, default=32,
+                    help="pillars per side of the central output region a step migrates")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the tile-edge x launch-shape autotuner (built-in shape, tile edge = region)")
     ap.add_argument("--vti-traces", type=int, default=500_000)
     ap.add_argument("--vti-out", type=int, default=512)
     ap.add_argument("--train-samples", type=int, default=10_000_000)
-    ap.add_argument("--train-steps", type=int, default=80_000)
-    ap.add_argument("--train-batch", type=int, default=16_384)
+    ap.add_argument("--train-steps", type=int, default=100_000)
+    ap.add_argument("--train-batch", type=int, default=65_536)
+    ap.add_argument("--weight-terms", type=int, default=2, choices=[1, 2],
+                    help="fused inference weights: 1 rounded to bf16 (quantisation-aware training), 2 as two bf16 "
+                         "terms (f32 training)")
     ap.add_argument("--node-spacing", type=int, default=16)
     ap.add_argument("--force-learned", action="store_true",
                     help="use the learned traveltime on every subset tile whatever its held-out error "
@@ -727,6 +750,8 @@ def main() -> None:
         "learned_vs_exact": head["learned_vs_exact"],
         "plain_straight_ray": {"vti_subset": head["plain_straight_ray"], "point_diffractor": plain_pd},
     }
+    if "autotune" in head:
+        out["autotune"] = head["autotune"]
     if cpu:
         out["gpu_over_cpu"] = {"learned": round(head["value"] / cpu["value"], 1),
                                "exact": round(head["learned_vs_exact"]["exact_gcontrib_s"] / cpu["value"], 1)}

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PSTM_HIP_ABI_VERSION 3
+#define PSTM_HIP_ABI_VERSION 4
 
 /* traveltime: KernelConfig.kernel_type (pstm/kernels/base.py:220-229) */
 #define PSTM_TT_STRAIGHT 0 /* DSR, numba_cpu_optimized.py:189-194            */
@@ -38,6 +38,16 @@ extern "C" {
  * MFMA, f32 accumulation); no counterpart in the reference.  See the mlp_*
  * fields below. */
 #define PSTM_TT_VTI_LEARNED 3
+/* Curved ray (V(z) = V0 + kz) with the learned traveltime surrogate: the same
+ * fused network and t0 lattice, inputs (tau, s_s, s_r, vg, k) with z the
+ * depth of t0 / 2, vg = sqrt(v0 v(z)) and s = k h / sqrt(vg^2 + k^2 h^2);
+ * outputs out_scale * c per leg, leg time z / sqrt(vg^2 + k^2 h^2) (1 + c)
+ * (the reference's log formula equals z / sqrt(xi^2 + k^2 h^2) for some xi
+ * between v0 and v(z), so c stays small), trained on the legs of
+ * pstm/algorithm/curved_ray.py:19-52, 239-262.  Needs |k| >= 1e-6.  No
+ * counterpart in the reference.  mlp_v_lo / mlp_v_hi scale vg, mlp_eta_max
+ * scales k. */
+#define PSTM_TT_CURVED_LEARNED 4
 
 /* anti-aliasing (the reference's registered kernel has none) */
 #define PSTM_AA_NONE 0
@@ -119,16 +129,18 @@ typedef struct PstmMigrateArgs {
     double curved_v0;             /* PSTM_TT_CURVED: V(z) = v0 + k z      */
     double curved_k;
     double eta_constant;          /* PSTM_ETA_CONST                       */
-    /* --- learned VTI traveltime (PSTM_TT_VTI_LEARNED only) ---
+    /* --- learned traveltime (PSTM_TT_VTI_LEARNED, PSTM_TT_CURVED_LEARNED) ---
      * Per (pillar, trace) pair the network is evaluated on a lattice of output
-     * samples k = 0, K, 2K, ... (K = mlp_node_spacing) with inputs
+     * samples k = 0, K, 2K, ... (K = mlp_node_spacing; only where some sample
+     * that reads the node is inside the pair's aperture) with inputs
      *   x = (2 tau / tau_max - 1, 2 s_s - 1, 2 s_r - 1,
      *        2 (v - v_lo) / (v_hi - v_lo) - 1, 2 eta / eta_max - 1),
      * tau = t0 / 2, v and eta at (pillar, t0), s = sine of the straight-ray
      * angle of a leg, d / (v sqrt(tau^2 + d^2 / v^2)).  Outputs are
      * out_scale * c for each leg; c is interpolated linearly between lattice
      * samples and the leg time is sqrt(tau^2 + (d / v)^2 (1 + c)), with t = t0
-     * for t0 <= 0 as in the exact VTI time.                                 */
+     * for t0 <= 0 as in the exact VTI time; lattice samples past the axis
+     * extrapolate tau linearly.                                             */
     const float* mlp_params;      /* PSTM_MLP_PARAMS f32 (network layout below);
                                      weights are rounded to bf16 on use       */
     int32_t mlp_node_spacing;     /* K: 8, 16, 32 or 64 output samples        */
@@ -138,6 +150,12 @@ typedef struct PstmMigrateArgs {
     double mlp_v_hi;
     double mlp_eta_max;
     double mlp_out_scale;
+    int32_t mlp_weight_terms;     /* 0 or 1: weights rounded to bf16 (a network
+                                     trained through that rounding);
+                                     2: each weight as two bf16 terms, high +
+                                     remainder (about 16 bits, an f32-trained
+                                     network): one more MFMA per product      */
+    int32_t reserved0;            /* 0                                        */
 } PstmMigrateArgs;
 
 /* ABI identity, for the loader's layout check. */
@@ -358,6 +376,11 @@ int pstm_mlp_forward(const float* params, const float* x, int n, float* y, void*
  * Uses the migration library's workspace: do not overlap with a migration
  * call on another stream. */
 int pstm_mlp_forward_mfma(const float* params, const float* x, int n, float* y, void* stream);
+
+/* The same with weight_terms = 1 (pstm_mlp_forward_mfma) or 2 (each weight
+ * as two bf16 terms, as mlp_weight_terms = 2 in the gather). */
+int pstm_mlp_forward_mfma_terms(const float* params, const float* x, int n, float* y, int weight_terms,
+                                void* stream);
 
 /* Mean squared error of network(x) against target (n, 2) into *loss (device
  * f32) and its gradient w.r.t. every parameter into grads. */

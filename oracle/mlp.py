@@ -98,3 +98,26 @@ def forward_bf16(p, x):
         z = a @ bf16_round(W).T + b
         a = bf16_pair(silu(z)) if i < len(ls) - 1 else z
     return a
+
+
+def forward_bf16_split(p, x):
+    """Forward pass with the split-weight numerics of the fused inference
+    (mlp_weight_terms = 2): each weight as a high bfloat16 term plus the
+    bfloat16 of its remainder, the input and each hidden activation carried
+    as a bfloat16 pair, and the product of the two remainders left out (the
+    MFMA sequence w_hi x_hi + w_hi x_lo + w_lo x_hi); biases in f32."""
+    ls = unpack(np.asarray(p, np.float32).astype(np.float64))
+
+    def split(a):
+        a32 = np.asarray(a, dtype=np.float32)
+        hi = bf16_round(a32)
+        return hi, bf16_round((a32.astype(np.float64) - hi).astype(np.float32))
+
+    xh, xl = split(x)
+    for i, (W, b) in enumerate(ls):
+        wh, wl = split(W)
+        z = (xh + xl) @ wh.T + xh @ wl.T + b
+        if i < len(ls) - 1:
+            xh, xl = split(silu(z))
+        else:
+            return z

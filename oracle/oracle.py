@@ -48,6 +48,8 @@ def _load(fast: bool = False) -> ctypes.CDLL:
     lib.oracle_vti_leg.argtypes = [ctypes.c_double] * 4
     lib.oracle_interp_sample.restype = ctypes.c_double
     lib.oracle_interp_sample.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.oracle_fold_tile.restype = ctypes.c_int
+    lib.oracle_fold_tile.argtypes = [ctypes.POINTER(PstmMigrateArgs), ctypes.c_int]
     lib.oracle_triangle_at.restype = ctypes.c_double
     lib.oracle_triangle_at.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     lib.oracle_integrate_traces.restype = None
@@ -171,6 +173,28 @@ def migrate(inp: MigrateInputs, image_init: np.ndarray | None = None,
     if rc != 0:
         raise ValueError(f"oracle_migrate_tile failed ({rc})")
     return OracleResult(image=image, fold2d=fold, n_effective=int(counter[0]))
+
+
+def fold(x_axis, y_axis, midpoint_x, midpoint_y, max_aperture_m: float, trace_index=None,
+         threads: int | None = None) -> np.ndarray:
+    """(nx, ny) i32 pillar fold of the given midpoints, the oracle's f64 test
+    (strict IEEE build: the decision at the aperture edge must be the
+    oracle's)."""
+    lib = _load(False)
+    keep: list = []
+    a = PstmMigrateArgs()
+    a.x_axis = _p(keep, x_axis, np.float64)
+    a.y_axis = _p(keep, y_axis, np.float64)
+    a.midpoint_x = _p(keep, midpoint_x, np.float64)
+    a.midpoint_y = _p(keep, midpoint_y, np.float64)
+    a.trace_index = _p(keep, trace_index, np.int32)
+    a.n_traces = len(trace_index) if trace_index is not None else len(midpoint_x)
+    a.nx, a.ny = len(x_axis), len(y_axis)
+    a.max_aperture_m = float(max_aperture_m)
+    out = np.zeros((a.nx, a.ny), np.int32)
+    a.fold = out.ctypes.data
+    lib.oracle_fold_tile(ctypes.byref(a), int(threads or os.cpu_count() or 1))
+    return out
 
 
 def curved_dsr(t0_s, h_s, h_r, v0, k) -> float:

@@ -364,6 +364,29 @@ int oracle_migrate_tile(const PstmMigrateArgs* a, double* image64, int num_threa
     return 0;
 }
 
+// Fold only: the per-pillar count of traces with midpoint distance <= max
+// aperture, added to a->fold (numba_cpu_optimized.py:154,167-175,231), the
+// same f64 test as oracle_migrate_tile's; for checking full-size jobs whose
+// whole image is too costly to recompute on the CPU.
+int oracle_fold_tile(const PstmMigrateArgs* a, int num_threads) {
+    const int nx = a->nx, ny = a->ny;
+    const int ntr = a->n_traces;
+    const double max_ap = a->max_aperture_m;
+    const int nthr = num_threads > 0 ? num_threads : 1;
+#pragma omp parallel for schedule(static) num_threads(nthr)
+    for (long idx = 0; idx < (long)nx * ny; ++idx) {
+        const double ox = a->x_axis[idx / ny], oy = a->y_axis[idx % ny];
+        int lf = 0;
+        for (int jj = 0; jj < ntr; ++jj) {
+            const int j = a->trace_index ? a->trace_index[jj] : jj;
+            const double dxm = ox - a->midpoint_x[j], dym = oy - a->midpoint_y[j];
+            if (!(std::sqrt(dxm * dxm + dym * dym) > max_ap)) ++lf;
+        }
+        a->fold[idx] += lf;
+    }
+    return 0;
+}
+
 // Double running sums of `n_traces` traces (rows of amp): p (n_traces, ns+1)
 // = P[0..ns] and s_end (n_traces,) = S[ns-1], the convention of
 // integrate_trace above.

@@ -16,10 +16,10 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 REPO = PKG.parent
-# pstm_hip.hip is the main translation unit; gather_part1..6 compile its gather
+# pstm_hip.hip is the main translation unit; gather_part1..7 compile its gather
 # kernel groups separately (the slowest part of the build) so they run in parallel
 SOURCES = [PKG / "csrc" / "pstm_hip.hip", PKG / "csrc" / "pstm_mlp.hip", PKG / "csrc" / "pstm_output.hip",
-           *(PKG / "csrc" / f"gather_part{i}.hip" for i in range(1, 7))]
+           *(PKG / "csrc" / f"gather_part{i}.hip" for i in range(1, 8))]
 OUT = PKG / "lib" / "libpstm_hip.so"
 
 HIPCC_FLAGS = [

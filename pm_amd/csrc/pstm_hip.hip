@@ -54,6 +54,7 @@ int pstm_gather_part3(int mw, bool count, bool lob, int interp, const void* kp, 
 int pstm_gather_part4(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
 int pstm_gather_part5(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
 int pstm_gather_part6(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
+int pstm_gather_part7(int mw, bool count, bool lob, int interp, const void* kp, int grid, size_t lds, hipStream_t st);
 
 #ifndef PSTM_GATHER_PART
 static __device__ __align__(256) unsigned char scratch[(size_t)64 << 20];
@@ -93,6 +94,9 @@ constexpr int kMaxBatch = 16;          // traces per LDS batch (upper bound)
 constexpr int kMaxHalfWidth = 64;      // largest anti-aliasing half width (aa_max_half_width)
 constexpr int kPF = 12;                // dwords of the next batch's rows each thread holds in flight
 constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS uses the rest
+
+// traveltimes read from the fused network's t0 lattice
+constexpr bool learned_tt(int tt) { return tt == PSTM_TT_VTI_LEARNED || tt == PSTM_TT_CURVED_LEARNED; }
 
 // per-(pillar, sample) terms, f32, built once per call in f64
 //   x : traveltime slowness^2 s = 1/v^2       (curved ray: v(z) at depth)
@@ -166,10 +170,13 @@ struct KParams {
     float eps_half;              // relative band around fraction 0.5 (nearest / sinc)
     double tan_dip, min_ap64, max_ap64, t_start_s, dt_s, cv0, ck, eta_const;
     // learned VTI traveltime: network operand images and the t0 lattice
-    const void* mlpA;            // bf16 weight operands (k_prep_mlp)
+    const void* mlpA;            // bf16 weight operands (k_prep_mlp): high terms, then remainders
+    int mlp_split;               // 1: weights as two bf16 terms (mlp_weight_terms = 2)
     const void* mlpB;            // f32 biases in accumulator layout
     int lat_off;                 // byte offset of the lattice in LDS
     int lat_nodes;               // lattice samples per (pair, window): 256 MW / K + 1
+    int lat_cimin_off;           // byte offset of the per-(pillar, node) widest-aperture table
+    int lat_list_off;            // byte offset of the list of lattice rows to evaluate
     int lat_shift;               // log2 K
     float lat_ifrac;             // 1 / K
     float m_tau_sc, m_v_lo, m_v_sc, m_eta_sc, m_inv_out;
@@ -227,13 +234,17 @@ __global__ void k_prep_time_terms(PstmMigrateArgs a, int tt_eff, int npv, double
     if (apv < a.min_aperture_m) apv = a.min_aperture_m;
     else if (apv > a.max_aperture_m) apv = a.max_aperture_m;
     double s, wv, q;
-    if (a.traveltime == PSTM_TT_CURVED) {
+    if (a.traveltime == PSTM_TT_CURVED || a.traveltime == PSTM_TT_CURVED_LEARNED) {
         const double v0 = a.curved_v0, kg = a.curved_k;
         const double tau = t_s / 2.0;
         const double z = fabs(kg) < 1e-10 ? v0 * tau : (v0 / kg) * (exp(kg * tau) - 1.0);
         const double vz = v0 + kg * z;
         wv = vz;
-        if (tt_eff == PSTM_TT_STRAIGHT) {  // k ~ 0: straight ray in v0
+        if (a.traveltime == PSTM_TT_CURVED_LEARNED) {
+            // learned legs: t = z / sqrt(vg^2 + k^2 h^2) (1 + c), vg = sqrt(v0 v(z))
+            s = z;
+            q = v0 * vz;  // vg^2
+        } else if (tt_eff == PSTM_TT_STRAIGHT) {  // k ~ 0: straight ray in v0
             s = 1.0 / (v0 * v0);
             q = tau * tau;
         } else {
@@ -440,18 +451,20 @@ __device__ __forceinline__ bool exact_decision(const KParams& P, const TileView&
 // ---------------------------------------------------------------------------
 // fused traveltime surrogate: BF16 MFMA, f32 accumulation
 // ---------------------------------------------------------------------------
-// One wavefront evaluates the network for 32 rows with
-// v_mfma_f32_32x32x8_bf16 in the transposed form D = W . X^T: A = weights
-// (32 outputs x 8 inputs), B = activations (8 inputs x 32 rows), D = 32
-// outputs x 32 rows.  Lane l holds row l % 32; its accumulator register r
-// holds output 8 (r / 4) + 4 (l / 32) + r % 4 of the M tile, which is exactly
-// the B operand layout (inputs 8 ks + 4 (l / 32) + 0..3) of K step
-// ks = 4 mt + r / 4 of the next layer: activations never leave registers.
-// The input and every hidden activation enter as a bf16 pair hi + lo (two
-// products per K step), the weights as bf16.
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+// One wavefront evaluates the network for 32 rows with gfx950's
+// v_mfma_f32_32x32x16_bf16 in the transposed form D = W . X^T: A = weights
+// (32 outputs x 16 inputs), B = activations (16 inputs x 32 rows), D = 32
+// outputs x 32 rows.  Lane l holds row l % 32 and, of an operand, the 8
+// inputs 8 (l / 32) .. 8 (l / 32) + 7 of the K step; its accumulator register
+// r holds output 8 (r / 4) + 4 (l / 32) + r % 4 of the M tile.  The 8
+// registers 8 s .. 8 s + 7 of M tile mt are therefore taken as the B operand
+// of K step j = 2 mt + s of the next layer, and the next layer's weights are
+// laid out for that order of its inputs (mlp_unit): activations never leave
+// registers.  The input and every hidden activation enter as a bf16 pair
+// hi + lo (two products per K step); the weights as bf16, or as two bf16
+// terms (one more product).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
-typedef __bf16 bf16v4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kMlpLayers = 5;
 __host__ __device__ constexpr int mlp_in(int l) { return l == 0 ? PSTM_MLP_IN : PSTM_MLP_HIDDEN; }
@@ -462,12 +475,19 @@ __host__ __device__ constexpr int mlp_w_off(int l) {
     return off;
 }
 static_assert(mlp_w_off(kMlpLayers) == PSTM_MLP_PARAMS, "network layout");
-// weight operands, in bf16x4 units: layer 0 [mt 2][lane 64]; layers 1-3
-// [mt 2][ks 8][lane 64]; layer 4 (outputs padded to 32) [ks 8][lane 64]
-constexpr int kA1 = 2 * 64, kA4 = kA1 + 3 * 16 * 64, kANum = kA4 + 8 * 64;
+
+// weight operands, in bf16x8 units: layer 0 [mt 2][lane 64]; layers 1-3
+// [mt 2][j 4][lane 64]; layer 4 (outputs padded to 32) [j 4][lane 64]; the
+// remainders of split weights follow as a second image of kAVec8 vectors
+constexpr int kA1 = 2 * 64, kA4 = kA1 + 3 * 8 * 64, kAVec8 = kA4 + 4 * 64, kAVals = kAVec8 * 8;
 // biases in accumulator layout, in f32x16 units: layers 0-3 [mt 2][lane 64],
 // layer 4 [lane 64]
 constexpr int kB4 = 4 * 2 * 64, kBNum = kB4 + 64;
+
+// hidden unit carried by element e of lane half h of K step j (see above)
+__host__ __device__ constexpr int mlp_unit(int j, int h, int e) {
+    return 32 * (j / 2) + 16 * (j % 2) + (e < 4 ? 4 * h + e : 8 + 4 * h + (e - 4));
+}
 
 __host__ __device__ inline unsigned short bf16_bits(float f) {
     unsigned u = __builtin_bit_cast(unsigned, f);
@@ -478,22 +498,26 @@ __host__ __device__ inline unsigned short bf16_bits(float f) {
 #ifndef PSTM_GATHER_PART
 __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restrict__ A, float* __restrict__ B) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < kANum * 4) {
-        const int v = e / 4, el = e % 4;
-        int l, mt, ks, lane;
+    if (e < kAVals) {
+        const int v = e / 8, el = e % 8;
+        int l, mt, j, lane;
         if (v < kA1) {
-            l = 0; mt = v / 64; ks = 0; lane = v % 64;
+            l = 0; mt = v / 64; j = 0; lane = v % 64;
         } else if (v < kA4) {
             const int r = v - kA1;
-            l = 1 + r / (16 * 64); mt = (r / (8 * 64)) % 2; ks = (r / 64) % 8; lane = r % 64;
+            l = 1 + r / (8 * 64); mt = (r / (4 * 64)) % 2; j = (r / 64) % 4; lane = r % 64;
         } else {
-            l = 4; mt = 0; ks = (v - kA4) / 64; lane = (v - kA4) % 64;
+            l = 4; mt = 0; j = (v - kA4) / 64; lane = (v - kA4) % 64;
         }
-        const int o = mt * 32 + lane % 32, i = 8 * ks + 4 * (lane / 32) + el;
+        const int o = mt * 32 + lane % 32, h = lane / 32;
+        const int i = l == 0 ? 8 * h + el : mlp_unit(j, h, el);
         const float w = (o < mlp_out(l) && i < mlp_in(l)) ? P[mlp_w_off(l) + o * mlp_in(l) + i] : 0.0f;
-        A[e] = bf16_bits(w);
-    } else if (e < kANum * 4 + kBNum * 16) {
-        const int f = e - kANum * 4, v = f / 16, r = f % 16;
+        const unsigned short hi = bf16_bits(w);
+        A[e] = hi;
+        // the weight's remainder as a second bf16 term (split-weight inference)
+        A[kAVals + e] = bf16_bits(w - __builtin_bit_cast(float, (unsigned)hi << 16));
+    } else if (e < kAVals + kBNum * 16) {
+        const int f = e - kAVals, v = f / 16, r = f % 16;
         int l, mt, lane;
         if (v < kB4) {
             l = v / 128; mt = (v / 64) % 2; lane = v % 64;
@@ -506,64 +530,76 @@ __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restri
 }
 #endif  // PSTM_GATHER_PART
 
-__device__ __forceinline__ void split_bf16(float a0, float a1, float a2, float a3, bf16x4_t& hi, bf16x4_t& lo) {
-    bf16v4_t h;
-    h.x = (__bf16)a0; h.y = (__bf16)a1; h.z = (__bf16)a2; h.w = (__bf16)a3;
-    bf16v4_t r;
-    r.x = (__bf16)(a0 - (float)h.x); r.y = (__bf16)(a1 - (float)h.y);
-    r.z = (__bf16)(a2 - (float)h.z); r.w = (__bf16)(a3 - (float)h.w);
-    hi = __builtin_bit_cast(bf16x4_t, h);
-    lo = __builtin_bit_cast(bf16x4_t, r);
-}
-
 __device__ __forceinline__ float silu(float z) { return z * frcp(1.0f + __expf(-z)); }
 
-// SiLU of the two M tiles, re-packed as the next layer's B operands
-__device__ __forceinline__ void act_pack(const f32x16_t (&acc)[2], bf16x4_t (&h)[8], bf16x4_t (&l)[8]) {
+// 8 values as a bf16 pair: hi = bf16(a), lo = bf16(a - hi)
+__device__ __forceinline__ void split8(const float (&a)[8], bf16x8_t& hi, bf16x8_t& lo) {
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-            split_bf16(silu(acc[mt][4 * g]), silu(acc[mt][4 * g + 1]), silu(acc[mt][4 * g + 2]),
-                       silu(acc[mt][4 * g + 3]), h[4 * mt + g], l[4 * mt + g]);
+    for (int i = 0; i < 8; ++i) {
+        hi[i] = (__bf16)a[i];
+        lo[i] = (__bf16)(a[i] - (float)hi[i]);
+    }
 }
 
-__device__ __forceinline__ f32x16_t mfma2(bf16x4_t a, bf16x4_t bh, bf16x4_t bl, f32x16_t c) {
-    c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bh, c, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bl, c, 0, 0, 0);
+// SiLU of the two M tiles, re-packed as the next layer's four B operands
+__device__ __forceinline__ void act_pack(const f32x16_t (&acc)[2], bf16x8_t (&h)[4], bf16x8_t (&l)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float a[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = silu(acc[j / 2][8 * (j % 2) + i]);
+        split8(a, h[j], l[j]);
+    }
+}
+
+// one K step: W (x_hi + x_lo), plus w_lo x_hi for split weights (the w_lo
+// x_lo term, about 2^-16 of the product, is left out)
+template <bool SPLIT>
+__device__ __forceinline__ f32x16_t mfma_w(const bf16x8_t* __restrict__ WA, int idx, bf16x8_t bh, bf16x8_t bl,
+                                            f32x16_t c) {
+    const bf16x8_t a = WA[idx];
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl, c, 0, 0, 0);
+    if (SPLIT) c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(WA[kAVec8 + idx], bh, c, 0, 0, 0);
+    return c;
 }
 
 // Network outputs (y0, y1) of the row held by this lane (valid in lanes
 // 0-31; lanes l and l + 32 carry the same row).  Wave-uniform call, all
-// lanes active.
-__device__ __forceinline__ void mlp_eval32(const bf16x4_t* __restrict__ WA, const f32x16_t* __restrict__ WB,
+// lanes active.  SPLIT: each weight carried as two bf16 terms (WA holds the
+// high terms, WA + kAVec8 the remainders).
+template <bool SPLIT>
+__device__ __forceinline__ void mlp_eval32(const bf16x8_t* __restrict__ WA, const f32x16_t* __restrict__ WB,
                                            const float (&x)[5], float& y0, float& y1) {
     const int lane = threadIdx.x & 63;
     const bool upper = lane >= 32;
-    bf16x4_t h[8], l[8];
+    bf16x8_t h[4], l[4];
     {
-        bf16x4_t xh, xl;
-        split_bf16(upper ? x[4] : x[0], upper ? 0.0f : x[1], upper ? 0.0f : x[2], upper ? 0.0f : x[3], xh, xl);
+        // layer 0: the 5 inputs are elements 0-4 of the lower lane half
+        const float xin[8] = {upper ? 0.0f : x[0], upper ? 0.0f : x[1], upper ? 0.0f : x[2],
+                              upper ? 0.0f : x[3], upper ? 0.0f : x[4], 0.0f, 0.0f, 0.0f};
+        bf16x8_t xh, xl;
+        split8(xin, xh, xl);
         f32x16_t acc[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma2(WA[mt * 64 + lane], xh, xl, WB[mt * 64 + lane]);
+        for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma_w<SPLIT>(WA, mt * 64 + lane, xh, xl, WB[mt * 64 + lane]);
         act_pack(acc, h, l);
     }
 #pragma unroll 1
     for (int layer = 1; layer < 4; ++layer) {
-        const bf16x4_t* wa = WA + kA1 + (layer - 1) * 16 * 64;
+        const int wa = kA1 + (layer - 1) * 8 * 64;
         f32x16_t acc[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             acc[mt] = WB[(layer * 2 + mt) * 64 + lane];
 #pragma unroll
-            for (int ks = 0; ks < 8; ++ks) acc[mt] = mfma2(wa[(mt * 8 + ks) * 64 + lane], h[ks], l[ks], acc[mt]);
+            for (int j = 0; j < 4; ++j) acc[mt] = mfma_w<SPLIT>(WA, wa + (mt * 4 + j) * 64 + lane, h[j], l[j], acc[mt]);
         }
         act_pack(acc, h, l);
     }
     f32x16_t o = WB[kB4 + lane];
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) o = mfma2(WA[kA4 + ks * 64 + lane], h[ks], l[ks], o);
+    for (int j = 0; j < 4; ++j) o = mfma_w<SPLIT>(WA, kA4 + j * 64 + lane, h[j], l[j], o);
     y0 = o[0];
     y1 = o[1];
 }
@@ -571,7 +607,8 @@ __device__ __forceinline__ void mlp_eval32(const bf16x4_t* __restrict__ WA, cons
 #ifndef PSTM_GATHER_PART
 // Batched inference through the fused path's device code: one wavefront per
 // 32 rows of x (n, 5), outputs y (n, 2).
-__global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x4_t* __restrict__ WA,
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x8_t* __restrict__ WA,
                                                           const f32x16_t* __restrict__ WB,
                                                           const float* __restrict__ x, int n, float* __restrict__ y) {
     const int lane = threadIdx.x & 63;
@@ -583,7 +620,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x4_t* __rest
         for (int i = 0; i < 5; ++i) xi[i] = x[r * 5 + i];
     }
     float y0, y1;
-    mlp_eval32(WA, WB, xi, y0, y1);
+    mlp_eval32<SPLIT>(WA, WB, xi, y0, y1);
     if (lane < 32 && r < n) {
         y[2 * r] = y0;
         y[2 * r + 1] = y1;
@@ -620,6 +657,11 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         // k > 0: v(z) >= v0, so the exact time is >= 0 (and exactly 0 at t0 = 0,
         // where the f32 ratio only rounds to 1 +- ulp)
         t = fmaxf(t, P.t_floor);
+    } else if (TT == PSTM_TT_CURVED_LEARNED) {
+        // learned curved legs: t = z / sqrt(vg^2 + k^2 h^2) (1 + c) with the
+        // depth z (cs), vg^2 (q), k^2 h^2 (e.x, e.y) and c from the lattice
+        // (e2v, e1v); 1 + c >= 0 keeps t >= 0, and t = 0 at t0 = 0 (z = 0)
+        t = cs * fmaf(frsq(e.x + q), fmaxf(1.0f + e2v, 0.0f), frsq(e.y + q) * fmaxf(1.0f + e1v, 0.0f));
     } else if (TT == PSTM_TT_VTI_LEARNED) {
         // learned legs: t^2 = tau^2 + h^2 s (1 + c), c from the lattice
         // (e2v, e1v here); 1 + c >= 0 keeps t >= t0 as the exact VTI time is
@@ -919,6 +961,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     // batch after; the copy is written again only after the next barrier
     __shared__ int s_rownext[2][kMaxBatch];
     __shared__ float2 s_invsq[kMaxHalfWidth + 2];  // AA: {1 / l^2, 1 / (l + 1)^2}
+    __shared__ int s_nrows;  // learned traveltime: lattice rows listed for evaluation
     if (AA) {
         for (int i = tid; i < kMaxHalfWidth + 2; i += kThreads)
             s_invsq[i] = make_float2(i > 0 ? 1.0f / (float)(i * i) : 0.0f, 1.0f / (float)((i + 1) * (i + 1)));
@@ -990,19 +1033,34 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             const int kb = kbase + 64 * (tw + kTW * m);
             if (kb < P.nt && P.t0k[kb] <= 0.002f) early |= 1u << m;
         }
-        if (TT == PSTM_TT_CURVED) early = (1u << MW) - 1u;  // t may be < t0 on a curved ray
+        if (TT == PSTM_TT_CURVED || TT == PSTM_TT_CURVED_LEARNED) early = (1u << MW) - 1u;  // t may be < t0 on a curved ray
         early = __builtin_amdgcn_readfirstlane(early);
 
         const int pt_p = tid / P.batch, pt_j = tid % P.batch;
         const bool pt_thread = tid < kNPil * P.batch;
 
-        // learned traveltime: lattice sample below each slot and the lerp fraction
+        // learned traveltime: the lattice sample below each slot, the lerp
+        // fraction, and per (pillar, node) the smallest taper slope (widest
+        // aperture) over the samples that read the node, (j - 1) K .. (j + 1) K
+        // - 1 of the window: a pair farther than that aperture needs no network
+        // evaluation at the node
         int nidx[MW];
         float nfr = 0.0f;
-        if (TT == PSTM_TT_VTI_LEARNED) {
+        if (learned_tt(TT)) {
 #pragma unroll
             for (int m = 0; m < MW; ++m) nidx[m] = (64 * (tw + kTW * m) + lane) >> P.lat_shift;
             nfr = (float)(lane & ((1 << P.lat_shift) - 1)) * P.lat_ifrac;
+            float* cimin = reinterpret_cast<float*>(smem + P.lat_cimin_off);
+            const int nn = P.lat_nodes, K = 1 << P.lat_shift;
+            for (int e = tid; e < kNPil * nn; e += kThreads) {
+                const int p = e / nn, j = e - p * nn;
+                const int pil = pil0 + p;
+                const int k0 = max(kbase + (j - 1) * K, kbase), k1 = min(kbase + (j + 1) * K, min(kbase + win, P.nt));
+                float c = INFINITY;
+                if (pil < npil)
+                    for (int k = k0; k < k1; ++k) c = fminf(c, T.kc[(size_t)pil * T.kc_pstride + k].z);
+                cimin[e] = c;
+            }
         }
 
         // Pipelined staging (P.pipe): while a batch computes, each thread holds
@@ -1100,6 +1158,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             // (pillar, trace) pair table; the max-aperture test is the
             // oracle's f64 one (numba_cpu_optimized.py:167-173)
             int inap = 0;
+            if (learned_tt(TT) && tid == 0) s_nrows = 0;
             if (tid < P.batch) {  // wave 0, which builds pairs and issues no row loads
                 if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
                 if (P.pipe || early_dma) {
@@ -1134,6 +1193,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             const float den_r = P.v0 + sqrtf(P.v0 * P.v0 + a_r);
                             e = make_float4(a_s, a_r, dm, g1.z);
                             f = make_float4(1.0f / (den_s * den_r), 0.0f, 0.0f, 0.0f);
+                        } else if (TT == PSTM_TT_CURVED_LEARNED) {
+                            e = make_float4(P.ck2 * ds2, P.ck2 * dr2, dm, g1.z);
                         } else {
                             e = make_float4(ds2, dr2, dm, g1.z);
                             // AA: offsets pre-scaled by bin size / dt, so the operator
@@ -1157,45 +1218,81 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 continue;
             }
 
-            if (TT == PSTM_TT_VTI_LEARNED) {
-                // network on the t0 lattice of every in-aperture pair of the
-                // batch, 32 (pair, lattice sample) rows per wavefront step
-                const int nn = P.lat_nodes;
+            if (learned_tt(TT)) {
+                // network on the t0 lattice: the (pair, node) rows whose node is
+                // read by some slot inside the pair's time-dependent aperture
+                // are listed (in any order), the others set to 0; then 32 listed
+                // rows per wavefront step
+                const int nn = P.lat_nodes, K = 1 << P.lat_shift;
                 const int R = kNPil * P.batch * nn;
                 float2* lat = reinterpret_cast<float2*>(smem + P.lat_off);
-                const float dt_last = P.nt > 1 ? P.t0k[P.nt - 1] - P.t0k[P.nt - 2] : 0.0f;
-                for (int c0 = wave * 32; c0 < R; c0 += kWaves * 32) {
-                    const int r = c0 + (lane & 31);
+                const float* cimin = reinterpret_cast<const float*>(smem + P.lat_cimin_off);
+                unsigned short* rlist = reinterpret_cast<unsigned short*>(smem + P.lat_list_off);
+                for (int r0 = 0; r0 < R; r0 += kThreads) {
+                    const int r = r0 + tid;
                     bool valid = false;
-                    float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                     if (r < R) {
                         const int pp = r / nn, node = r - pp * nn;
                         const float4 e = pairs[pp * kPairVec];
-                        if (e.z != INFINITY) {
-                            valid = true;
-                            const int pil = pil0 + pp / P.batch;
-                            const int k = kbase + (node << P.lat_shift);
-                            const int kk = min(k, P.nt - 1);
-                            const float t0 = P.t0k[kk] + (float)(k - kk) * dt_last;
-                            const float tau = 0.5f * t0;
-                            const float q = tau * tau;
+                        valid = e.z != INFINITY &&
+                                fmaf(e.z, cimin[(pp / P.batch) * nn + node], P.taper_off) <= 1.0f + P.eps_x;
+                        if (!valid) lat[r] = make_float2(0.0f, 0.0f);
+                    }
+                    const unsigned long long bal = __ballot(valid);
+                    int base = 0;
+                    if (lane == 0 && bal) base = atomicAdd(&s_nrows, (int)__popcll(bal));
+                    base = __shfl(base, 0, 64);
+                    if (valid) rlist[base + (int)__popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)r;
+                }
+                __syncthreads();
+                const int nrows = s_nrows;
+                const float dt_last = P.nt > 1 ? P.t0k[P.nt - 1] - P.t0k[P.nt - 2] : 0.0f;
+                for (int c0 = wave * 32; c0 < nrows; c0 += kWaves * 32) {
+                    const int li = c0 + (lane & 31);
+                    const bool valid = li < nrows;
+                    const int r = valid ? (int)rlist[li] : 0;
+                    float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                    if (valid) {
+                        const int pp = r / nn, node = r - pp * nn;
+                        const float4 e = pairs[pp * kPairVec];
+                        const int pil = pil0 + pp / P.batch;
+                        // node at output sample k = kbase + node K; past the axis t0
+                        // is extrapolated linearly and v, eta taken at its end
+                        const int k = kbase + node * K;
+                        const int kk = min(k, P.nt - 1);
+                        const float t0 = P.t0k[kk] + (float)(k - kk) * dt_last;
+                        const float tau = 0.5f * t0;
+                        const float q = tau * tau;
+                        float eta, sn_s, sn_r, v;
+                        if (TT == PSTM_TT_VTI_LEARNED) {
                             const float sl = T.kc[(size_t)pil * T.kc_pstride + kk].x;  // 1/v^2
-                            const float eta = 0.5f * T.ek[(size_t)pil * T.ek_pstride + kk].x;
+                            eta = 0.5f * T.ek[(size_t)pil * T.ek_pstride + kk].x;
                             const float hs = e.x * sl, hr = e.y * sl;
                             const float ts2 = q + hs, tr2 = q + hr;
-                            const float sn_s = ts2 > 0.0f ? fsqrt(hs / ts2) : 0.0f;
-                            const float sn_r = tr2 > 0.0f ? fsqrt(hr / tr2) : 0.0f;
-                            const float v = frsq(sl);
-                            x[0] = fmaf(tau, P.m_tau_sc, -1.0f);
-                            x[1] = fmaf(2.0f, sn_s, -1.0f);
-                            x[2] = fmaf(2.0f, sn_r, -1.0f);
-                            x[3] = fmaf(v - P.m_v_lo, P.m_v_sc, -1.0f);
-                            x[4] = fmaf(eta, P.m_eta_sc, -1.0f);
+                            sn_s = ts2 > 0.0f ? fsqrt(hs / ts2) : 0.0f;
+                            sn_r = tr2 > 0.0f ? fsqrt(hr / tr2) : 0.0f;
+                            v = frsq(sl);
+                        } else {
+                            // curved: sines k h / sqrt(vg^2 + k^2 h^2), vg and the gradient k
+                            const float qg = P.qk[kk];  // vg^2
+                            sn_s = fsqrt(e.x / (e.x + qg));
+                            sn_r = fsqrt(e.y / (e.y + qg));
+                            v = fsqrt(qg);
+                            eta = (float)P.ck;
                         }
+                        x[0] = fmaf(tau, P.m_tau_sc, -1.0f);
+                        x[1] = fmaf(2.0f, sn_s, -1.0f);
+                        x[2] = fmaf(2.0f, sn_r, -1.0f);
+                        x[3] = fmaf(v - P.m_v_lo, P.m_v_sc, -1.0f);
+                        x[4] = fmaf(eta, P.m_eta_sc, -1.0f);
                     }
-                    if (!__any(valid)) continue;  // wave-uniform
                     float y0, y1;
-                    mlp_eval32(static_cast<const bf16x4_t*>(P.mlpA), static_cast<const f32x16_t*>(P.mlpB), x, y0, y1);
+                    if (P.mlp_split)
+                        mlp_eval32<true>(static_cast<const bf16x8_t*>(P.mlpA), static_cast<const f32x16_t*>(P.mlpB), x,
+                                         y0, y1);
+                    else
+                        mlp_eval32<false>(static_cast<const bf16x8_t*>(P.mlpA), static_cast<const f32x16_t*>(P.mlpB), x,
+                                          y0, y1);
                     if (lane < 32 && valid) lat[r] = make_float2(y0 * P.m_inv_out, y1 * P.m_inv_out);
                 }
             }
@@ -1292,7 +1389,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             any_in |= blk[m];
                         }
                         if (!any_in) continue;  // time-dependent aperture excludes every slot
-                        if (TT == PSTM_TT_VTI_LEARNED) {
+                        if (learned_tt(TT)) {
                             const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * P.lat_nodes;
     #pragma unroll
                             for (int m = 0; m < MW; ++m) {
@@ -1311,7 +1408,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                                                  y1, y2);
                             const float u = fmaf(t, P.idt, uoff_v);
                             // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
-                            const bool in_ap = xt[m] <= ((COUNT && TT != PSTM_TT_VTI_LEARNED) ? 1.0f - P.eps_x : 1.0f);
+                            const bool in_ap = xt[m] <= ((COUNT && !learned_tt(TT)) ? 1.0f - P.eps_x : 1.0f);
                             const bool below = u < P.umax_lo;
                             bool near = in_ap && !below && u < P.umax_hi;
                             if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
@@ -1322,10 +1419,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                 near = near || (in_ap && below &&
                                                  fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < fmaf(u, P.eps_half, 1e-5f));
                             // the learned time has no f64 counterpart to decide against
-                            if (TT == PSTM_TT_VTI_LEARNED) near = false;
+                            if (learned_tt(TT)) near = false;
                             any_near |= near;
                             if constexpr (kHalfJump<INTERP>) near_slots |= wave_any(near) ? 1u << (p * MW + m) : 0u;
-                            bool ok = in_ap && (TT == PSTM_TT_VTI_LEARNED ? u < P.umax : below) && !near;
+                            bool ok = in_ap && (learned_tt(TT) ? u < P.umax : below) && !near;
                             if (LOB) ok = ok && u >= 0.0f;
                             const float us = ok ? u : 0.0f;
                             const int i0 = (int)us;
@@ -1361,7 +1458,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     }
                     // slots near a threshold: recompute which, decide in f64 as the
                     // oracle does (rare; the slots were left out above)
-                    if (TT != PSTM_TT_VTI_LEARNED && any_near) {
+                    if (!learned_tt(TT) && any_near) {
     #pragma unroll
                         for (int p = 0; p < kPW; ++p) {
                             const int pi = (pg * kPW + p) * P.batch + jj;
@@ -1550,7 +1647,7 @@ int validate(const PstmMigrateArgs* a) {
                             !a->receiver_y || !a->midpoint_x || !a->midpoint_y))
         return fail(1, "trace pointers must be set");
     if (a->vel_mode != PSTM_VEL_1D && a->vel_mode != PSTM_VEL_3D) return fail(1, "invalid vel_mode");
-    if (a->traveltime < PSTM_TT_STRAIGHT || a->traveltime > PSTM_TT_VTI_LEARNED) return fail(1, "invalid traveltime");
+    if (a->traveltime < PSTM_TT_STRAIGHT || a->traveltime > PSTM_TT_CURVED_LEARNED) return fail(1, "invalid traveltime");
     if (a->aa_mode != PSTM_AA_NONE && a->aa_mode != PSTM_AA_TRIANGLE) return fail(1, "invalid aa_mode");
     if (a->aa_mode == PSTM_AA_TRIANGLE && a->traveltime != PSTM_TT_STRAIGHT)
         return fail(1, "triangle anti-aliasing is implemented for the straight-ray traveltime only");
@@ -1560,15 +1657,19 @@ int validate(const PstmMigrateArgs* a) {
         if (a->eta_mode < PSTM_ETA_CONST || a->eta_mode > PSTM_ETA_3D) return fail(1, "invalid eta_mode");
         if (a->eta_mode != PSTM_ETA_CONST && !a->eta) return fail(1, "eta pointer must be set");
     }
-    if (a->traveltime == PSTM_TT_VTI_LEARNED) {
+    if (a->traveltime == PSTM_TT_VTI_LEARNED || a->traveltime == PSTM_TT_CURVED_LEARNED) {
         const int K = a->mlp_node_spacing;
         if (!a->mlp_params) return fail(1, "mlp_params must be set for the learned traveltime");
         if (K != 8 && K != 16 && K != 32 && K != 64) return fail(1, "mlp_node_spacing must be 8, 16, 32 or 64");
+        if (a->mlp_weight_terms < 0 || a->mlp_weight_terms > 2) return fail(1, "mlp_weight_terms must be 0, 1 or 2");
         if (!(a->mlp_tau_max > 0.0) || !(a->mlp_v_hi > a->mlp_v_lo) || !(a->mlp_eta_max > 0.0) ||
             !(a->mlp_out_scale > 0.0))
             return fail(1, "learned traveltime encoding: need tau_max > 0, v_hi > v_lo, eta_max > 0, out_scale > 0");
     }
-    if (a->traveltime == PSTM_TT_CURVED && !(a->curved_v0 > 0.0))
+    if (a->traveltime == PSTM_TT_CURVED_LEARNED && !(fabs(a->curved_k) >= 1e-6))
+        return fail(1, "the learned curved-ray traveltime needs a velocity gradient |k| >= 1e-6 (k ~ 0 is the "
+                       "straight ray: use PSTM_TT_CURVED)");
+    if ((a->traveltime == PSTM_TT_CURVED || a->traveltime == PSTM_TT_CURVED_LEARNED) && !(a->curved_v0 > 0.0))
         return fail(1, "curved_v0 must be positive");
     if (a->interpolation < PSTM_INTERP_LINEAR || a->interpolation > PSTM_INTERP_LANCZOS5)
         return fail(1, "invalid interpolation");
@@ -1678,7 +1779,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     sh.tt = a.traveltime;
     if (sh.tt == PSTM_TT_CURVED && fabs(a.curved_k) < 1e-10) sh.tt = PSTM_TT_STRAIGHT;
     sh.aa = a.aa_mode == PSTM_AA_TRIANGLE;
-    sh.learned = sh.tt == PSTM_TT_VTI_LEARNED;
+    sh.learned = learned_tt(sh.tt);
     sh.vti_like = sh.tt == PSTM_TT_VTI || sh.learned;
     sh.lmax = sh.aa ? a.aa_max_half_width : 0;
     sh.lo = sh.aa ? sh.lmax + 1 : 0;
@@ -1712,7 +1813,10 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     auto lds_for = [&](int b) {
         return align_up((size_t)nbuf * align_up((size_t)b * sh.rowlen * sh.elem, 16) +
                             (size_t)kNPil * b * sh.pair_vec * 16, 16) +
-               sh.tot_bytes + align_up((size_t)kNPil * b * sh.lat_nodes * 8, 16);
+               sh.tot_bytes + align_up((size_t)kNPil * b * sh.lat_nodes * 8, 16) +
+               (sh.learned ? align_up((size_t)kNPil * sh.lat_nodes * 4, 16) +
+                                 align_up((size_t)kNPil * b * sh.lat_nodes * 2, 16)
+                           : 0);
     };
     int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
     sh.batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
@@ -1726,7 +1830,8 @@ Shape launch_shape(const PstmMigrateArgs& a) {
 }
 
 bool lo_check(const PstmMigrateArgs& a, const Shape& sh) {
-    return a.t_start_in_ms > 0.0 || (sh.tt == PSTM_TT_CURVED && a.curved_k < 0.0) || sh.vti_like;
+    return a.t_start_in_ms > 0.0 || (sh.tt == PSTM_TT_CURVED && a.curved_k < 0.0) || sh.vti_like ||
+           sh.tt == PSTM_TT_CURVED_LEARNED;
 }
 
 // Kernel parameters common to every tile (everything but the tile table).
@@ -1790,6 +1895,8 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.eta_const = a.eta_constant;
     kp.lat_off = (int)(kp.tot_off + sh.tot_bytes);
     kp.lat_nodes = sh.lat_nodes;
+    kp.lat_cimin_off = kp.lat_off + (int)align_up((size_t)kNPil * sh.batch * sh.lat_nodes * 8, 16);
+    kp.lat_list_off = kp.lat_cimin_off + (int)align_up((size_t)kNPil * sh.lat_nodes * 4, 16);
     kp.lat_shift = sh.learned ? __builtin_ctz((unsigned)a.mlp_node_spacing) : 0;
     kp.lat_ifrac = sh.learned ? 1.0f / (float)a.mlp_node_spacing : 0.0f;
     kp.m_tau_sc = sh.learned ? (float)(2.0 / a.mlp_tau_max) : 0.0f;
@@ -1799,6 +1906,7 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.m_inv_out = sh.learned ? (float)(1.0 / a.mlp_out_scale) : 0.0f;
     kp.mlpA = nullptr;
     kp.mlpB = nullptr;
+    kp.mlp_split = sh.learned && a.mlp_weight_terms == 2;
 }
 
 // Per-tile buffers: f32 tile-relative geometry (ntr x 32 B), per-(pillar,
@@ -1899,6 +2007,7 @@ int dispatch(const Shape& sh, bool count, bool lob, const KParams& kp, int grid,
     else if (sh.tt == PSTM_TT_STRAIGHT) part = sh.aa ? pstm_gather_part1 : pstm_gather_part2;
     else if (sh.tt == PSTM_TT_CURVED) part = pstm_gather_part3;
     else if (sh.tt == PSTM_TT_VTI) part = pstm_gather_part4;
+    else if (sh.tt == PSTM_TT_CURVED_LEARNED) part = pstm_gather_part7;
     else part = pstm_gather_part5;
     return part(sh.mw, count, lob, sh.interp, &kp, grid, sh.lds, st);
 }
@@ -1942,7 +2051,7 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     const size_t o_tile = off; off = align_up(off + tile_bytes(a, sh, ntr), 256);
     const size_t o_qk = off; off = align_up(off + (size_t)nt * 4, 256);
     const size_t o_t0 = off; off = align_up(off + (size_t)nt * 4, 256);
-    const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kANum * 8 : 0), 256);
+    const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kAVals * 4 : 0), 256);
     const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBNum * 64 : 0), 256);
     const size_t o_cnt = off; off = align_up(off + (size_t)grid * 8, 256);
     const size_t o_q = off; off = align_up(off + 8, 256);
@@ -1970,7 +2079,7 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     rc = prep_tile(a, sh, tan_dip, tb, qk, t0k, st);
     if (rc) return rc;
     if (sh.learned) {
-        const int n = kANum * 4 + kBNum * 16;
+        const int n = kAVals + kBNum * 16;
         hipLaunchKernelGGL(k_prep_mlp, dim3((n + 255) / 256), dim3(256), 0, st, a.mlp_params,
                            reinterpret_cast<unsigned short*>(ws + o_mlpA), reinterpret_cast<float*>(ws + o_mlpB));
         HIP_TRY(hipGetLastError());
@@ -2059,7 +2168,7 @@ int plan_enqueue(PstmPlan* pl, hipStream_t st) {
     const PstmMigrateArgs& a = pl->common;
     const Shape& sh = pl->sh;
     if (sh.learned) {
-        const int n = kANum * 4 + kBNum * 16;
+        const int n = kAVals + kBNum * 16;
         hipLaunchKernelGGL(k_prep_mlp, dim3((n + 255) / 256), dim3(256), 0, st, a.mlp_params,
                            static_cast<unsigned short*>(pl->mlpA), static_cast<float*>(pl->mlpB));
         HIP_TRY(hipGetLastError());
@@ -2201,7 +2310,7 @@ int pstm_hip_plan_create(const PstmMigrateArgs* common, const PstmTileDesc* tile
     pl->grid = (int)std::min<long>(n_items, (long)g_state.n_cu * sh.occupancy);
     const size_t o_cnt = off; off = align_up(off + (size_t)pl->grid * 8, 256);
     const size_t o_q = off; off = align_up(off + 8, 256);
-    const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kANum * 8 : 0), 256);
+    const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kAVals * 4 : 0), 256);
     const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBNum * 64 : 0), 256);
     const size_t o_prow = off; off = align_up(off + (sh.aa ? (size_t)pl->block_rows * sh.rowlen * 8 : 0), 256);
     hipError_t e = hipMalloc(&pl->mem, off);
@@ -2465,26 +2574,37 @@ int pstm_hip_fold_tile(const PstmMigrateArgs* args, void* stream) {
     return 0;
 }
 
-int pstm_mlp_forward_mfma(const float* params, const float* x, int n, float* y, void* stream) {
+int pstm_mlp_forward_mfma_terms(const float* params, const float* x, int n, float* y, int weight_terms,
+                                void* stream) {
     g_last_error.clear();
     if (!params || !x || !y || n < 0) return fail(1, "pstm_mlp_forward_mfma: bad arguments");
+    if (weight_terms != 1 && weight_terms != 2) return fail(1, "pstm_mlp_forward_mfma: weight_terms must be 1 or 2");
     if (n == 0) return 0;
     int rc = ensure_init();
     if (rc) return rc;
-    rc = ensure_workspace((size_t)kANum * 8 + (size_t)kBNum * 64);
+    rc = ensure_workspace((size_t)kAVals * 4 + (size_t)kBNum * 64);
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(g_state.ws);
-    const int np = kANum * 4 + kBNum * 16;
+    const int np = kAVals + kBNum * 16;
     hipLaunchKernelGGL(k_prep_mlp, dim3((np + 255) / 256), dim3(256), 0, st, params,
-                       reinterpret_cast<unsigned short*>(ws), reinterpret_cast<float*>(ws + (size_t)kANum * 8));
+                       reinterpret_cast<unsigned short*>(ws), reinterpret_cast<float*>(ws + (size_t)kAVals * 4));
     HIP_TRY(hipGetLastError());
     const long waves = ((long)n + 31) / 32;
-    hipLaunchKernelGGL(k_mlp_forward_mfma, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st,
-                       reinterpret_cast<const bf16x4_t*>(ws),
-                       reinterpret_cast<const f32x16_t*>(ws + (size_t)kANum * 8), x, n, y);
+    const auto* wa = reinterpret_cast<const bf16x8_t*>(ws);
+    const auto* wb = reinterpret_cast<const f32x16_t*>(ws + (size_t)kAVals * 4);
+    if (weight_terms == 2)
+        hipLaunchKernelGGL(k_mlp_forward_mfma<true>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, wa, wb, x,
+                           n, y);
+    else
+        hipLaunchKernelGGL(k_mlp_forward_mfma<false>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, wa, wb,
+                           x, n, y);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+int pstm_mlp_forward_mfma(const float* params, const float* x, int n, float* y, void* stream) {
+    return pstm_mlp_forward_mfma_terms(params, x, n, y, 1, stream);
 }
 
 int pstm_hip_migrate_bins(const PstmMigrateArgs* args, const int32_t* group_start, int32_t n_groups, void* stream) {
@@ -2529,6 +2649,9 @@ __attribute__((visibility("hidden"))) int PSTM_GATHER_CAT(pstm_gather_part, PSTM
 #elif PSTM_GATHER_PART == 5
     (void)interp;
     return dispatch_mw<PSTM_TT_VTI_LEARNED, false>(mw, count, lob, kp, grid, lds, st);
+#elif PSTM_GATHER_PART == 7
+    (void)interp;
+    return dispatch_mw<PSTM_TT_CURVED_LEARNED, false>(mw, count, lob, kp, grid, lds, st);
 #else
     (void)mw;
     (void)count;

@@ -13,9 +13,9 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
-TT_STRAIGHT, TT_CURVED, TT_VTI, TT_VTI_LEARNED = 0, 1, 2, 3
+TT_STRAIGHT, TT_CURVED, TT_VTI, TT_VTI_LEARNED, TT_CURVED_LEARNED = 0, 1, 2, 3, 4
 AA_NONE, AA_TRIANGLE = 0, 1
 VEL_1D, VEL_3D = 0, 1
 ETA_CONST, ETA_1D, ETA_3D = 0, 1, 2
@@ -71,6 +71,7 @@ class PstmMigrateArgs(ctypes.Structure):
         ("mlp_params", _vp), ("mlp_node_spacing", ctypes.c_int32), ("interpolation", ctypes.c_int32),
         ("mlp_tau_max", ctypes.c_double), ("mlp_v_lo", ctypes.c_double), ("mlp_v_hi", ctypes.c_double),
         ("mlp_eta_max", ctypes.c_double), ("mlp_out_scale", ctypes.c_double),
+        ("mlp_weight_terms", ctypes.c_int32), ("reserved0", ctypes.c_int32),
     ]
 
 
@@ -95,7 +96,7 @@ EXPORTED_SYMBOLS = (
     "pstm_hip_plan_create", "pstm_hip_plan_run", "pstm_hip_plan_info", "pstm_hip_plan_destroy",
     "pstm_hip_bin_headers", "pstm_hip_select_rows", "pstm_hip_fold_normalize", "pstm_hip_resample_uniform",
     "pstm_mlp_param_count", "pstm_mlp_last_error", "pstm_mlp_set_precision", "pstm_mlp_forward",
-    "pstm_mlp_forward_mfma", "pstm_mlp_loss_grad",
+    "pstm_mlp_forward_mfma", "pstm_mlp_forward_mfma_terms", "pstm_mlp_loss_grad",
     "pstm_mlp_adam", "pstm_mlp_train_step", "pstm_mlp_train", "pstm_mlp_loss_grad_fused", "pstm_vti_leg_times",
 )
 
@@ -173,6 +174,8 @@ def declare(lib: ctypes.CDLL) -> ctypes.CDLL:
     lib.pstm_mlp_forward.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
     lib.pstm_mlp_forward_mfma.restype = ctypes.c_int
     lib.pstm_mlp_forward_mfma.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
+    lib.pstm_mlp_forward_mfma_terms.restype = ctypes.c_int
+    lib.pstm_mlp_forward_mfma_terms.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]
     lib.pstm_mlp_loss_grad.restype = ctypes.c_int
     lib.pstm_mlp_loss_grad.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
     lib.pstm_mlp_adam.restype = ctypes.c_int

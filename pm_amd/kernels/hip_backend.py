@@ -411,7 +411,7 @@ class HipKernel:
         a.grid_dx, a.grid_dy = float(config.grid_dx), float(config.grid_dy)
         a.curved_v0, a.curved_k = float(config.curved_ray_v0), float(config.curved_ray_k)
         a.eta_constant = float(config.vti_eta_constant)
-        if tt == _abi.TT_VTI_LEARNED:
+        if tt in (_abi.TT_VTI_LEARNED, _abi.TT_CURVED_LEARNED):
             if surrogate is None:
                 raise ValueError("the learned traveltime needs a trained surrogate")
             n = surrogate.norm
@@ -420,6 +420,7 @@ class HipKernel:
             a.mlp_node_spacing = int(node_spacing)
             a.mlp_tau_max, a.mlp_v_lo, a.mlp_v_hi = float(n.tau_max), float(n.v_lo), float(n.v_hi)
             a.mlp_eta_max, a.mlp_out_scale = float(n.eta_max), float(n.out_scale)
+            a.mlp_weight_terms = int(getattr(surrogate, "weight_terms", 1))
         return a, keep
 
     # -- internals -----------------------------------------------------------

@@ -47,6 +47,13 @@ class DeviceTile:
         return (self.x1 - self.x0) * (self.y1 - self.y0)
 
 
+def learned_traveltime(surrogate) -> int:
+    """The PSTM_TT_* code of a trained surrogate's traveltime."""
+    from pm_amd.kernels import _abi
+
+    return _abi.TT_CURVED_LEARNED if getattr(surrogate, "kind", "vti") == "curved" else _abi.TT_VTI_LEARNED
+
+
 class TileExecutor:
     """Runs one migration job tile by tile on the GPU through HipKernel."""
 
@@ -145,7 +152,7 @@ class TileExecutor:
             t = self.tiles[i]
             tt = None
             if surrogate is not None and t.learned_ok:
-                tt = _abi.TT_VTI_LEARNED
+                tt = learned_traveltime(surrogate)
             e = self.kernel.migrate_device(self.traces, t.image, t.fold, t.x_axis, t.y_axis,
                                            self.t_axis, t.vrms, self.config, trace_index=t.rows,
                                            count_effective=count_effective, eta=t.eta, traveltime=tt,
@@ -186,7 +193,7 @@ class TileExecutor:
         traveltime.  Returns the per-tile maxima."""
         import torch
 
-        from pm_amd.surrogate import lattice_times, vti_leg_times
+        from pm_amd.surrogate import curved_leg_times, curved_velocity_at, lattice_times, vti_leg_times
 
         g = torch.Generator(device=self.device)
         g.manual_seed(seed)
@@ -207,17 +214,29 @@ class TileExecutor:
             pil, row, k, ox, oy = pil[keep][:n], row[keep][:n], k[keep][:n], ox[keep][:n], oy[keep][:n]
             ds = ((ox - tr.source_x[row]) ** 2 + (oy - tr.source_y[row]) ** 2).sqrt()
             dr = ((ox - tr.receiver_x[row]) ** 2 + (oy - tr.receiver_y[row]) ** 2).sqrt()
-            vflat = t.vrms.reshape(-1, self.nt) if t.vrms.dim() == 3 else None
-            eflat = t.eta.reshape(-1, self.nt) if t.eta is not None else None
-            v_at = (lambda kk: vflat[pil, kk]) if vflat is not None else (lambda kk: t.vrms[kk])
-            if eflat is not None:
-                e_at = lambda kk: eflat[pil, kk]  # noqa: E731
+            if getattr(surrogate, "kind", "vti") == "curved":
+                if float(surrogate.curved_v0) != float(self.config.curved_ray_v0):
+                    raise ValueError("the curved surrogate was trained for another surface velocity")
+                # v(z) at the image depth of each sample, the gradient k as the fifth input
+                v0, kg = float(self.config.curved_ray_v0), float(self.config.curved_ray_k)
+                vg_axis = (v0 * curved_velocity_at(tau_axis, v0, kg)).sqrt()
+                v_at = lambda kk: vg_axis[kk]  # noqa: E731
+                e_at = lambda kk: torch.full(kk.shape, kg, dtype=torch.float64, device=self.device)  # noqa: E731
+                learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing)
+                tau = tau_axis[k]
+                exact = curved_leg_times(tau, ds, v0, kg) + curved_leg_times(tau, dr, v0, kg)
             else:
-                e_at = lambda kk: torch.full(kk.shape, float(self.config.vti_eta_constant),  # noqa: E731
-                                             dtype=torch.float64, device=self.device)
-            learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing)
-            tau = tau_axis[k]
-            exact = vti_leg_times(tau, ds, v_at(k), e_at(k)) + vti_leg_times(tau, dr, v_at(k), e_at(k))
+                vflat = t.vrms.reshape(-1, self.nt) if t.vrms.dim() == 3 else None
+                eflat = t.eta.reshape(-1, self.nt) if t.eta is not None else None
+                v_at = (lambda kk: vflat[pil, kk]) if vflat is not None else (lambda kk: t.vrms[kk])
+                if eflat is not None:
+                    e_at = lambda kk: eflat[pil, kk]  # noqa: E731
+                else:
+                    e_at = lambda kk: torch.full(kk.shape, float(self.config.vti_eta_constant),  # noqa: E731
+                                                 dtype=torch.float64, device=self.device)
+                learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing)
+                tau = tau_axis[k]
+                exact = vti_leg_times(tau, ds, v_at(k), e_at(k)) + vti_leg_times(tau, dr, v_at(k), e_at(k))
             inside = exact / dt_in < ns - 1
             err = float(((learned - exact).abs() / dt_in)[inside].max()) if bool(inside.any()) else 0.0
             t.heldout_error = err

@@ -76,6 +76,20 @@ class Normalizer:
                 2.0 * eta / max(self.eta_max, 1e-6) - 1.0]
         return torch.stack(cols, 1).float().contiguous()
 
+    def encode_curved(self, tau, d_s, d_r, vg, k):
+        """Raw curved-ray inputs (tau, d_s, d_r, vg, k) -> (n, 5) f32 network
+        inputs; the sines are k d / sqrt(vg^2 + k^2 d^2)."""
+        import torch
+
+        def sine(d):
+            kd = k * d
+            return kd / (vg * vg + kd * kd).sqrt()
+
+        cols = [2.0 * tau / self.tau_max - 1.0, 2.0 * sine(d_s) - 1.0, 2.0 * sine(d_r) - 1.0,
+                2.0 * (vg - self.v_lo) / (self.v_hi - self.v_lo) - 1.0,
+                2.0 * k / max(self.eta_max, 1e-6) - 1.0]
+        return torch.stack(cols, 1).float().contiguous()
+
     def apply(self, x):
         """x (n, 5) raw (tau, d_s, d_r, v, eta) -> encoded f32."""
         return self.encode(x[:, 0], x[:, 1], x[:, 2], x[:, 3], x[:, 4])
@@ -172,6 +186,84 @@ def learned_leg_time(tau, d, v, c):
     return (tau * tau + (d / v) ** 2 * (1.0 + c).clamp_min(0.0)).sqrt()
 
 
+def curved_velocity_at(tau, v0: float, k: float):
+    """v(z) at the depth whose vertical one-way time is tau in
+    V(z) = v0 + k z: z = (v0 / k)(e^{k tau} - 1) (curved_ray.py:239-262),
+    v(z) = v0 + k z (f64 tensors)."""
+    import torch
+
+    tau = torch.as_tensor(tau, dtype=torch.float64)
+    if abs(k) < 1e-10:
+        return torch.full_like(tau, float(v0))
+    z = (v0 / k) * (torch.exp(k * tau) - 1.0)
+    return v0 + k * z
+
+
+def curved_leg_times(tau, h, v0: float, k: float):
+    """One-way curved-ray time of the reference (curved_ray.py:19-52, the
+    depth from tau by :239-262) as f64 tensor arithmetic: the exact targets
+    of the learned curved traveltime."""
+    import torch
+
+    tau = torch.as_tensor(tau, dtype=torch.float64)
+    h = torch.as_tensor(h, dtype=torch.float64)
+    if abs(k) < 1e-10:
+        z = v0 * tau
+        return (h * h + z * z).sqrt() / v0
+    vz = curved_velocity_at(tau, v0, k)
+    kx = k * h
+    t1 = vz + (vz * vz + kx * kx).sqrt()
+    t2 = v0 + (v0 * v0 + kx * kx).sqrt()
+    return torch.log(t1 / t2) / k
+
+
+def curved_depth_and_vg(tau, v0: float, k: float):
+    """Depth z of the vertical one-way time tau in V(z) = v0 + k z and the
+    geometric-mean velocity vg = sqrt(v0 v(z)) of the learned curved leg."""
+    import torch
+
+    vz = curved_velocity_at(tau, v0, k)
+    return (vz - v0) / k, (v0 * vz).sqrt() if isinstance(vz, torch.Tensor) else np.sqrt(v0 * vz)
+
+
+def learned_leg_time_curved(tau, d, c, v0: float, k: float):
+    """z / sqrt(vg^2 + k^2 d^2) max(1 + c, 0), as the gather evaluates the
+    learned curved leg.  The reference's leg (1/k) ln[(v(z) + sqrt(v(z)^2 +
+    k^2 d^2)) / (v0 + sqrt(v0^2 + k^2 d^2))] equals z / sqrt(xi^2 + k^2 d^2)
+    for some xi between v0 and v(z) (mean value theorem on asinh), so c is a
+    small correction of the geometric-mean choice xi = vg."""
+    z, vg = curved_depth_and_vg(tau, v0, k)
+    return z / (vg * vg + (k * d) ** 2).sqrt() * (1.0 + c).clamp_min(0.0)
+
+
+def make_training_set_curved(v0: float, k: float, n: int, d_max: float, norm: Normalizer, seed: int,
+                             t_max_ms: float, device="cuda") -> TrainingSet:
+    """n samples of the job's V(z) = v0 + k z: t0 uniform over [0, t_max],
+    d_s, d_r uniform over [0, d_max]; raw inputs (tau, d_s, d_r, vg, k),
+    encoded with the curved sines k d / sqrt(vg^2 + k^2 d^2); targets
+    out_scale * c with c = t / (z / sqrt(vg^2 + k^2 d^2)) - 1 per leg (0 at
+    t0 = 0, where both times are 0)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    tau = torch.as_tensor(rng.uniform(0.0, t_max_ms, n) / 2000.0, device=device)
+    dst = torch.as_tensor(rng.uniform(0.0, d_max, n), device=device)
+    drt = torch.as_tensor(rng.uniform(0.0, d_max, n), device=device)
+    z, vg = curved_depth_and_vg(tau, v0, k)
+    kk = torch.full_like(tau, float(k))
+    t_s = curved_leg_times(tau, dst, v0, k)
+    t_r = curved_leg_times(tau, drt, v0, k)
+
+    def factor(t, d):
+        tref = z / (vg * vg + (k * d) ** 2).sqrt()
+        return torch.where(tref > 0, t / tref.clamp_min(1e-300) - 1.0, torch.zeros_like(tref))
+
+    xr = torch.stack([tau, dst, drt, vg, kk], 1)
+    c = torch.stack([factor(t_s, dst), factor(t_r, drt)], 1)
+    return TrainingSet(xr, norm.encode_curved(tau, dst, drt, vg, kk), (c * norm.out_scale).float().contiguous(),
+                       torch.stack([t_s, t_r], 1))
+
+
 def _trilinear(cube, x, y, t):
     """Trilinear interpolation of a VelocityCube-like grid at scattered points
     (numpy, clamped), vectorised."""
@@ -207,6 +299,15 @@ class TraveltimeSurrogate:
     grads: object = None
     step: int = 0
     history: list = field(default_factory=list)
+    # numerics of the fused inference: 1 = weights rounded to bf16 (train with
+    # a quantisation-aware tail), 2 = each weight as two bf16 terms (an
+    # f32-trained network keeps about 16 bits)
+    weight_terms: int = 1
+    # traveltime the network learns: "vti" (inputs tau, s_s, s_r, v_rms,
+    # eta) or "curved" (tau, s_s, s_r, vg, k; PSTM_TT_CURVED_LEARNED) with
+    # the job's surface velocity curved_v0
+    kind: str = "vti"
+    curved_v0: float = 0.0
 
     @classmethod
     def create(cls, norm: Normalizer, seed: int = 0, device="cuda") -> "TraveltimeSurrogate":
@@ -229,6 +330,21 @@ class TraveltimeSurrogate:
                    "pstm_mlp_forward")
         finally:
             set_precision(False)
+        return y
+
+    def forward_fused(self, x):
+        """Network output (n, 2) through the migration kernel's own fused
+        inference code (BF16 MFMA, f32 accumulation, this surrogate's
+        weight_terms), for normalised inputs x (n, 5)."""
+        import torch
+
+        lib = _abi.load_library()
+        x = x.float().contiguous()
+        y = torch.empty((x.shape[0], 2), dtype=torch.float32, device=x.device)
+        rc = lib.pstm_mlp_forward_mfma_terms(_p(self.params), _p(x), int(x.shape[0]), _p(y), int(self.weight_terms),
+                                             _stream())
+        if rc != 0:
+            raise (ValueError if rc == 1 else RuntimeError)(f"pstm_mlp_forward_mfma_terms: {_abi.last_error(lib)}")
         return y
 
     def loss_grad(self, x, y):
@@ -314,11 +430,22 @@ class TraveltimeSurrogate:
         return self
 
     def leg_times(self, ts: TrainingSet, bf16: bool = False):
-        """Predicted (t_s, t_r) f64 for a sample set."""
+        """Predicted (t_s, t_r) f64 for a sample set: f32 numerics, or (bf16)
+        those of the fused inference."""
         import torch
 
-        c = self.forward(ts.x, bf16).double() / self.norm.out_scale
+        y = self.forward_fused(ts.x) if bf16 else self.forward(ts.x)
+        c = y.double() / self.norm.out_scale
         tau, ds, dr, v = ts.x_raw[:, 0], ts.x_raw[:, 1], ts.x_raw[:, 2], ts.x_raw[:, 3]
+        if self.kind == "curved":
+            # raw inputs (tau, d_s, d_r, vg, k): z = (vg^2 / v0 - v0) / k
+            kg = ts.x_raw[:, 4]
+            z = (v * v / self.curved_v0 - self.curved_v0) / kg
+
+            def leg(d, cc):
+                return z / (v * v + (kg * d) ** 2).sqrt() * (1.0 + cc).clamp_min(0.0)
+
+            return torch.stack([leg(ds, c[:, 0]), leg(dr, c[:, 1])], 1)
         return torch.stack([learned_leg_time(tau, ds, v, c[:, 0]), learned_leg_time(tau, dr, v, c[:, 1])], 1)
 
     def error_samples(self, ts: TrainingSet, dt_ms: float, bf16: bool = False) -> dict:
@@ -338,7 +465,8 @@ class TraveltimeSurrogate:
 
 def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, node_spacing: int):
     """Learned two-leg time of output samples k as the gather forms it: the
-    network (bf16 numerics) on the lattice samples k0 = K floor(k / K) and
+    network (the fused inference's own code) on the lattice samples
+    k0 = K floor(k / K) and
     k0 + K (times past the axis extrapolated from its last step, v and eta
     taken at the last sample), c interpolated linearly, then
     sum of sqrt(tau^2 + (d / v)^2 max(1 + c, 0)); t = t0 where t0 <= 0.
@@ -357,12 +485,20 @@ def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, n
     def at_node(kn):
         kk = kn.clamp(max=nt - 1)
         tau = tau_axis[kk] + (kn - kk).double() * dtau
-        x = sur.norm.encode(tau, ds, dr, v_at(kk), eta_at(kk))
-        return sur.forward(x, bf16=True).double() / sur.norm.out_scale
+        enc = sur.norm.encode_curved if sur.kind == "curved" else sur.norm.encode
+        x = enc(tau, ds, dr, v_at(kk), eta_at(kk))
+        return sur.forward_fused(x).double() / sur.norm.out_scale
 
     c0, c1 = at_node(k0), at_node(k0 + K)
     c = c0 + frac * (c1 - c0)
     tau = tau_axis[k]
     v = v_at(k)
+    if sur.kind == "curved":
+        # v_at is vg at the sample, eta_at the gradient k; t = 0 at t0 = 0
+        kg = float(eta_at(k[:1])[0])
+        vg = v
+        z = (vg * vg / float(sur.curved_v0) - float(sur.curved_v0)) / kg
+        leg = lambda d, cc: z / (vg * vg + (kg * d) ** 2).sqrt() * (1.0 + cc).clamp_min(0.0)  # noqa: E731
+        return leg(ds, c[:, 0]) + leg(dr, c[:, 1])
     t = learned_leg_time(tau, ds, v, c[:, 0]) + learned_leg_time(tau, dr, v, c[:, 1])
     return torch.where(tau > 0, t, 2.0 * tau)

@@ -303,3 +303,43 @@ def vti_diffractor_job(seed: int = 4, n_traces: int = 500_000, n_samples: int = 
                        2.0 * np.arange(nt), cfg, cube=vc, tile=tile,
                        meta={"eta_cube": ec, "diffractors": np.stack([xd, yd, t0d], 1)})
     return job
+
+
+def curved_diffractor_job(seed: int = 6, n_traces: int = 200_000, n_samples: int = 1501, n_out: int = 128,
+                          nt: int = 1000, d_out: float = 12.5, tile: int = 32, n_diffractors: int = 120,
+                          v0: float = 1600.0, k: float = 0.6, device: str = "cpu") -> SyntheticJob:
+    """Curved-ray job: random diffractors at the reference's curved-ray DSR
+    time (pstm/algorithm/curved_ray.py:287-316: one-way legs of
+    curved_ray_traveltime at the depth z(t0 / 2) of V(z) = v0 + k z), 30 Hz
+    Ricker; output n_out x n_out bins x nt samples; v_rms for the aperture is
+    the velocity at the image depth, v0 e^{k t0 / 2}."""
+    import torch
+
+    from pm_amd.surrogate import curved_leg_times
+
+    rng = np.random.default_rng(seed)
+    x_hi = d_out * (n_out - 1)
+    lo, hi = -500.0, x_hi + 500.0
+    sx, sy, rx, ry = _geometry(rng, n_traces, lo, hi, 3000.0)
+    dt_s = 0.002
+    tr = torch.zeros((n_traces, n_samples), dtype=torch.float64, device=device)
+    S = [torch.as_tensor(a, device=device) for a in (sx, sy, rx, ry)]
+    xd = rng.uniform(0.0, x_hi, n_diffractors)
+    yd = rng.uniform(0.0, x_hi, n_diffractors)
+    t0d = rng.uniform(0.2, 0.98 * nt * dt_s, n_diffractors)
+    ad = rng.uniform(0.5, 1.0, n_diffractors)
+    for i in range(n_diffractors):
+        tau = torch.full_like(S[0], t0d[i] / 2)
+        hs = ((S[0] - xd[i]) ** 2 + (S[1] - yd[i]) ** 2).sqrt()
+        hr = ((S[2] - xd[i]) ** 2 + (S[3] - yd[i]) ** 2).sqrt()
+        t = curved_leg_times(tau, hs, v0, k) + curved_leg_times(tau, hr, v0, k)
+        add_events(tr, t, torch.full_like(t, ad[i]), 30.0, dt_s)
+    x = d_out * np.arange(n_out)
+    t_ms = 2.0 * np.arange(nt)
+    cfg = KernelConfig(max_aperture_m=2500.0, min_aperture_m=500.0, max_dip_degrees=60.0,
+                       taper_fraction=0.1, apply_spreading=True, apply_obliquity=True,
+                       grid_dx=d_out, grid_dy=d_out, kernel_type="curved_ray", curved_ray_enabled=True,
+                       curved_ray_v0=v0, curved_ray_k=k)
+    return SyntheticJob("curved_diffractors", tr.float().contiguous(), sx, sy, rx, ry, 2.0, 0.0, x, x.copy(),
+                        t_ms, cfg, vrms_1d=v0 * np.exp(k * t_ms / 2000.0), tile=tile,
+                        meta={"diffractors": np.stack([xd, yd, t0d], 1), "v0": v0, "k": k})

@@ -29,18 +29,21 @@ def test_more_than_one_gpu_exits_with_a_message():
     assert p.stdout.strip() == ""
 
 
-def test_subset_is_the_central_tiles_and_roofline_arithmetic():
-    """The timed step's tile subset is fixed: the tiles nearest the output
-    grid's centre; the roofline entry is F x E over the launch time against
-    the FP32 VALU peak."""
+def test_region_is_central_and_tile_aligned_and_roofline_arithmetic():
+    """The timed step's region is fixed: the central pillars, aligned so that
+    tiles of every tuned edge cover it exactly; the roofline entry is F x E
+    over the launch time against the FP32 VALU peak."""
     from types import SimpleNamespace
 
+    from pm_amd.autotune import region_tiles
+
     mod = _bench_module()
-    tiles = [SimpleNamespace(x0=x, x1=x + 32, y0=y, y1=y + 32) for x in range(0, 512, 32) for y in range(0, 512, 32)]
-    ex = SimpleNamespace(shape=(512, 512, 1500), tiles=tiles)
-    four = mod.subset_tiles(ex, 4)
-    assert sorted((tiles[i].x0, tiles[i].y0) for i in four) == [(224, 224), (224, 256), (256, 224), (256, 256)]
-    assert mod.subset_tiles(ex, 1) == mod.subset_tiles(ex, 1)
+    x0, x1, y0, y1 = mod.central_region(512, 512, 32, 32)
+    assert (x0, x1, y0, y1) == (224, 256, 224, 256)
+    for edge, count in ((16, 4), (32, 1)):
+        tiles = [SimpleNamespace(x0=x, x1=x + edge, y0=y, y1=y + edge) for x in range(0, 512, edge)
+                 for y in range(0, 512, edge)]
+        assert len(region_tiles(SimpleNamespace(tiles=tiles), x0, x1, y0, y1)) == count
     rl = mod.roofline_entry(29, 10**12, 1000.0, "k")
     assert rl["achieved"] == pytest.approx(29.0) and rl["frac"] == pytest.approx(29.0 / 157.3, abs=1e-4)
     assert rl["bound"] == "valu" and rl["unit"] == "TFLOP/s"

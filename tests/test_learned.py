@@ -6,10 +6,11 @@
   bf16 NumPy/torch restatement of the lattice inference
   (pm_amd.surrogate.lattice_times over oracle-pinned bf16 numerics), and the
   exact VTI path with the f64 VTI time.
-* Job level: on a small VTI diffractor job the network trained on the job's
-  model passes the per-tile held-out check, the learned image is compared with
-  the exact-traveltime image, and with the image the method gives without the
-  network (isotropic straight-ray times in the same v_rms).
+* Job level: on a small VTI diffractor job the network trained (f32) on the
+  job's model and evaluated with split bf16 weights passes the per-tile
+  held-out check, and the learned image is within relative L2 1e-3 of the
+  exact-traveltime image, which the image the method gives without the
+  network (isotropic straight-ray times in the same v_rms) misses by far.
 * Fallback: a tile that fails the held-out check migrates with the exact
   traveltime, bit for bit.
 """
@@ -119,16 +120,18 @@ def test_exact_vti_readback(kernel):
     _compare(_readback(kernel, c, _abi.TT_VTI), _expected(c, False), 2e-3)
 
 
-@pytest.mark.parametrize("spacing", [8, 16, 64])
-def test_learned_readback(kernel, spacing):
+@pytest.mark.parametrize("spacing,terms", [(8, 1), (16, 1), (64, 1), (16, 2)])
+def test_learned_readback(kernel, spacing, terms):
     """The fused gather's learned times equal the restated lattice inference
-    (an untrained network: the check is of the arithmetic, not of accuracy)."""
+    (an untrained network: the check is of the arithmetic, not of accuracy),
+    with weights rounded to bf16 and with split (two-term) weights."""
     import torch
 
     from pm_amd.surrogate import Normalizer, TraveltimeSurrogate
 
     norm = Normalizer(tau_max=1.0, d_max=2000.0, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
     sur = TraveltimeSurrogate.create(norm, seed=3)
+    sur.weight_terms = terms
     # non-zero biases and larger outputs so every layer matters
     g = torch.Generator(device="cuda").manual_seed(5)
     sur.params += 0.05 * torch.randn(sur.params.shape, device="cuda", generator=g)
@@ -165,8 +168,11 @@ def vti_small(kernel):
     vc, ec = job.cube, job.meta["eta_cube"]
     d_max = job.config.max_aperture_m + 1500.0
     norm = Normalizer(tau_max=0.5, d_max=d_max, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
-    tr = make_training_set(vc, ec, 1_000_000, d_max, norm, seed=1, t_max_ms=1000.0)
-    sur = TraveltimeSurrogate.create(norm, seed=0).train(tr, steps=20000, batch=8192, qat_fraction=0.25)
+    tr = make_training_set(vc, ec, 2_000_000, d_max, norm, seed=1, t_max_ms=1000.0)
+    # f32 training, inference with split (two-term bf16) weights
+    sur = TraveltimeSurrogate.create(norm, seed=0)
+    sur.weight_terms = 2
+    sur.train(tr, steps=60000, batch=32768, qat_fraction=0.0)
     torch.cuda.synchronize()
     return job, ex, sur
 
@@ -215,8 +221,8 @@ def test_learned_job_against_exact(kernel, vti_small):
         times[name] = time.perf_counter() - t0
     print(f"image relL2 learned vs exact {rel:.3e}; without the network {rel_iso:.3e}; "
           f"pass time exact {times['exact'] * 1e3:.1f} ms, learned {times['learned'] * 1e3:.1f} ms")
-    assert rel < 1e-2
-    assert rel_iso > 10 * rel
+    assert rel <= 1e-3
+    assert rel_iso > 100 * rel
 
 
 def test_failed_tile_falls_back_to_exact(kernel, vti_small):

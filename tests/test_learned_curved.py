@@ -1,0 +1,213 @@
+"""Learned curved-ray traveltime fused into the gather (PSTM_TT_CURVED_LEARNED).
+
+The network learns the reference's one-way curved-ray time
+(pstm/algorithm/curved_ray.py:19-52, depth z(t0 / 2) by :239-262) as the
+factor c of z / sqrt(vg^2 + k^2 h^2) (1 + c), vg = sqrt(v0 v(z)), on the
+job's own V(z) = v0 + k z (the reference's log formula equals
+z / sqrt(xi^2 + k^2 h^2) for some xi in [v0, v(z)], so c is small).
+
+* Targets: the f64 torch restatement of the leg time equals the oracle's
+  curved DSR time (golden-pinned against the reference) leg by leg.
+* Read-back: a single ramp trace A[i] = i makes every image sample the
+  position of its contribution; the gather's learned curved times equal the
+  restated lattice inference (pm_amd.surrogate.lattice_times) for bf16 and
+  split weights.
+* Job level: on a small curved-ray diffractor job the network passes the
+  per-tile held-out check, the learned image is within relative L2 1e-3 of
+  the exact curved-ray image, and the method without the network (the
+  straight-ray time in the velocity at the image depth) is far off.
+* Fallback: a tile that fails the check migrates with the exact curved-ray
+  traveltime, bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from pm_amd.kernels import _abi
+
+V0, KG = 1700.0, 0.55
+
+
+def test_curved_targets_match_the_oracle():
+    import torch
+
+    from oracle import oracle as orc
+    from pm_amd.surrogate import curved_leg_times
+
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        t0, hs, hr = rng.uniform(0.0, 3.0), rng.uniform(0, 4000), rng.uniform(0, 4000)
+        v0, k = rng.uniform(1500, 2500), rng.uniform(0.1, 0.9)
+        tau = torch.tensor([t0 / 2], dtype=torch.float64)
+        h = torch.tensor([hs, hr], dtype=torch.float64)
+        got = float(curved_leg_times(tau, h[:1], v0, k)[0] + curved_leg_times(tau, h[1:], v0, k)[0])
+        assert got == pytest.approx(orc.curved_dsr(t0, hs, hr, v0, k), rel=1e-12, abs=1e-14)
+
+
+def test_curved_training_set_factor_reproduces_the_leg_time():
+    """The stored factor rebuilds the exact legs, and it is a small
+    correction: between sqrt((vg^2 + k^2 d^2) / (v(z)^2 + k^2 d^2)) - 1 and
+    sqrt((vg^2 + k^2 d^2) / (v0^2 + k^2 d^2)) - 1 (the mean value theorem)."""
+    import torch
+
+    from pm_amd.surrogate import (Normalizer, curved_velocity_at, learned_leg_time_curved,
+                                  make_training_set_curved)
+
+    norm = Normalizer(tau_max=1.0, d_max=3000.0, v_lo=1500.0, v_hi=4500.0, eta_max=1.0, out_scale=4.0)
+    ts = make_training_set_curved(V0, KG, 5000, 3000.0, norm, seed=2, t_max_ms=2000.0, device="cpu")
+    c = ts.y.double() / norm.out_scale
+    tau, ds, dr, vg = ts.x_raw[:, 0], ts.x_raw[:, 1], ts.x_raw[:, 2], ts.x_raw[:, 3]
+    t = learned_leg_time_curved(tau, ds, c[:, 0], V0, KG) + learned_leg_time_curved(tau, dr, c[:, 1], V0, KG)
+    err = (t - ts.t_exact.sum(1)).abs().max()
+    assert float(err) < 1e-6          # c is stored in f32
+    vz = curved_velocity_at(tau, V0, KG)
+    kd = KG * ds
+    lo = ((vg ** 2 + kd ** 2) / (vz ** 2 + kd ** 2)).sqrt() - 1 - 1e-6
+    hi = ((vg ** 2 + kd ** 2) / (V0 ** 2 + kd ** 2)).sqrt() - 1 + 1e-6
+    inside = (c[:, 0] >= lo) & (c[:, 0] <= hi) | (tau == 0)
+    assert bool(inside.all()) and float(c.abs().max()) < 0.35
+
+
+@pytest.fixture(scope="module")
+def kernel():
+    from pm_amd.kernels import HipKernel, KernelConfig
+
+    k = HipKernel()
+    k.initialize(KernelConfig())
+    yield k
+    k.cleanup()
+
+
+def _ramp_case(seed: int, n_samples=1201, nt=480, nxy=4):
+    import torch
+
+    from pm_amd.kernels.hip_backend import DeviceTraces
+
+    rng = np.random.default_rng(seed)
+    sx, sy = rng.uniform(0, 1500, 2)
+    rx, ry = rng.uniform(0, 1500, 2)
+    xa = np.linspace(600.0, 900.0, nxy)
+    ya = np.linspace(500.0, 800.0, nxy)
+    t_ms = 2.0 * np.arange(nt)
+    d = lambda a: torch.as_tensor(np.atleast_1d(np.asarray(a, dtype=np.float64)), device="cuda")  # noqa: E731
+    amp = np.arange(n_samples, dtype=np.float32)[None, :]
+    tr = DeviceTraces(torch.as_tensor(amp, device="cuda"), d(sx), d(sy), d(rx), d(ry), d((sx + rx) / 2),
+                      d((sy + ry) / 2), None, 2.0, 0.0)
+    return dict(tr=tr, xa=xa, ya=ya, t_ms=t_ms, src=(sx, sy), rec=(rx, ry), xa_d=d(xa), ya_d=d(ya), t_d=d(t_ms),
+                v_d=d(np.full(nt, 3000.0)))
+
+
+def _ramp_config():
+    from pm_amd.kernels import KernelConfig
+
+    return KernelConfig(max_aperture_m=1e5, min_aperture_m=1e5, max_dip_degrees=89.0, taper_fraction=1e-6,
+                        apply_spreading=False, apply_obliquity=False, kernel_type="curved_ray",
+                        curved_ray_enabled=True, curved_ray_v0=V0, curved_ray_k=KG)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spacing,terms", [(16, 1), (16, 2), (64, 2)])
+def test_learned_curved_readback(kernel, spacing, terms):
+    import torch
+
+    from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, curved_velocity_at, lattice_times
+
+    c = _ramp_case(4)
+    norm = Normalizer(tau_max=1.0, d_max=2000.0, v_lo=1500.0, v_hi=4500.0, eta_max=1.0, out_scale=4.0)
+    sur = TraveltimeSurrogate.create(norm, seed=3)
+    sur.kind, sur.weight_terms, sur.curved_v0 = "curved", terms, V0
+    g = torch.Generator(device="cuda").manual_seed(5)
+    sur.params += 0.05 * torch.randn(sur.params.shape, device="cuda", generator=g)
+    nx = ny = len(c["xa"])
+    nt = len(c["t_ms"])
+    img = torch.zeros((nx, ny, nt), dtype=torch.float32, device="cuda")
+    fold = torch.zeros((nx, ny), dtype=torch.int32, device="cuda")
+    kernel.migrate_device(c["tr"], img, fold, c["xa_d"], c["ya_d"], c["t_d"], c["v_d"], _ramp_config(),
+                          traveltime=_abi.TT_CURVED_LEARNED, surrogate=sur, node_spacing=spacing)
+    torch.cuda.synchronize()
+    got = img.double().cpu().numpy() * 2e-3
+    X, Y, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nt), indexing="ij")
+    pil = torch.as_tensor((X * ny + Y).ravel(), device="cuda")
+    k = torch.as_tensor(K.ravel(), device="cuda")
+    ox, oy = c["xa_d"][pil // ny], c["ya_d"][pil % ny]
+    ds = ((ox - c["src"][0]) ** 2 + (oy - c["src"][1]) ** 2).sqrt()
+    dr = ((ox - c["rec"][0]) ** 2 + (oy - c["rec"][1]) ** 2).sqrt()
+    tau_axis = c["t_d"] / 2000.0
+    vg = (V0 * curved_velocity_at(tau_axis, V0, KG)).sqrt()
+    exp = lattice_times(sur, tau_axis, k, ds, dr, lambda kk: vg[kk],
+                        lambda kk: torch.full(kk.shape, KG, dtype=torch.float64, device="cuda"), spacing)
+    exp = exp.cpu().numpy().reshape(nx, ny, nt)
+    u = exp / 2e-3
+    inside = (u > 1.0) & (u < 1199.0 - 1.0)
+    assert inside.sum() > 1000
+    err = np.abs(got - exp)[inside] / 2e-3
+    print(f"curved read-back error (samples, K={spacing}, terms={terms}): max {err.max():.2e}")
+    assert err.max() < 5e-3
+
+
+@pytest.fixture(scope="module")
+def curved_small(kernel):
+    import torch
+
+    from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set_curved
+    from pm_amd.synthetic import curved_diffractor_job
+
+    job = curved_diffractor_job(n_traces=30_000, n_samples=1001, n_out=64, nt=480, tile=32, n_diffractors=40,
+                                v0=V0, k=KG, device="cuda")
+    ex = TileExecutor(kernel, job)
+    d_max = job.config.max_aperture_m + 1500.0
+    norm = Normalizer(tau_max=0.5, d_max=d_max, v_lo=1500.0, v_hi=2500.0, eta_max=1.0, out_scale=10.0)
+    tr = make_training_set_curved(V0, KG, 2_000_000, d_max, norm, seed=1, t_max_ms=1000.0)
+    sur = TraveltimeSurrogate.create(norm, seed=0)
+    sur.kind, sur.weight_terms, sur.curved_v0 = "curved", 2, V0
+    sur.train(tr, steps=60000, batch=32768, qat_fraction=0.0)
+    torch.cuda.synchronize()
+    return job, ex, sur
+
+
+def _image(ex, **kw):
+    import torch
+
+    ex.reset()
+    ex.run(**kw)
+    img, _ = ex.assemble()
+    torch.cuda.synchronize()
+    return img.double()
+
+
+@pytest.mark.gpu
+def test_learned_curved_job_against_exact(kernel, curved_small):
+    from dataclasses import replace
+
+    import torch
+
+    job, ex, sur = curved_small
+    errs = ex.check_surrogate(sur, node_spacing=16)
+    print("per-tile held-out max error (samples)", errs)
+    assert all(e <= 0.25 for e in errs)
+    exact = _image(ex)
+    learned = _image(ex, surrogate=sur, node_spacing=16)
+    rel = float(torch.linalg.norm(learned - exact) / torch.linalg.norm(exact))
+    saved = ex.config
+    ex.config = replace(saved, kernel_type="straight_ray", curved_ray_enabled=False)
+    try:
+        straight = _image(ex)
+    finally:
+        ex.config = saved
+    rel_st = float(torch.linalg.norm(straight - exact) / torch.linalg.norm(exact))
+    print(f"curved: image relL2 learned vs exact {rel:.3e}; without the network (straight ray in v(z)) {rel_st:.3e}")
+    assert rel <= 1e-3
+    assert rel_st > 100 * rel
+
+
+@pytest.mark.gpu
+def test_learned_curved_fallback_is_exact(kernel, curved_small):
+    import torch
+
+    job, ex, sur = curved_small
+    ex.check_surrogate(sur, node_spacing=16, bar=1e-9)
+    assert not any(t.learned_ok for t in ex.tiles)
+    assert torch.equal(_image(ex), _image(ex, surrogate=sur, node_spacing=16))

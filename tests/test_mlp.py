@@ -156,8 +156,17 @@ class TestHipSurrogate:
         ref = omlp.forward_bf16(p, x)
         pd, xd = torch.as_tensor(p, device="cuda"), torch.as_tensor(x, device="cuda")
         scale = np.max(np.abs(ref))
-        for mode in ("bf16", "mfma"):
+        ref_split = omlp.forward_bf16_split(p, x)
+        for mode in ("bf16", "mfma", "mfma_split"):
             y = torch.full((1000, 2), float("nan"), device="cuda")
+            if mode == "mfma_split":
+                assert lib.pstm_mlp_forward_mfma_terms(pd.data_ptr(), xd.data_ptr(), 1000, y.data_ptr(), 2,
+                                                       _stream()) == 0
+                err = np.max(np.abs(y.cpu().numpy() - ref_split)) / scale
+                print(mode, "max abs error / max |y|", err, "(bf16 weights:", np.max(np.abs(ref - ref_split)) / scale,
+                      ")")
+                assert err < 2e-5
+                continue
             if mode == "bf16":
                 assert lib.pstm_mlp_set_precision(1) == 0
                 try:
