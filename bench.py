@@ -220,6 +220,7 @@ def vti_headline(kernel, args) -> dict:
     from pm_amd import roofline
     from pm_amd.kernels import _abi
     from pm_amd.pipeline.executor import TileExecutor
+    from pm_amd.lattice import Lattice
     from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set
     from pm_amd.synthetic import vti_diffractor_job
 
@@ -242,11 +243,13 @@ def vti_headline(kernel, args) -> dict:
     tune = None
     if args.autotune:
         # output bins per tile (tile edge) x launch shape, on the exact
-        # traveltime over a fixed region of 2 x region pillars per side
+        # traveltime over the step's own region (the same work items per
+        # workgroup as the timed steps: a larger region would favour shapes
+        # whose item count does not divide evenly over the workgroups here)
         from pm_amd.autotune import autotune_tiles
 
-        edges = tuple(e for e in (16, 32, 64) if e <= 2 * args.region)
-        tune = autotune_tiles(kernel, job, edges=edges, region=2 * args.region,
+        edges = tuple(e for e in (16, 32, 64) if e <= args.region)
+        tune = autotune_tiles(kernel, job, edges=edges, region=args.region,
                               candidates=[(3, 16, 1), (2, 16, 1), (4, 8, 0)], max_edge=args.region)
         log(f"autotune: edge {tune.edge}, mw {tune.mw}, batch {tune.batch}, pipe {tune.pipe}")
     edge = tune.edge if tune else args.region
@@ -271,16 +274,16 @@ def vti_headline(kernel, args) -> dict:
     t0 = time.perf_counter()
     sur = TraveltimeSurrogate.create(norm, seed=0)
     sur.weight_terms = args.weight_terms
-    # bf16-rounded weights train through that rounding for the last quarter;
-    # split weights keep an f32-trained network
-    sur.train(tr, steps=args.train_steps, batch=args.train_batch, qat_fraction=0.25 if args.weight_terms == 1 else 0.0,
-              log_every=max(args.train_steps // 8, 1), verbose=True)
+    # an f32-trained network; the gather rounds its weights to bf16 (one
+    # term) or splits them in two (--weight-terms 2)
+    sur.train(tr, steps=args.train_steps, batch=args.train_batch, log_every=max(args.train_steps // 8, 1),
+              verbose=True)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     held = sur.error_samples(tr, job.dt_ms, bf16=True) if args.train_samples <= 2_000_000 else None
     del tr
     log(f"surrogate: training set {set_s:.1f} s, training {train_s:.1f} s")
-    K = args.node_spacing
+    K = Lattice(*(int(v) for v in args.lattice.split(",")))
     errs = ex.check_surrogate(sur, node_spacing=K, tiles=sub)
     if args.force_learned:
         for i in sub:
@@ -402,7 +405,9 @@ def vti_headline(kernel, args) -> dict:
             "launch_shape": {"learned": learned_info, "exact": exact_info},
             "surrogate": {"train_samples": args.train_samples, "steps": args.train_steps,
                           "batch": args.train_batch, "train_s": round(train_s, 2),
-                          "training_set_s": round(set_s, 2), "node_spacing": K,
+                          "training_set_s": round(set_s, 2),
+                          "lattice": {"spacing": K.spacing, "growth": K.growth, "max_spacing": K.max_spacing,
+                                      "nodes_per_trace": int(len(K.nodes(ex.nt)))},
                           "heldout_max_samples_per_tile": [round(e, 4) for e in errs],
                           "tiles_learned": len(learned), "tiles_exact_fallback": len(fallback),
                           "training_set_error_samples": held, "forced": bool(args.force_learned),
@@ -700,10 +705,11 @@ def main() -> None:
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=100_000)
     ap.add_argument("--train-batch", type=int, default=65_536)
-    ap.add_argument("--weight-terms", type=int, default=2, choices=[1, 2],
-                    help="fused inference weights: 1 rounded to bf16 (quantisation-aware training), 2 as two bf16 "
-                         "terms (f32 training)")
-    ap.add_argument("--node-spacing", type=int, default=16)
+    ap.add_argument("--weight-terms", type=int, default=1, choices=[1, 2],
+                    help="fused inference weights of the f32-trained network: 1 rounded to bf16, 2 as two bf16 "
+                         "terms (one more MFMA per product)")
+    ap.add_argument("--lattice", default="8,77,48",
+                    help="t0 lattice of the learned traveltime: smallest spacing, growth (1/256), largest spacing")
     ap.add_argument("--force-learned", action="store_true",
                     help="use the learned traveltime on every subset tile whatever its held-out error "
                          "(profiling runs with a briefly trained network)")

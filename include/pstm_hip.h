@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PSTM_HIP_ABI_VERSION 4
+#define PSTM_HIP_ABI_VERSION 5
 
 /* traveltime: KernelConfig.kernel_type (pstm/kernels/base.py:220-229) */
 #define PSTM_TT_STRAIGHT 0 /* DSR, numba_cpu_optimized.py:189-194            */
@@ -131,19 +131,27 @@ typedef struct PstmMigrateArgs {
     double eta_constant;          /* PSTM_ETA_CONST                       */
     /* --- learned traveltime (PSTM_TT_VTI_LEARNED, PSTM_TT_CURVED_LEARNED) ---
      * Per (pillar, trace) pair the network is evaluated on a lattice of output
-     * samples k = 0, K, 2K, ... (K = mlp_node_spacing; only where some sample
-     * that reads the node is inside the pair's aperture) with inputs
+     * samples n_0 = 0, n_{j+1} = min(n_j + min(max((n_j g) >> 8, K), K_max),
+     * nt - 1) up to the last sample nt - 1 (K = mlp_node_spacing, g =
+     * mlp_node_growth, K_max = mlp_node_max or K; g = 0: uniform spacing K),
+     * only at nodes read by some sample inside the pair's aperture, with
+     * inputs
      *   x = (2 tau / tau_max - 1, 2 s_s - 1, 2 s_r - 1,
      *        2 (v - v_lo) / (v_hi - v_lo) - 1, 2 eta / eta_max - 1),
      * tau = t0 / 2, v and eta at (pillar, t0), s = sine of the straight-ray
      * angle of a leg, d / (v sqrt(tau^2 + d^2 / v^2)).  Outputs are
-     * out_scale * c for each leg; c is interpolated linearly between lattice
-     * samples and the leg time is sqrt(tau^2 + (d / v)^2 (1 + c)), with t = t0
-     * for t0 <= 0 as in the exact VTI time; lattice samples past the axis
-     * extrapolate tau linearly.                                             */
+     * out_scale * g for each leg, g = c / eta the anisotropic factor c of the
+     * leg, (t^2 - tau^2) / (d / v)^2 - 1, per unit eta (for the exact VTI
+     * time g = -2 s^2 / (1 + 2 eta s^2), in [-2, 0]); g is interpolated
+     * linearly between lattice nodes and the leg time is
+     * sqrt(tau^2 + (d / v)^2 max(1 + eta g, 0)) with v and eta at the output
+     * sample itself, t = t0 for t0 <= 0 as in the exact VTI time.  Carrying
+     * g rather than c leaves the lattice to follow only the factor's own
+     * change with t0, not that of eta in the model.  Needs nt >= 2.        */
     const float* mlp_params;      /* PSTM_MLP_PARAMS f32 (network layout below);
                                      weights are rounded to bf16 on use       */
-    int32_t mlp_node_spacing;     /* K: 8, 16, 32 or 64 output samples        */
+    int32_t mlp_node_spacing;     /* K: smallest node spacing, 4 .. 512
+                                     output samples                           */
     int32_t interpolation;        /* PSTM_INTERP_* (0 = linear)               */
     double mlp_tau_max;
     double mlp_v_lo;
@@ -155,6 +163,10 @@ typedef struct PstmMigrateArgs {
                                      2: each weight as two bf16 terms, high +
                                      remainder (about 16 bits, an f32-trained
                                      network): one more MFMA per product      */
+    int32_t mlp_node_growth;      /* g: spacing growth, 0 .. 1024 (1/256 of the
+                                     node's sample index)                     */
+    int32_t mlp_node_max;         /* K_max: largest node spacing, K .. 1024
+                                     (0: K)                                   */
     int32_t reserved0;            /* 0                                        */
 } PstmMigrateArgs;
 

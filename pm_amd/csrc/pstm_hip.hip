@@ -41,7 +41,7 @@
 // launches read, and the gather launchers of each kernel group.  The groups
 // (straight ray with / without anti-aliasing, curved ray, VTI, learned VTI,
 // interpolators) are compiled from this file in translation units of their
-// own (gather_part1..6.hip, PSTM_GATHER_PART = 1..6) so that the kernel
+// own (gather_part1..7.hip, PSTM_GATHER_PART = 1..7) so that the kernel
 // variants build in parallel; without PSTM_GATHER_PART this file is the rest
 // of the library.
 void pstm_set_last_error(const std::string& msg);
@@ -97,6 +97,14 @@ constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS use
 
 // traveltimes read from the fused network's t0 lattice
 constexpr bool learned_tt(int tt) { return tt == PSTM_TT_VTI_LEARNED || tt == PSTM_TT_CURVED_LEARNED; }
+
+// The t0 lattice of the learned traveltime: nodes n_0 = 0, n_{j+1} = n_j +
+// min(max((n_j grow) >> 8, kmin), kmax), the last one clamped to the axis end
+// `last` (integer arithmetic, the same on the host, the device and in Python)
+__host__ __device__ inline int lattice_next(int n, int kmin, int grow, int kmax, int last) {
+    const int step = min(max((n * grow) >> 8, kmin), kmax);
+    return min(n + step, last);
+}
 
 // per-(pillar, sample) terms, f32, built once per call in f64
 //   x : traveltime slowness^2 s = 1/v^2       (curved ray: v(z) at depth)
@@ -174,11 +182,14 @@ struct KParams {
     int mlp_split;               // 1: weights as two bf16 terms (mlp_weight_terms = 2)
     const void* mlpB;            // f32 biases in accumulator layout
     int lat_off;                 // byte offset of the lattice in LDS
-    int lat_nodes;               // lattice samples per (pair, window): 256 MW / K + 1
+    int lat_nodes;               // largest lattice node count of a time window
     int lat_cimin_off;           // byte offset of the per-(pillar, node) widest-aperture table
     int lat_list_off;            // byte offset of the list of lattice rows to evaluate
-    int lat_shift;               // log2 K
-    float lat_ifrac;             // 1 / K
+    int node_off;                // byte offset of the per-(pillar, node) network inputs
+    int mlp_off;                 // byte offset of the weight operands (then the biases) in LDS
+    int lat_kmin, lat_grow, lat_kmax;  // node spacing law (lattice_next)
+    int lat_total;               // lattice nodes over the whole output axis
+    int lat_pos_off;             // byte offset of the node positions (output samples) in LDS
     float m_tau_sc, m_v_lo, m_v_sc, m_eta_sc, m_inv_out;
 };
 
@@ -187,18 +198,13 @@ __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x)
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
 
-// 0.5 (1 + cos(pi x)) for x in [0, 1]: with y = x - 1/2, cos(pi x) = -sin(pi y);
-// odd Taylor series of sin(pi y) through y^11 (|pi y| <= pi/2, error < 6e-8).
+// 0.5 (1 + cos(pi x)) for x in [0, 1] (the reference's cosine taper,
+// numba_cpu_optimized.py:211-215) with the hardware cosine, whose argument is
+// in revolutions (cos(2 pi r), r = x / 2 in [0, 0.5]): one transcendental
+// and two VALU operations; measured against the f64 taper over [0, 1]: at
+// most 9.2e-8 absolute
 __device__ __forceinline__ float taper_weight(float x) {
-    const float y = x - 0.5f;
-    const float z = y * y;
-    float p = -7.370430945714348e-03f;        // -pi^11/11!
-    p = fmaf(p, z, 8.214588661112819e-02f);   //  pi^9/9!
-    p = fmaf(p, z, -5.992645293207919e-01f);  // -pi^7/7!
-    p = fmaf(p, z, 2.550164039877345e+00f);   //  pi^5/5!
-    p = fmaf(p, z, -5.167712780049969e+00f);  // -pi^3/3!
-    p = fmaf(p, z, 3.141592653589793e+00f);   //  pi
-    return fmaf(-0.5f, p * y, 0.5f);
+    return fmaf(0.5f, __builtin_amdgcn_cosf(0.5f * x), 0.5f);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,9 +486,11 @@ static_assert(mlp_w_off(kMlpLayers) == PSTM_MLP_PARAMS, "network layout");
 // [mt 2][j 4][lane 64]; layer 4 (outputs padded to 32) [j 4][lane 64]; the
 // remainders of split weights follow as a second image of kAVec8 vectors
 constexpr int kA1 = 2 * 64, kA4 = kA1 + 3 * 8 * 64, kAVec8 = kA4 + 4 * 64, kAVals = kAVec8 * 8;
-// biases in accumulator layout, in f32x16 units: layers 0-3 [mt 2][lane 64],
-// layer 4 [lane 64]
-constexpr int kB4 = 4 * 2 * 64, kBNum = kB4 + 64;
+// biases in accumulator layout, in f32x16 units: vector v (layers 0-3:
+// 2 layer + mt, layer 4: 8) for lane half h at 2 v + h; the accumulator
+// layout depends on the lane only through its half, so every lane of a half
+// reads the same 64 bytes
+constexpr int kBVec = 4 * 2 + 1, kBVals = kBVec * 2 * 16;
 
 // hidden unit carried by element e of lane half h of K step j (see above)
 __host__ __device__ constexpr int mlp_unit(int j, int h, int e) {
@@ -516,15 +524,10 @@ __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restri
         A[e] = hi;
         // the weight's remainder as a second bf16 term (split-weight inference)
         A[kAVals + e] = bf16_bits(w - __builtin_bit_cast(float, (unsigned)hi << 16));
-    } else if (e < kAVals + kBNum * 16) {
-        const int f = e - kAVals, v = f / 16, r = f % 16;
-        int l, mt, lane;
-        if (v < kB4) {
-            l = v / 128; mt = (v / 64) % 2; lane = v % 64;
-        } else {
-            l = 4; mt = 0; lane = v - kB4;
-        }
-        const int o = mt * 32 + 8 * (r / 4) + 4 * (lane / 32) + r % 4;
+    } else if (e < kAVals + kBVals) {
+        const int f = e - kAVals, v = f / 32, h = (f / 16) % 2, r = f % 16;
+        const int l = v < 8 ? v / 2 : 4, mt = v < 8 ? v % 2 : 0;
+        const int o = mt * 32 + 8 * (r / 4) + 4 * h + r % 4;
         B[f] = o < mlp_out(l) ? P[mlp_w_off(l) + mlp_in(l) * mlp_out(l) + o] : 0.0f;
     }
 }
@@ -554,9 +557,8 @@ __device__ __forceinline__ void act_pack(const f32x16_t (&acc)[2], bf16x8_t (&h)
 
 // one K step: W (x_hi + x_lo), plus w_lo x_hi for split weights (the w_lo
 // x_lo term, about 2^-16 of the product, is left out)
-template <bool SPLIT>
-__device__ __forceinline__ f32x16_t mfma_w(const bf16x8_t* __restrict__ WA, int idx, bf16x8_t bh, bf16x8_t bl,
-                                            f32x16_t c) {
+template <bool SPLIT, typename PA>
+__device__ __forceinline__ f32x16_t mfma_w(PA WA, int idx, bf16x8_t bh, bf16x8_t bl, f32x16_t c) {
     const bf16x8_t a = WA[idx];
     c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bh, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl, c, 0, 0, 0);
@@ -567,12 +569,13 @@ __device__ __forceinline__ f32x16_t mfma_w(const bf16x8_t* __restrict__ WA, int 
 // Network outputs (y0, y1) of the row held by this lane (valid in lanes
 // 0-31; lanes l and l + 32 carry the same row).  Wave-uniform call, all
 // lanes active.  SPLIT: each weight carried as two bf16 terms (WA holds the
-// high terms, WA + kAVec8 the remainders).
-template <bool SPLIT>
-__device__ __forceinline__ void mlp_eval32(const bf16x8_t* __restrict__ WA, const f32x16_t* __restrict__ WB,
-                                           const float (&x)[5], float& y0, float& y1) {
+// high terms, WA + kAVec8 the remainders).  WA, WB: the operand images in
+// global memory or (the fused gather) in LDS.
+template <bool SPLIT, typename PA, typename PB>
+__device__ __forceinline__ void mlp_eval32(PA WA, PB WB, const float (&x)[5], float& y0, float& y1) {
     const int lane = threadIdx.x & 63;
     const bool upper = lane >= 32;
+    const int hb = lane >> 5;
     bf16x8_t h[4], l[4];
     {
         // layer 0: the 5 inputs are elements 0-4 of the lower lane half
@@ -582,7 +585,7 @@ __device__ __forceinline__ void mlp_eval32(const bf16x8_t* __restrict__ WA, cons
         split8(xin, xh, xl);
         f32x16_t acc[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma_w<SPLIT>(WA, mt * 64 + lane, xh, xl, WB[mt * 64 + lane]);
+        for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma_w<SPLIT>(WA, mt * 64 + lane, xh, xl, WB[mt * 2 + hb]);
         act_pack(acc, h, l);
     }
 #pragma unroll 1
@@ -591,13 +594,13 @@ __device__ __forceinline__ void mlp_eval32(const bf16x8_t* __restrict__ WA, cons
         f32x16_t acc[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            acc[mt] = WB[(layer * 2 + mt) * 64 + lane];
+            acc[mt] = WB[(layer * 2 + mt) * 2 + hb];
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[mt] = mfma_w<SPLIT>(WA, wa + (mt * 4 + j) * 64 + lane, h[j], l[j], acc[mt]);
         }
         act_pack(acc, h, l);
     }
-    f32x16_t o = WB[kB4 + lane];
+    f32x16_t o = WB[8 * 2 + hb];
 #pragma unroll
     for (int j = 0; j < 4; ++j) o = mfma_w<SPLIT>(WA, kA4 + j * 64 + lane, h[j], l[j], o);
     y0 = o[0];
@@ -632,7 +635,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x8_t* __rest
 // (pillar, sample) terms.  y1/y2 return 1/t_src, 1/t_rec for anti-aliasing.
 template <int TT, bool AA>
 __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e, const float4& f, float q, float cs,
-                                              float e2v, float e1v, float t0v, float& y1, float& y2) {
+                                              float e2v, float e1v, float t0v, float etav, float& y1, float& y2) {
     float t;
     y1 = 0.f;
     y2 = 0.f;
@@ -663,10 +666,11 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         // (e2v, e1v); 1 + c >= 0 keeps t >= 0, and t = 0 at t0 = 0 (z = 0)
         t = cs * fmaf(frsq(e.x + q), fmaxf(1.0f + e2v, 0.0f), frsq(e.y + q) * fmaxf(1.0f + e1v, 0.0f));
     } else if (TT == PSTM_TT_VTI_LEARNED) {
-        // learned legs: t^2 = tau^2 + h^2 s (1 + c), c from the lattice
-        // (e2v, e1v here); 1 + c >= 0 keeps t >= t0 as the exact VTI time is
-        const float a1 = fmaf(e.x * cs, fmaxf(1.0f + e2v, 0.0f), q);
-        const float a2 = fmaf(e.y * cs, fmaxf(1.0f + e1v, 0.0f), q);
+        // learned legs: t^2 = tau^2 + h^2 s (1 + eta g), g = c / eta from the
+        // lattice (e2v, e1v here) and eta at the sample (etav); 1 + eta g >= 0
+        // keeps t >= t0 as the exact VTI time is
+        const float a1 = fmaf(e.x * cs, fmaxf(fmaf(etav, e2v, 1.0f), 0.0f), q);
+        const float a2 = fmaf(e.y * cs, fmaxf(fmaf(etav, e1v, 1.0f), 0.0f), q);
         t = fsqrt(a1) + fsqrt(a2);
         if (!(t0v > 0.f)) t = t0v;
     } else {
@@ -966,6 +970,26 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         for (int i = tid; i < kMaxHalfWidth + 2; i += kThreads)
             s_invsq[i] = make_float2(i > 0 ? 1.0f / (float)(i * i) : 0.0f, 1.0f / (float)((i + 1) * (i + 1)));
     }  // visible after the first barrier of the item loop
+    // learned traveltime: the network's operand images stay in LDS for the
+    // workgroup's life (a weight operand per MFMA from global memory would
+    // wait out the L2 latency once per product)
+    using LdsA = __attribute__((address_space(3))) const bf16x8_t;
+    using LdsB = __attribute__((address_space(3))) const f32x16_t;
+    const int mlp_nv = learned_tt(TT) ? kAVec8 * (P.mlp_split ? 2 : 1) : 0;
+    if (learned_tt(TT)) {
+        if (tid == 0) {
+            int* npos = reinterpret_cast<int*>(smem + P.lat_pos_off);
+            npos[0] = 0;
+            for (int j = 1; j < P.lat_total; ++j)
+                npos[j] = lattice_next(npos[j - 1], P.lat_kmin, P.lat_grow, P.lat_kmax, P.nt - 1);
+        }
+        const uint4* ws = static_cast<const uint4*>(P.mlpA);
+        uint4* wd = reinterpret_cast<uint4*>(smem + P.mlp_off);
+        for (int i = tid; i < mlp_nv; i += kThreads) wd[i] = ws[i];
+        const float4* bs = static_cast<const float4*>(P.mlpB);
+        float4* bd = reinterpret_cast<float4*>(smem + P.mlp_off + mlp_nv * 16);
+        for (int i = tid; i < kBVals / 4; i += kThreads) bd[i] = bs[i];
+    }  // visible after the first barrier of the item loop
 
     // persistent workgroups drain the work queue; every workgroup leaves when
     // the counter passes n_items
@@ -990,8 +1014,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         const int kbase = kwin * win;
 
         // register-resident per-(pillar, sample) terms
-        float q[MW], cs[kPW][MW], cw[kPW][MW], ci[kPW][MW];
+        float q[MW], cs[kPW][MW], ci[kPW][MW];
         float e2[kPW][MW], e1[kPW][MW], t0r[MW];
+        float etar[kPW][MW];  // learned VTI: eta at (pillar, sample)
         float acc[kPW][MW];
         double* tot = reinterpret_cast<double*>(smem + P.tot_off);  // [kPW*MW][kThreads], lane-private
 #pragma unroll
@@ -1007,23 +1032,33 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 tot[(p * MW + m) * kThreads + tid] = 0.0;
                 e2[p][m] = 0.0f;
                 e1[p][m] = 1.0f;
+                etar[p][m] = 0.0f;
                 if (kv && pil < npil) {
                     const float4 c = T.kc[(size_t)pil * T.kc_pstride + k];
                     cs[p][m] = c.x;
-                    cw[p][m] = c.y;
                     ci[p][m] = c.z;
                     if (TT == PSTM_TT_VTI) {
                         const float2 ee = T.ek[(size_t)pil * T.ek_pstride + k];
                         e2[p][m] = ee.x;
                         e1[p][m] = ee.y;
+                    } else if (TT == PSTM_TT_VTI_LEARNED) {
+                        etar[p][m] = 0.5f * T.ek[(size_t)pil * T.ek_pstride + k].x;
                     }
                 } else {
                     cs[p][m] = 0.0f;
-                    cw[p][m] = 0.0f;
                     ci[p][m] = INFINITY;  // x_t = +inf (d > 0) or NaN (d = 0): never in aperture
                 }
             }
         }
+
+        // the weight constant kc.y of a slot: constant over the traces, so
+        // applied to the slot's f64 total when the item ends; only early blocks
+        // (below) need it per contribution, where it is read when used
+        auto cw_at = [&](int p, int m) -> float {
+            const int k = kbase + 64 * (tw + kTW * m) + lane;
+            const int pil = pil0 + pg * kPW + p;
+            return (k < P.nt && pil < npil) ? T.kc[(size_t)pil * T.kc_pstride + k].y : 0.0f;
+        };
 
         // sample blocks of this wave whose first t0 is <= 2 ms: only there can
         // t <= 1 ms switch the spreading/obliquity weights off (wave-uniform)
@@ -1043,23 +1078,69 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // fraction, and per (pillar, node) the smallest taper slope (widest
         // aperture) over the samples that read the node, (j - 1) K .. (j + 1) K
         // - 1 of the window: a pair farther than that aperture needs no network
-        // evaluation at the node
+        // evaluation at the node.  The node's own network inputs (t0 / 2,
+        // slowness^2 or vg^2, encoded velocity and eta / gradient) are tabled
+        // here too, so that the evaluation reads LDS only.  A node past the
+        // axis extrapolates t0 linearly and takes v, eta at the axis end.
         int nidx[MW];
-        float nfr = 0.0f;
+        float nfr[MW];
+        int nn = 0;  // lattice nodes of this window (wave-uniform)
         if (learned_tt(TT)) {
+            // the window's nodes are jf .. jl of the axis lattice: the last at or
+            // before its first sample, the first at or after its last one; each
+            // slot reads the two nodes of its segment (clamped to the window's)
+            const int* npos = reinterpret_cast<const int*>(smem + P.lat_pos_off);
+            auto upper = [&](int k) {  // first node past sample k
+                int lo = 0, hi = P.lat_total;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (npos[mid] <= k) lo = mid + 1;
+                    else hi = mid;
+                }
+                return lo;
+            };
+            const int kend = min(kbase + win, P.nt);
+            const int jf = min(upper(kbase) - 1, P.lat_total - 2);
+            const int jl = max(upper(kend - 2), jf + 1);
+            nn = __builtin_amdgcn_readfirstlane(jl - jf + 1);
 #pragma unroll
-            for (int m = 0; m < MW; ++m) nidx[m] = (64 * (tw + kTW * m) + lane) >> P.lat_shift;
-            nfr = (float)(lane & ((1 << P.lat_shift) - 1)) * P.lat_ifrac;
+            for (int m = 0; m < MW; ++m) {
+                const int k = min(kbase + 64 * (tw + kTW * m) + lane, kend - 1);
+                const int j = max(jf, min(upper(k) - 1, jl - 1));
+                nidx[m] = j - jf;
+                nfr[m] = (float)(k - npos[j]) / (float)(npos[j + 1] - npos[j]);
+            }
             float* cimin = reinterpret_cast<float*>(smem + P.lat_cimin_off);
-            const int nn = P.lat_nodes, K = 1 << P.lat_shift;
+            float4* ntab = reinterpret_cast<float4*>(smem + P.node_off);
             for (int e = tid; e < kNPil * nn; e += kThreads) {
-                const int p = e / nn, j = e - p * nn;
+                const int p = e / nn, i = e - p * nn;
                 const int pil = pil0 + p;
-                const int k0 = max(kbase + (j - 1) * K, kbase), k1 = min(kbase + (j + 1) * K, min(kbase + win, P.nt));
+                const int j = jf + i;
+                // samples that read node j: segments j - 1 and j inside the
+                // window, and the window's last sample (clamped to its last
+                // segment, which may end at it)
+                const int k0 = i == 0 ? kbase : max(kbase, npos[j - 1]);
+                const int k1 = i == nn - 1 ? kend : min(kend, npos[j + 1] + 1);
                 float c = INFINITY;
-                if (pil < npil)
+                float4 nd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (pil < npil) {
                     for (int k = k0; k < k1; ++k) c = fminf(c, T.kc[(size_t)pil * T.kc_pstride + k].z);
+                    const int kk = npos[j];
+                    const float tau = 0.5f * P.t0k[kk];
+                    float s, v, eta;
+                    if (TT == PSTM_TT_VTI_LEARNED) {
+                        s = T.kc[(size_t)pil * T.kc_pstride + kk].x;  // 1/v^2
+                        v = frsq(s);
+                        eta = 0.5f * T.ek[(size_t)pil * T.ek_pstride + kk].x;
+                    } else {
+                        s = P.qk[kk];  // vg^2
+                        v = fsqrt(s);
+                        eta = (float)P.ck;
+                    }
+                    nd = make_float4(tau, s, fmaf(v - P.m_v_lo, P.m_v_sc, -1.0f), fmaf(eta, P.m_eta_sc, -1.0f));
+                }
                 cimin[e] = c;
+                ntab[e] = nd;
             }
         }
 
@@ -1155,6 +1236,19 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         for (int j0 = 0, par = 0; j0 < T.ntr; j0 += P.batch, par ^= 1) {
             const int nb = min(P.batch, T.ntr - j0);
             if (early_dma) dma(j0, 0, j0 == 0 ? kRowsGlobal : kRowsNext + (par ^ 1), w_pairs);
+            // pipelined f32 rows, loaded into registers during the batch
+            // before (or before the first): to LDS at the batch start, while
+            // the row buffer is free (the batch before ended with a barrier),
+            // so that those registers are free for the rest of the batch (the
+            // network evaluation above all); visible after the pair-table
+            // barrier.  A skipped batch stores rows nobody reads.
+            if (!AA && P.pipe) {
+                uint32_t* dst = reinterpret_cast<uint32_t*>(rows);
+                const int n = nb * rl_dw;
+#pragma unroll
+                for (int i = 0; i < kPF; ++i)
+                    if (tid + i * kThreads < n) dst[tid + i * kThreads] = pf[i];
+            }
             // (pillar, trace) pair table; the max-aperture test is the
             // oracle's f64 one (numba_cpu_optimized.py:167-173)
             int inap = 0;
@@ -1223,7 +1317,6 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 // read by some slot inside the pair's time-dependent aperture
                 // are listed (in any order), the others set to 0; then 32 listed
                 // rows per wavefront step
-                const int nn = P.lat_nodes, K = 1 << P.lat_shift;
                 const int R = kNPil * P.batch * nn;
                 float2* lat = reinterpret_cast<float2*>(smem + P.lat_off);
                 const float* cimin = reinterpret_cast<const float*>(smem + P.lat_cimin_off);
@@ -1246,7 +1339,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 }
                 __syncthreads();
                 const int nrows = s_nrows;
-                const float dt_last = P.nt > 1 ? P.t0k[P.nt - 1] - P.t0k[P.nt - 2] : 0.0f;
+                const float4* ntab = reinterpret_cast<const float4*>(smem + P.node_off);
+                LdsA* wa = (LdsA*)(smem + P.mlp_off);
+                LdsB* wb = (LdsB*)(smem + P.mlp_off + mlp_nv * 16);
                 for (int c0 = wave * 32; c0 < nrows; c0 += kWaves * 32) {
                     const int li = c0 + (lane & 31);
                     const bool valid = li < nrows;
@@ -1255,44 +1350,31 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     if (valid) {
                         const int pp = r / nn, node = r - pp * nn;
                         const float4 e = pairs[pp * kPairVec];
-                        const int pil = pil0 + pp / P.batch;
-                        // node at output sample k = kbase + node K; past the axis t0
-                        // is extrapolated linearly and v, eta taken at its end
-                        const int k = kbase + node * K;
-                        const int kk = min(k, P.nt - 1);
-                        const float t0 = P.t0k[kk] + (float)(k - kk) * dt_last;
-                        const float tau = 0.5f * t0;
-                        const float q = tau * tau;
-                        float eta, sn_s, sn_r, v;
+                        const float4 nd = ntab[(pp / P.batch) * nn + node];  // {tau, s, v enc., eta enc.}
+                        const float tau = nd.x;
+                        float sn_s, sn_r;
                         if (TT == PSTM_TT_VTI_LEARNED) {
-                            const float sl = T.kc[(size_t)pil * T.kc_pstride + kk].x;  // 1/v^2
-                            eta = 0.5f * T.ek[(size_t)pil * T.ek_pstride + kk].x;
-                            const float hs = e.x * sl, hr = e.y * sl;
+                            const float q = tau * tau;
+                            const float hs = e.x * nd.y, hr = e.y * nd.y;
                             const float ts2 = q + hs, tr2 = q + hr;
                             sn_s = ts2 > 0.0f ? fsqrt(hs / ts2) : 0.0f;
                             sn_r = tr2 > 0.0f ? fsqrt(hr / tr2) : 0.0f;
-                            v = frsq(sl);
                         } else {
-                            // curved: sines k h / sqrt(vg^2 + k^2 h^2), vg and the gradient k
-                            const float qg = P.qk[kk];  // vg^2
-                            sn_s = fsqrt(e.x / (e.x + qg));
-                            sn_r = fsqrt(e.y / (e.y + qg));
-                            v = fsqrt(qg);
-                            eta = (float)P.ck;
+                            // curved: sines k h / sqrt(vg^2 + k^2 h^2)
+                            sn_s = fsqrt(e.x / (e.x + nd.y));
+                            sn_r = fsqrt(e.y / (e.y + nd.y));
                         }
                         x[0] = fmaf(tau, P.m_tau_sc, -1.0f);
                         x[1] = fmaf(2.0f, sn_s, -1.0f);
                         x[2] = fmaf(2.0f, sn_r, -1.0f);
-                        x[3] = fmaf(v - P.m_v_lo, P.m_v_sc, -1.0f);
-                        x[4] = fmaf(eta, P.m_eta_sc, -1.0f);
+                        x[3] = nd.z;
+                        x[4] = nd.w;
                     }
                     float y0, y1;
                     if (P.mlp_split)
-                        mlp_eval32<true>(static_cast<const bf16x8_t*>(P.mlpA), static_cast<const f32x16_t*>(P.mlpB), x,
-                                         y0, y1);
+                        mlp_eval32<true>(wa, wb, x, y0, y1);
                     else
-                        mlp_eval32<false>(static_cast<const bf16x8_t*>(P.mlpA), static_cast<const f32x16_t*>(P.mlpB), x,
-                                          y0, y1);
+                        mlp_eval32<false>(wa, wb, x, y0, y1);
                     if (lane < 32 && valid) lat[r] = make_float2(y0 * P.m_inv_out, y1 * P.m_inv_out);
                 }
             }
@@ -1302,11 +1384,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 rows = reinterpret_cast<Row*>(smem + (size_t)cur * P.rows_bytes);
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
             } else if (P.pipe) {
-                uint32_t* dst = reinterpret_cast<uint32_t*>(rows);
-                const int n = nb * rl_dw;
-#pragma unroll
-                for (int i = 0; i < kPF; ++i)
-                    if (tid + i * kThreads < n) dst[tid + i * kThreads] = pf[i];
+                // f32 rows: stored at the batch start
             } else if (AA) {
                 // the batch's rows, loaded at the batch start (early_dma)
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
@@ -1339,7 +1417,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     }
                 }
             }
-            __syncthreads();
+            // pipelined f32 rows are visible since the pair-table barrier: only
+            // DMA'd or loaded rows and the lattice need this one
+            if (AA || !P.pipe || learned_tt(TT)) __syncthreads();
             if (P.pipe) {  // in flight during the batch
                 if (AA) dma(j0 + P.batch, cur ^ 1, kRowsNext + par, 0);
                 else prefetch(j0 + P.batch, kRowsNext + par);
@@ -1356,7 +1436,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 asm("" : "+v"(uoff_v));
                 for (int jj = 0; jj < nb; ++jj) {
                     const Row* row = rows + (size_t)jj * P.rowlen;
-                    bool any_near = false;  // some slot within eps of a threshold (rare)
+                    bool any_near = false;  // some slot within eps of a threshold (rare; wave-uniform)
                     // interpolators that jump at fraction 0.5 have a near lane in
                     // most traces: the slots (bit p * MW + m) holding one, so
                     // that only those are recomputed
@@ -1385,17 +1465,20 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     #pragma unroll
                         for (int m = 0; m < MW; ++m) {
                             xt[m] = fmaf(e.z, ci[p][m], P.taper_off);
-                            blk[m] = wave_any(xt[m] <= 1.0f + P.eps_x);
+                            // !(x > th), not x <= th: a predicate of its own, so that the
+                            // compiler does not carry this compare's lane mask into the
+                            // slot's block (through a VGPR) in place of in_ap's compare
+                            blk[m] = wave_any(!(xt[m] > ((COUNT && !learned_tt(TT)) ? 1.0f + P.eps_x : 1.0f)));
                             any_in |= blk[m];
                         }
                         if (!any_in) continue;  // time-dependent aperture excludes every slot
                         if (learned_tt(TT)) {
-                            const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * P.lat_nodes;
+                            const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * nn;
     #pragma unroll
                             for (int m = 0; m < MW; ++m) {
                                 const float2 c0 = lp[nidx[m]], c1 = lp[nidx[m] + 1];
-                                e2[p][m] = fmaf(nfr, c1.x - c0.x, c0.x);
-                                e1[p][m] = fmaf(nfr, c1.y - c0.y, c0.y);
+                                e2[p][m] = fmaf(nfr[m], c1.x - c0.x, c0.x);
+                                e1[p][m] = fmaf(nfr[m], c1.y - c0.y, c0.y);
                             }
                         }
                         // one pass per sample block, so that only one block's
@@ -1405,24 +1488,43 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             if (!blk[m]) continue;
                             float y1, y2;
                             const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m],
-                                                                 y1, y2);
+                                                                 etar[p][m], y1, y2);
                             const float u = fmaf(t, P.idt, uoff_v);
                             // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
                             const bool in_ap = xt[m] <= ((COUNT && !learned_tt(TT)) ? 1.0f - P.eps_x : 1.0f);
                             const bool below = u < P.umax_lo;
-                            bool near = in_ap && !below && u < P.umax_hi;
-                            if (LOB) near = near || (in_ap && u < P.lo_eps && u > -P.lo_eps);
-                            if (COUNT) near = near || (fabsf(xt[m] - 1.0f) < P.eps_x);
+                            // thresholds that can hold lanes below umax - eps
+                            bool near_o = false;
+                            if (LOB) near_o = near_o || (in_ap && u < P.lo_eps && u > -P.lo_eps);
+                            if (COUNT) near_o = near_o || (fabsf(xt[m] - 1.0f) < P.eps_x);
                             // nearest sample and windowed sinc jump at fraction 0.5 (the
                             // sample / the tap window moves by one): decided in f64 there
                             if (kHalfJump<INTERP>)
-                                near = near || (in_ap && below &&
-                                                 fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < fmaf(u, P.eps_half, 1e-5f));
-                            // the learned time has no f64 counterpart to decide against
-                            if (learned_tt(TT)) near = false;
-                            any_near |= near;
-                            if constexpr (kHalfJump<INTERP>) near_slots |= wave_any(near) ? 1u << (p * MW + m) : 0u;
-                            bool ok = in_ap && (learned_tt(TT) ? u < P.umax : below) && !near;
+                                near_o = near_o || (in_ap && below &&
+                                                     fabsf(__builtin_amdgcn_fractf(u) - 0.5f) < fmaf(u, P.eps_half, 1e-5f));
+                            // some lane of the slot near a threshold (wave-uniform): the
+                            // trace-end band [umax - eps, umax + eps) inside the aperture,
+                            // a scalar test on the three compare masks, or near_o; the
+                            // learned time has no f64 counterpart to decide against
+                            bool near_w = false;
+                            if (!learned_tt(TT)) {
+                                // (!(u < lo) as a compare of its own: the complement of below's
+                                // mask would be built through a VGPR)
+                                unsigned long long band =
+                                    wave_mask(in_ap) & wave_mask(!(u < P.umax_lo)) & wave_mask(u < P.umax_hi);
+                                // taken here, while the masks are in scalar registers (sunk
+                                // past the taper branch, below's mask would be carried
+                                // across it through a VGPR: two more VALU operations)
+                                asm volatile("" : "+s"(band));
+                                near_w = band != 0;
+                                if (LOB || COUNT || kHalfJump<INTERP>) near_w = near_w || wave_any(near_o);
+                            }
+                            any_near = any_near || near_w;
+                            if constexpr (kHalfJump<INTERP>) near_slots |= near_w ? 1u << (p * MW + m) : 0u;
+                            // band lanes are never below umax - eps: only near_o takes lanes
+                            // out of the ones below
+                            bool ok = in_ap && (learned_tt(TT) ? u < P.umax : below);
+                            if (LOB || COUNT || kHalfJump<INTERP>) ok = ok && !near_o;
                             if (LOB) ok = ok && u >= 0.0f;
                             const float us = ok ? u : 0.0f;
                             const int i0 = (int)us;
@@ -1436,8 +1538,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             // taper zone 0 < x_t <= 1 on some lane (wave-uniform test: the
                             // ballots of two plain compares, ANDed as masks, stay scalar)
                             if ((wave_mask(xt[m] > 0.0f) & wave_mask(xt[m] <= 1.0f)) != 0) {
-                                // a select, not a branch: the polynomial runs for the
-                                // whole wave either way, without exec-mask juggling
+                                // kept a branch (most waves skip it): the empty asm stops
+                                // the compiler from computing the taper for every wave
+                                asm volatile("");
+                                // a select inside: the taper runs for the whole wave,
+                                // without exec-mask juggling
                                 const float tw = taper_weight(__builtin_amdgcn_fmed3f(xt[m], 0.0f, 1.0f));
                                 w = xt[m] > 0.0f ? w * tw : w;
                             }
@@ -1446,10 +1551,13 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             else if constexpr (INTERP == PSTM_INTERP_LINEAR) amp = lin_finish(tl, f0);
                             else amp = interp_at<INTERP>(row, i0, f0, P.ns);
                             if (W2 || P.wpow != 0) {
+                                // spreading / obliquity: cw / t^wpow; cw, constant over the
+                                // traces, is applied to the slot's total when the item ends,
+                                // except in early blocks, where t <= 1 ms switches the factor off
                                 const float r = frcp(t);
-                                float wf = cw[p][m] * ((W2 || P.wpow == 2) ? r * r : r);
+                                float wf = (W2 || P.wpow == 2) ? r * r : r;
                                 // t > 1 ms is certain unless the block holds t0 <= 2 ms (early)
-                                if (EARLY && (early & (1u << m))) wf = t > 0.001f ? wf : 1.0f;
+                                if (EARLY && (early & (1u << m))) wf = t > 0.001f ? cw_at(p, m) * wf : 1.0f;
                                 w *= wf;
                             }
                             acc[p][m] = fmaf(amp, w, acc[p][m]);
@@ -1472,7 +1580,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                 const float xt = fmaf(e.z, ci[p][m], P.taper_off);
                                 float y1, y2;
                                 const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m],
-                                                                     t0r[m], y1, y2);
+                                                                     t0r[m], etar[p][m], y1, y2);
                                 const float u = fmaf(t, P.idt, P.uoff);
                                 const bool in_ap = xt <= (COUNT ? 1.0f - P.eps_x : 1.0f);
                                 bool near = in_ap && u >= P.umax_lo && u < P.umax_hi;
@@ -1490,8 +1598,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                     // the nearest sample is rounded in f64: (int)(u + 0.5)
                                     const int i0 = INTERP == PSTM_INTERP_NEAREST ? (int)(u64 + 0.5) : (int)u64;
                                     const float f0 = INTERP == PSTM_INTERP_NEAREST ? 0.0f : (float)(u64 - (double)i0);
+                                    // cw in early blocks only (applied to the totals elsewhere)
+                                    const float cwe = (early & (1u << m)) ? cw_at(p, m) : 1.0f;
                                     acc[p][m] = fmaf(contrib_amp<AA, INTERP>(P, row, i0, f0, f, cs[p][m], y1, y2, s_invsq),
-                                                     contrib_weight(P, e.w, xt, t, cw[p][m]), acc[p][m]);
+                                                     contrib_weight(P, e.w, xt, t, cwe), acc[p][m]);
                                     if (COUNT) ++cnt;
                                 }
                             }
@@ -1524,8 +1634,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             for (int p = 0; p < kPW; ++p) {
                 const int pil = pil0 + pg * kPW + p;
                 if (pil < npil) {
+                    // the total times cw (early blocks carry it per contribution)
+                    const double cwb = (early & (1u << m)) ? 1.0 : (double)cw_at(p, m);
                     float* dst = T.image + (size_t)pil * P.nt + k;
-                    *dst = *dst + (float)tot[(p * MW + m) * kThreads + tid];
+                    *dst = *dst + (float)(tot[(p * MW + m) * kThreads + tid] * cwb);
                 }
             }
         }
@@ -1660,7 +1772,12 @@ int validate(const PstmMigrateArgs* a) {
     if (a->traveltime == PSTM_TT_VTI_LEARNED || a->traveltime == PSTM_TT_CURVED_LEARNED) {
         const int K = a->mlp_node_spacing;
         if (!a->mlp_params) return fail(1, "mlp_params must be set for the learned traveltime");
-        if (K != 8 && K != 16 && K != 32 && K != 64) return fail(1, "mlp_node_spacing must be 8, 16, 32 or 64");
+        if (K < 4 || K > 512) return fail(1, "mlp_node_spacing must be in [4, 512]");
+        if (a->mlp_node_growth < 0 || a->mlp_node_growth > 1024)
+            return fail(1, "mlp_node_growth must be in [0, 1024]");
+        if (a->mlp_node_max != 0 && (a->mlp_node_max < K || a->mlp_node_max > 1024))
+            return fail(1, "mlp_node_max must be 0 or in [mlp_node_spacing, 1024]");
+        if (a->nt < 2) return fail(1, "the learned traveltime needs at least 2 output samples");
         if (a->mlp_weight_terms < 0 || a->mlp_weight_terms > 2) return fail(1, "mlp_weight_terms must be 0, 1 or 2");
         if (!(a->mlp_tau_max > 0.0) || !(a->mlp_v_hi > a->mlp_v_lo) || !(a->mlp_eta_max > 0.0) ||
             !(a->mlp_out_scale > 0.0))
@@ -1761,6 +1878,29 @@ __global__ __launch_bounds__(256) void k_fold_add(const TileView* tiles) {
 struct Shape;
 bool lo_check(const PstmMigrateArgs& a, const Shape& sh);
 
+// learned traveltime: the axis lattice (lattice_next) and the largest node
+// count of one time window of `win` samples (the kernel's jf .. jl)
+std::vector<int> lattice_positions(const PstmMigrateArgs& a) {
+    const int kmax = a.mlp_node_max > 0 ? a.mlp_node_max : a.mlp_node_spacing;
+    std::vector<int> pos{0};
+    while (pos.back() < a.nt - 1)
+        pos.push_back(lattice_next(pos.back(), a.mlp_node_spacing, a.mlp_node_growth, kmax, a.nt - 1));
+    return pos;
+}
+
+int lattice_window_nodes(const std::vector<int>& pos, int nt, int win) {
+    const int total = (int)pos.size();
+    auto upper = [&](int k) { return (int)(std::upper_bound(pos.begin(), pos.end(), k) - pos.begin()); };
+    int most = 0;
+    for (int kb = 0; kb < nt; kb += win) {
+        const int kend = std::min(kb + win, nt);
+        const int jf = std::min(upper(kb) - 1, total - 2);
+        const int jl = std::max(upper(kend - 2), jf + 1);
+        most = std::max(most, jl - jf + 1);
+    }
+    return most;
+}
+
 // Launch shape and LDS layout shared by every tile of a call or plan.
 struct Shape {
     int tt = 0;                  // traveltime evaluated (curved with |k| < 1e-10 -> straight)
@@ -1768,6 +1908,8 @@ struct Shape {
     int lmax = 0, lo = 0, rowlen = 0, pair_vec = 1;
     size_t elem = 4;
     int mw = 1, batch = 1, lat_nodes = 0, win = 256, nwin = 1;
+    int lat_total = 0;           // learned: lattice nodes over the output axis
+    int mlp_terms = 1;           // learned: bf16 terms per weight
     size_t tot_bytes = 0, lds = 0;
     int occupancy = 1;
     bool pipe = false;
@@ -1796,9 +1938,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     sh.mw = std::min(mw_cap, std::max(1, (per_win + 255) / 256));
     sh.interp = a.interpolation;
     if (sh.interp != PSTM_INTERP_LINEAR) sh.mw = interp_mw(sh.interp);  // their one instantiation
-    // batch size from the LDS budget: trace rows + pair table + f64 totals (+ lattice)
-    sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * sizeof(double);
-    sh.lat_nodes = sh.learned ? 256 * sh.mw / a.mlp_node_spacing + 1 : 0;
+    sh.mlp_terms = sh.learned && a.mlp_weight_terms == 2 ? 2 : 1;
     // pipelined staging: f32 rows through kPF registers per thread (the batch's
     // rows must fit), AA rows by direct global -> LDS loads into a second row
     // buffer (LDS holds two buffers, so the batch shrinks); automatic: on for
@@ -1810,14 +1950,31 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     const int pipe_batch = sh.aa ? kMaxBatch : kPF * kThreads / rl_dw;
     sh.pipe = pipe == 1 && pipe_batch >= 1;
     const int nbuf = (sh.pipe && sh.aa) ? 2 : 1;
+    // LDS: trace rows + pair table + f64 totals; learned: + lattice, widest
+    // apertures, row list, node inputs, network operands (kept for the
+    // workgroup's life)
     auto lds_for = [&](int b) {
         return align_up((size_t)nbuf * align_up((size_t)b * sh.rowlen * sh.elem, 16) +
                             (size_t)kNPil * b * sh.pair_vec * 16, 16) +
                sh.tot_bytes + align_up((size_t)kNPil * b * sh.lat_nodes * 8, 16) +
                (sh.learned ? align_up((size_t)kNPil * sh.lat_nodes * 4, 16) +
-                                 align_up((size_t)kNPil * b * sh.lat_nodes * 2, 16)
+                                 align_up((size_t)kNPil * b * sh.lat_nodes * 2, 16) +
+                                 (size_t)kNPil * sh.lat_nodes * 16 + (size_t)sh.mlp_terms * kAVec8 * 16 +
+                                 (size_t)kBVals * 4 + align_up((size_t)sh.lat_total * 4, 16)
                            : 0);
     };
+    std::vector<int> npos;
+    if (sh.learned) {
+        npos = lattice_positions(a);
+        sh.lat_total = (int)npos.size();
+    }
+    // sample blocks per wave: the learned layout must fit one trace per batch
+    for (;;) {
+        sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * sizeof(double);
+        sh.lat_nodes = sh.learned ? lattice_window_nodes(npos, nt, 256 * sh.mw) : 0;
+        if (sh.mw == 1 || lds_for(1) <= kLdsBudget) break;
+        --sh.mw;
+    }
     int batch_cap = g_tune.batch_cap > 0 ? g_tune.batch_cap : env_int("PSTM_HIP_BATCH");
     sh.batch = batch_cap > 0 ? std::min(batch_cap, kMaxBatch) : kMaxBatch;
     if (sh.pipe) sh.batch = std::min(sh.batch, pipe_batch);
@@ -1897,8 +2054,13 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.lat_nodes = sh.lat_nodes;
     kp.lat_cimin_off = kp.lat_off + (int)align_up((size_t)kNPil * sh.batch * sh.lat_nodes * 8, 16);
     kp.lat_list_off = kp.lat_cimin_off + (int)align_up((size_t)kNPil * sh.lat_nodes * 4, 16);
-    kp.lat_shift = sh.learned ? __builtin_ctz((unsigned)a.mlp_node_spacing) : 0;
-    kp.lat_ifrac = sh.learned ? 1.0f / (float)a.mlp_node_spacing : 0.0f;
+    kp.node_off = kp.lat_list_off + (int)align_up((size_t)kNPil * sh.batch * sh.lat_nodes * 2, 16);
+    kp.mlp_off = kp.node_off + kNPil * sh.lat_nodes * 16;
+    kp.lat_pos_off = kp.mlp_off + (int)((size_t)sh.mlp_terms * kAVec8 * 16 + (size_t)kBVals * 4);
+    kp.lat_total = sh.lat_total;
+    kp.lat_kmin = a.mlp_node_spacing;
+    kp.lat_grow = a.mlp_node_growth;
+    kp.lat_kmax = a.mlp_node_max > 0 ? a.mlp_node_max : a.mlp_node_spacing;
     kp.m_tau_sc = sh.learned ? (float)(2.0 / a.mlp_tau_max) : 0.0f;
     kp.m_v_lo = (float)a.mlp_v_lo;
     kp.m_v_sc = sh.learned ? (float)(2.0 / (a.mlp_v_hi - a.mlp_v_lo)) : 0.0f;
@@ -2052,7 +2214,7 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     const size_t o_qk = off; off = align_up(off + (size_t)nt * 4, 256);
     const size_t o_t0 = off; off = align_up(off + (size_t)nt * 4, 256);
     const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kAVals * 4 : 0), 256);
-    const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBNum * 64 : 0), 256);
+    const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBVals * 4 : 0), 256);
     const size_t o_cnt = off; off = align_up(off + (size_t)grid * 8, 256);
     const size_t o_q = off; off = align_up(off + 8, 256);
     const size_t o_view = off; off = align_up(off + sizeof(TileView), 256);
@@ -2079,7 +2241,7 @@ int run_migration(const PstmMigrateArgs* in, int row0, int ntr, int do_fold, hip
     rc = prep_tile(a, sh, tan_dip, tb, qk, t0k, st);
     if (rc) return rc;
     if (sh.learned) {
-        const int n = kAVals + kBNum * 16;
+        const int n = kAVals + kBVals;
         hipLaunchKernelGGL(k_prep_mlp, dim3((n + 255) / 256), dim3(256), 0, st, a.mlp_params,
                            reinterpret_cast<unsigned short*>(ws + o_mlpA), reinterpret_cast<float*>(ws + o_mlpB));
         HIP_TRY(hipGetLastError());
@@ -2168,7 +2330,7 @@ int plan_enqueue(PstmPlan* pl, hipStream_t st) {
     const PstmMigrateArgs& a = pl->common;
     const Shape& sh = pl->sh;
     if (sh.learned) {
-        const int n = kAVals + kBNum * 16;
+        const int n = kAVals + kBVals;
         hipLaunchKernelGGL(k_prep_mlp, dim3((n + 255) / 256), dim3(256), 0, st, a.mlp_params,
                            static_cast<unsigned short*>(pl->mlpA), static_cast<float*>(pl->mlpB));
         HIP_TRY(hipGetLastError());
@@ -2311,7 +2473,7 @@ int pstm_hip_plan_create(const PstmMigrateArgs* common, const PstmTileDesc* tile
     const size_t o_cnt = off; off = align_up(off + (size_t)pl->grid * 8, 256);
     const size_t o_q = off; off = align_up(off + 8, 256);
     const size_t o_mlpA = off; off = align_up(off + (sh.learned ? (size_t)kAVals * 4 : 0), 256);
-    const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBNum * 64 : 0), 256);
+    const size_t o_mlpB = off; off = align_up(off + (sh.learned ? (size_t)kBVals * 4 : 0), 256);
     const size_t o_prow = off; off = align_up(off + (sh.aa ? (size_t)pl->block_rows * sh.rowlen * 8 : 0), 256);
     hipError_t e = hipMalloc(&pl->mem, off);
     if (e != hipSuccess) {
@@ -2582,11 +2744,11 @@ int pstm_mlp_forward_mfma_terms(const float* params, const float* x, int n, floa
     if (n == 0) return 0;
     int rc = ensure_init();
     if (rc) return rc;
-    rc = ensure_workspace((size_t)kAVals * 4 + (size_t)kBNum * 64);
+    rc = ensure_workspace((size_t)kAVals * 4 + (size_t)kBVals * 4);
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(g_state.ws);
-    const int np = kAVals + kBNum * 16;
+    const int np = kAVals + kBVals;
     hipLaunchKernelGGL(k_prep_mlp, dim3((np + 255) / 256), dim3(256), 0, st, params,
                        reinterpret_cast<unsigned short*>(ws), reinterpret_cast<float*>(ws + (size_t)kAVals * 4));
     HIP_TRY(hipGetLastError());

@@ -13,7 +13,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 TT_STRAIGHT, TT_CURVED, TT_VTI, TT_VTI_LEARNED, TT_CURVED_LEARNED = 0, 1, 2, 3, 4
 AA_NONE, AA_TRIANGLE = 0, 1
@@ -71,7 +71,8 @@ class PstmMigrateArgs(ctypes.Structure):
         ("mlp_params", _vp), ("mlp_node_spacing", ctypes.c_int32), ("interpolation", ctypes.c_int32),
         ("mlp_tau_max", ctypes.c_double), ("mlp_v_lo", ctypes.c_double), ("mlp_v_hi", ctypes.c_double),
         ("mlp_eta_max", ctypes.c_double), ("mlp_out_scale", ctypes.c_double),
-        ("mlp_weight_terms", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("mlp_weight_terms", ctypes.c_int32), ("mlp_node_growth", ctypes.c_int32),
+        ("mlp_node_max", ctypes.c_int32), ("reserved0", ctypes.c_int32),
     ]
 
 

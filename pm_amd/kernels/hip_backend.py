@@ -25,6 +25,7 @@ from typing import Any
 import numpy as np
 
 from pm_amd.kernels import _abi
+from pm_amd.lattice import as_lattice
 from pm_amd.kernels.base import (
     KernelCapability,
     KernelConfig,
@@ -209,15 +210,15 @@ class HipKernel:
                        vrms, config: KernelConfig, *, traveltime: int | None = None,
                        trace_index=None, eta=None, count_effective: bool = False,
                        group_start: np.ndarray | None = None, surrogate=None,
-                       node_spacing: int = 16) -> int | None:
+                       node_spacing=16) -> int | None:
         """Accumulate one tile into device tensors ``image`` (nx, ny, nt) f32
         and ``fold2d`` (nx, ny) i32.  All array arguments are device tensors
         (vrms (nt,) or (nx, ny, nt) f64).  Asynchronous on the current stream
         unless ``count_effective`` is set, in which case the effective
         contribution count E is returned.  ``traveltime=TT_VTI_LEARNED`` needs
         ``surrogate`` (a trained pm_amd.surrogate.TraveltimeSurrogate) and
-        evaluates it on a t0 lattice of ``node_spacing`` output samples inside
-        the gather."""
+        evaluates it on a t0 lattice inside the gather: ``node_spacing`` is a
+        pm_amd.lattice.Lattice or an int (uniform spacing in output samples)."""
         self._require_init()
         torch = _torch()
         lib = self._lib
@@ -241,7 +242,7 @@ class HipKernel:
 
     def create_plan(self, dt: DeviceTraces, tiles: list[dict], t_axis_ms, config: KernelConfig, *,
                     traveltime: int | None = None, count_effective: bool = False, surrogate=None,
-                    node_spacing: int = 16) -> "HipPlan":
+                    node_spacing=16) -> "HipPlan":
         """A device-resident work queue over many output tiles: one persistent,
         hipGraph-captured gather per run (``HipPlan.run``).  Each tile is a
         dict with image, fold (device tensors, accumulated), x_axis, y_axis,
@@ -417,7 +418,8 @@ class HipKernel:
             n = surrogate.norm
             keep.append(surrogate.params)
             a.mlp_params = _ptr(surrogate.params)
-            a.mlp_node_spacing = int(node_spacing)
+            lat = as_lattice(node_spacing)
+            a.mlp_node_spacing, a.mlp_node_growth, a.mlp_node_max = lat.spacing, lat.growth, lat.max_spacing
             a.mlp_tau_max, a.mlp_v_lo, a.mlp_v_hi = float(n.tau_max), float(n.v_lo), float(n.v_hi)
             a.mlp_eta_max, a.mlp_out_scale = float(n.eta_max), float(n.out_scale)
             a.mlp_weight_terms = int(getattr(surrogate, "weight_terms", 1))

@@ -61,7 +61,8 @@ class Normalizer:
     v_lo: float
     v_hi: float
     eta_max: float
-    out_scale: float = 10.0   # network outputs are out_scale * (c_s, c_r)
+    out_scale: float = 1.5    # network outputs are out_scale * the two legs' factors
+                              # (VTI: g = c / eta in [-2, 0]; curved ray: c)
 
     def encode(self, tau, d_s, d_r, v, eta):
         """Raw inputs (torch, any float dtype) -> (n, 5) f32 network inputs."""
@@ -145,7 +146,9 @@ def make_training_set(vcube, ecube, n: int, d_max: float, norm: Normalizer, seed
                       device="cuda", t_max_ms: float | None = None) -> TrainingSet:
     """n samples on the job's own model: x, y uniform over the cube's extent,
     t0 uniform over [0, t_max], d_s, d_r uniform over [0, d_max]; v and eta by
-    trilinear interpolation of the cubes."""
+    trilinear interpolation of the cubes.  Targets out_scale * g per leg, g =
+    c / eta (anisotropic_factor per unit eta); samples with eta < 1e-6 are
+    left out (fewer than n where the model reaches eta = 0)."""
     import torch
 
     rng = np.random.default_rng(seed)
@@ -162,9 +165,13 @@ def make_training_set(vcube, ecube, n: int, d_max: float, norm: Normalizer, seed
     drt = torch.as_tensor(dr, device=device)
     t_s = vti_leg_times(tau, dst, v, e)
     t_r = vti_leg_times(tau, drt, v, e)
+    # targets g = c / eta; where eta < 1e-6 the time does not depend on g
+    # (and c / eta is not defined at eta = 0): those samples are left out
+    keep = e >= 1e-6
+    tau, dst, drt, v, e, t_s, t_r = (a[keep] for a in (tau, dst, drt, v, e, t_s, t_r))
     xr = torch.stack([tau, dst, drt, v, e], 1)
-    c = torch.stack([anisotropic_factor(t_s, tau, dst, v, e), anisotropic_factor(t_r, tau, drt, v, e)], 1)
-    y = (c * norm.out_scale).float().contiguous()
+    g = torch.stack([anisotropic_factor(t_s, tau, dst, v, e), anisotropic_factor(t_r, tau, drt, v, e)], 1) / e[:, None]
+    y = (g * norm.out_scale).float().contiguous()
     return TrainingSet(xr, norm.apply(xr), y, torch.stack([t_s, t_r], 1))
 
 
@@ -446,7 +453,10 @@ class TraveltimeSurrogate:
                 return z / (v * v + (kg * d) ** 2).sqrt() * (1.0 + cc).clamp_min(0.0)
 
             return torch.stack([leg(ds, c[:, 0]), leg(dr, c[:, 1])], 1)
-        return torch.stack([learned_leg_time(tau, ds, v, c[:, 0]), learned_leg_time(tau, dr, v, c[:, 1])], 1)
+        # VTI: the outputs are g = c / eta
+        eta = ts.x_raw[:, 4]
+        return torch.stack([learned_leg_time(tau, ds, v, eta * c[:, 0]), learned_leg_time(tau, dr, v, eta * c[:, 1])],
+                           1)
 
     def error_samples(self, ts: TrainingSet, dt_ms: float, bf16: bool = False) -> dict:
         """Error of t_s + t_r against the exact sum, in input samples."""
@@ -463,33 +473,34 @@ class TraveltimeSurrogate:
         return self.params.to(torch.bfloat16)
 
 
-def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, node_spacing: int):
+def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, node_spacing):
     """Learned two-leg time of output samples k as the gather forms it: the
-    network (the fused inference's own code) on the lattice samples
-    k0 = K floor(k / K) and
-    k0 + K (times past the axis extrapolated from its last step, v and eta
-    taken at the last sample), c interpolated linearly, then
-    sum of sqrt(tau^2 + (d / v)^2 max(1 + c, 0)); t = t0 where t0 <= 0.
+    network (the fused inference's own code) on the lattice nodes n_j <= k <
+    n_{j+1} around each sample (pm_amd.lattice; ``node_spacing`` a Lattice or
+    an int, the uniform spacing), its outputs interpolated linearly, then the
+    sum of the legs: VTI sqrt(tau^2 + (d / v)^2 max(1 + eta g, 0)) with g the
+    interpolated output and v, eta at the sample (t = t0 where t0 <= 0);
+    curved ray z / sqrt(vg^2 + k^2 d^2) max(1 + c, 0).
 
     tau_axis: (nt,) one-way times t0 / 2 (f64); k: (n,) int64 sample indices;
     ds, dr: (n,) leg distances; v_at(kk), eta_at(kk): (n,) f64 values at the
     sample's own pillar and samples kk."""
     import torch
 
+    from pm_amd.lattice import as_lattice
+
     nt = int(tau_axis.shape[0])
-    K = int(node_spacing)
-    dtau = (tau_axis[-1] - tau_axis[-2]) if nt > 1 else tau_axis.new_tensor(0.0)
-    k0 = torch.div(k, K, rounding_mode="floor") * K
-    frac = ((k - k0).double() / K)[:, None]
+    nodes = torch.as_tensor(as_lattice(node_spacing).nodes(nt), device=k.device)
+    j = (torch.searchsorted(nodes, k, right=True) - 1).clamp(0, nodes.shape[0] - 2)
+    k0, k1 = nodes[j], nodes[j + 1]
+    frac = ((k - k0).double() / (k1 - k0).double())[:, None]
 
     def at_node(kn):
-        kk = kn.clamp(max=nt - 1)
-        tau = tau_axis[kk] + (kn - kk).double() * dtau
         enc = sur.norm.encode_curved if sur.kind == "curved" else sur.norm.encode
-        x = enc(tau, ds, dr, v_at(kk), eta_at(kk))
+        x = enc(tau_axis[kn], ds, dr, v_at(kn), eta_at(kn))
         return sur.forward_fused(x).double() / sur.norm.out_scale
 
-    c0, c1 = at_node(k0), at_node(k0 + K)
+    c0, c1 = at_node(k0), at_node(k1)
     c = c0 + frac * (c1 - c0)
     tau = tau_axis[k]
     v = v_at(k)
@@ -500,5 +511,7 @@ def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, n
         z = (vg * vg / float(sur.curved_v0) - float(sur.curved_v0)) / kg
         leg = lambda d, cc: z / (vg * vg + (kg * d) ** 2).sqrt() * (1.0 + cc).clamp_min(0.0)  # noqa: E731
         return leg(ds, c[:, 0]) + leg(dr, c[:, 1])
-    t = learned_leg_time(tau, ds, v, c[:, 0]) + learned_leg_time(tau, dr, v, c[:, 1])
+    # VTI: the lattice carries g = c / eta; the sample's own eta makes c
+    eta = eta_at(k)
+    t = learned_leg_time(tau, ds, v, eta * c[:, 0]) + learned_leg_time(tau, dr, v, eta * c[:, 1])
     return torch.where(tau > 0, t, 2.0 * tau)

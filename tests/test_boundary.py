@@ -157,9 +157,16 @@ def test_invalid_arguments_are_rejected_without_device_work():
     a.traveltime = _abi.TT_VTI_LEARNED
     assert lib.pstm_hip_migrate_tile(ctypes.byref(a), None) == 1
     assert "mlp_params" in _abi.last_error(lib)
-    a.mlp_params, a.mlp_node_spacing = 8, 12
+    a.mlp_params, a.mlp_node_spacing = 8, 2
     assert lib.pstm_hip_migrate_tile(ctypes.byref(a), None) == 1
     assert "mlp_node_spacing" in _abi.last_error(lib)
+    a.mlp_node_spacing, a.mlp_node_growth = 8, 2000
+    assert lib.pstm_hip_migrate_tile(ctypes.byref(a), None) == 1
+    assert "mlp_node_growth" in _abi.last_error(lib)
+    a.mlp_node_growth, a.mlp_node_max = 0, 4
+    assert lib.pstm_hip_migrate_tile(ctypes.byref(a), None) == 1
+    assert "mlp_node_max" in _abi.last_error(lib)
+    a.mlp_node_max = 0
     # plans, tuning knobs
     h = ctypes.c_void_p()
     assert lib.pstm_hip_plan_create(None, None, 0, ctypes.byref(h)) == 1

@@ -17,8 +17,9 @@ F32_FLOOR_*; the 1e-5 bar holds at 1001 samples, tests/test_gpu_parity.py).
   output (the whole job resident: traces, velocity and eta volumes, image,
   fold and every tile's trace list); migrated: the stated central region of
   32 x 32 pillars, exact VTI against the oracle and the learned traveltime
-  (f32-trained network, split bf16 weights) against exact to image relative
-  L2 1e-3, with its per-tile held-out check.
+  (f32-trained network, bf16 weights, the growing t0 lattice of the
+  benchmark, pm_amd.lattice.GEOMETRIC) against exact to image relative L2
+  1e-3, with its per-tile held-out check.
 
 Progress lines name the job and step (each job takes tens of seconds).
 """
@@ -180,6 +181,7 @@ def test_vti_full_size_region(kernel):
     import torch
 
     from pm_amd.autotune import region_tiles
+    from pm_amd.lattice import GEOMETRIC
     from pm_amd.pipeline.executor import TileExecutor
     from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set
     from pm_amd.synthetic import vti_diffractor_job
@@ -201,15 +203,15 @@ def test_vti_full_size_region(kernel):
     norm = Normalizer(tau_max=1.5, d_max=d_max, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
     tr = make_training_set(job.cube, job.meta["eta_cube"], 10_000_000, d_max, norm, seed=1, t_max_ms=3000.0)
     sur = TraveltimeSurrogate.create(norm, seed=0)
-    sur.weight_terms = 2
+    sur.weight_terms = 1
     t0 = time.perf_counter()
     sur.train(tr, steps=100_000, batch=65536)
     torch.cuda.synchronize()
     say(f"vti: surrogate trained in {time.perf_counter() - t0:.1f} s")
     del tr
-    errs = ex.check_surrogate(sur, node_spacing=16, tiles=sub)
+    errs = ex.check_surrogate(sur, node_spacing=GEOMETRIC, tiles=sub)
     assert errs[0] <= 0.25
-    dt = _run_plan(ex, tiles=sub, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=16)
+    dt = _run_plan(ex, tiles=sub, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=GEOMETRIC)
     learned = ex.tiles[sub[0]].image.double()
     rl = float(torch.linalg.norm(learned - exact) / torch.linalg.norm(exact))
     say(f"vti: learned traveltime in {dt:.1f} s, held-out {errs[0]:.3f} sample, image rel-L2 vs exact {rl:.3e}")

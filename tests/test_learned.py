@@ -21,6 +21,7 @@ import numpy as np
 import pytest
 
 from pm_amd.kernels import _abi
+from pm_amd.lattice import Lattice
 
 pytestmark = pytest.mark.gpu
 
@@ -120,7 +121,8 @@ def test_exact_vti_readback(kernel):
     _compare(_readback(kernel, c, _abi.TT_VTI), _expected(c, False), 2e-3)
 
 
-@pytest.mark.parametrize("spacing,terms", [(8, 1), (16, 1), (64, 1), (16, 2)])
+@pytest.mark.parametrize("spacing,terms", [(8, 1), (16, 1), (64, 1), (16, 2), (12, 1),
+                                           (Lattice(8, 77, 64), 2), (Lattice(4, 256, 1024), 1)])
 def test_learned_readback(kernel, spacing, terms):
     """The fused gather's learned times equal the restated lattice inference
     (an untrained network: the check is of the arithmetic, not of accuracy),
@@ -146,8 +148,8 @@ def test_learned_rejects_bad_arguments(kernel):
 
     c = _ramp_case(3)
     sur = TraveltimeSurrogate.create(Normalizer(1.0, 2000.0, 1500.0, 4500.0, 0.2))
-    with pytest.raises(ValueError, match="mlp_node_spacing"):
-        _readback(kernel, c, _abi.TT_VTI_LEARNED, sur, node_spacing=12)
+    with pytest.raises(ValueError, match="node spacing"):
+        _readback(kernel, c, _abi.TT_VTI_LEARNED, sur, node_spacing=2)
     with pytest.raises(ValueError, match="surrogate"):
         _readback(kernel, c, _abi.TT_VTI_LEARNED, None)
 

@@ -26,6 +26,7 @@ import numpy as np
 import pytest
 
 from pm_amd.kernels import _abi
+from pm_amd.lattice import Lattice
 
 V0, KG = 1700.0, 0.55
 
@@ -108,7 +109,7 @@ def _ramp_config():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("spacing,terms", [(16, 1), (16, 2), (64, 2)])
+@pytest.mark.parametrize("spacing,terms", [(16, 1), (16, 2), (64, 2), (Lattice(8, 77, 64), 2)])
 def test_learned_curved_readback(kernel, spacing, terms):
     import torch
 

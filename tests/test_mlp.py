@@ -315,6 +315,11 @@ def test_anisotropic_factor_closed_form():
     c = anisotropic_factor(T(t), T(tau), T(d), T(v), T(eta)).numpy()
     s = d / (v * hyperbolic(T(tau), T(d), T(v)).numpy())
     np.testing.assert_allclose(c, -2 * eta * s ** 2 / (1 + 2 * eta * s ** 2), rtol=1e-7, atol=1e-10)
+    # the learned target g = c / eta: -2 s^2 / (1 + 2 eta s^2), in [-2, 0]
+    big = eta > 1e-3
+    g = c[big] / eta[big]
+    np.testing.assert_allclose(g, -2 * s[big] ** 2 / (1 + 2 * eta[big] * s[big] ** 2), rtol=1e-6, atol=1e-7)
+    assert (g <= 0).all() and (g >= -2).all()
     np.testing.assert_allclose(learned_leg_time(T(tau), T(d), T(v), T(c)).numpy(), t, rtol=1e-12)
     c0 = anisotropic_factor(T([0.0]), T([0.0]), T([300.0]), T([2000.0]), T([0.1])).numpy()
     np.testing.assert_allclose(c0, [-0.2 / 1.2])
