@@ -55,6 +55,11 @@ METRIC = ("Gcontrib/s (trace×output-sample), learned-TT vs exact-TT, image "
           "relL2≤1e-3")
 
 
+# False in profiler runs (--profile): every plan is launched directly, no
+# hipGraph capture or replay, and the surrogate trains step by step
+USE_GRAPH = True
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -94,7 +99,7 @@ def plan_launch_ms(lib, plans, reps: int = 1) -> float:
     lib.pstm_hip_profile_reset(1)
     for _ in range(reps):
         for p in plans:
-            p.run()
+            p.run(use_graph=USE_GRAPH)
     kms, nl = ctypes.c_double(), ctypes.c_longlong()
     lib.pstm_hip_profile_read(ctypes.byref(kms), ctypes.byref(nl))
     lib.pstm_hip_profile_reset(0)
@@ -194,11 +199,11 @@ def point_diffractor_plain(kernel, reps: int) -> dict:
     ex = TileExecutor(kernel, job)
     e = count_effective(ex, [0])
     plan = ex.plan()
-    plan.run()                   # capture
+    plan.run(use_graph=USE_GRAPH)  # capture
     torch.cuda.synchronize()
     ms = plan_launch_ms(lib, [plan], reps)
     ex.reset()
-    plan.run()
+    plan.run(use_graph=USE_GRAPH)
     torch.cuda.synchronize()
     c = ex.contributions()
     info = plan.info()
@@ -276,8 +281,12 @@ def vti_headline(kernel, args) -> dict:
     sur.weight_terms = args.weight_terms
     # an f32-trained network; the gather rounds its weights to bf16 (one
     # term) or splits them in two (--weight-terms 2)
-    sur.train(tr, steps=args.train_steps, batch=args.train_batch, log_every=max(args.train_steps // 8, 1),
-              verbose=True)
+    if USE_GRAPH:
+        sur.train(tr, steps=args.train_steps, batch=args.train_batch, log_every=max(args.train_steps // 8, 1),
+                  verbose=True)
+    else:
+        sur.train_stepwise(tr, steps=args.train_steps, batch=args.train_batch,
+                           log_every=max(args.train_steps // 8, 1), verbose=True)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     held = sur.error_samples(tr, job.dt_ms, bf16=True) if args.train_samples <= 2_000_000 else None
@@ -296,7 +305,7 @@ def vti_headline(kernel, args) -> dict:
     # exact VTI traveltime on the subset: E, one captured run, one timed run
     e_exact = count_effective(ex, sub)
     exact_plan = ex.plan(tiles=sub)
-    exact_plan.run()
+    exact_plan.run(use_graph=USE_GRAPH)
     torch.cuda.synchronize()
     reset_tiles(ex, sub)
     exact_ms = plan_launch_ms(lib, [exact_plan])
@@ -312,7 +321,7 @@ def vti_headline(kernel, args) -> dict:
     ex.config = replace(saved, kernel_type="straight_ray", vti_enabled=False)
     e_plain = count_effective(ex, sub)
     iso_plan = ex.plan(tiles=sub)
-    iso_plan.run()
+    iso_plan.run(use_graph=USE_GRAPH)
     torch.cuda.synchronize()
     reset_tiles(ex, sub)
     plain_ms = plan_launch_ms(lib, [iso_plan])
@@ -330,7 +339,7 @@ def vti_headline(kernel, args) -> dict:
         plans.append(ex.plan(tiles=learned, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=K))
     if fallback:
         plans.append(ex.plan(tiles=fallback))
-    use_graph = not args.profile
+    use_graph = USE_GRAPH
     for _ in range(args.warmup):
         reset_tiles(ex, sub)
         for p in plans:
@@ -366,7 +375,7 @@ def vti_headline(kernel, args) -> dict:
                             "k_migrate<VTI_LEARNED> (fused BF16-MFMA lattice inference + VALU gather)")
     elif learned:
         lp = ex.plan(tiles=learned, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=K)
-        lp.run()
+        lp.run(use_graph=USE_GRAPH)
         torch.cuda.synchronize()
         rl = roofline_entry(f_learned, e_learned, plan_launch_ms(lib, [lp]),
                             "k_migrate<VTI_LEARNED> (learned tiles only)")
@@ -708,7 +717,7 @@ def main() -> None:
     ap.add_argument("--weight-terms", type=int, default=1, choices=[1, 2],
                     help="fused inference weights of the f32-trained network: 1 rounded to bf16, 2 as two bf16 "
                          "terms (one more MFMA per product)")
-    ap.add_argument("--lattice", default="8,77,48",
+    ap.add_argument("--lattice", default="8,128,96",
                     help="t0 lattice of the learned traveltime: smallest spacing, growth (1/256), largest spacing")
     ap.add_argument("--force-learned", action="store_true",
                     help="use the learned traveltime on every subset tile whatever its held-out error "
@@ -740,6 +749,8 @@ def main() -> None:
     from pm_amd.kernels import KernelConfig
     from pm_amd.kernels import _abi
 
+    global USE_GRAPH
+    USE_GRAPH = not args.profile
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     kernel = HipKernel()
     kernel.initialize(KernelConfig())

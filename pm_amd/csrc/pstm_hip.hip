@@ -190,6 +190,7 @@ struct KParams {
     int lat_kmin, lat_grow, lat_kmax;  // node spacing law (lattice_next)
     int lat_total;               // lattice nodes over the whole output axis
     int lat_pos_off;             // byte offset of the node positions (output samples) in LDS
+    int coef_off;                // byte offset of the per-(pair, segment) Hermite coefficients in LDS
     float m_tau_sc, m_v_lo, m_v_sc, m_eta_sc, m_inv_out;
 };
 
@@ -631,6 +632,32 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x8_t* __rest
 }
 #endif  // PSTM_GATHER_PART
 
+// Derivative at node i (1 or 2) of values f[0..3] at node positions x[0..3]
+// (ok: the node's value is there): the three-point difference over i - 1,
+// i, i + 1 where both neighbours are there, else the one-sided three-point
+// difference over i, i -+ 1, i -+ 2, else the two-point one (second order:
+// exact for a quadratic).
+__device__ __forceinline__ float d3(float a, float b, float c, float xa, float xb, float xc) {
+    // derivative at xa of the parabola through (xa, a), (xb, b), (xc, c)
+    const float h1 = xb - xa, h2 = xc - xb;
+    return -(2.0f * h1 + h2) / (h1 * (h1 + h2)) * a + (h1 + h2) / (h1 * h2) * b - h1 / (h2 * (h1 + h2)) * c;
+}
+
+__device__ __forceinline__ float node_derivative(const float (&f)[4], const bool (&ok)[4], const float (&x)[4], int i) {
+    if (i == 1) {
+        if (ok[0]) {
+            const float h1 = x[1] - x[0], h2 = x[2] - x[1];
+            return (h1 * h1 * (f[2] - f[1]) + h2 * h2 * (f[1] - f[0])) / (h1 * h2 * (h1 + h2));
+        }
+        return ok[3] ? d3(f[1], f[2], f[3], x[1], x[2], x[3]) : (f[2] - f[1]) / (x[2] - x[1]);
+    }
+    if (ok[3]) {
+        const float h1 = x[2] - x[1], h2 = x[3] - x[2];
+        return (h1 * h1 * (f[3] - f[2]) + h2 * h2 * (f[2] - f[1])) / (h1 * h2 * (h1 + h2));
+    }
+    return ok[0] ? d3(f[2], f[1], f[0], x[2], x[1], x[0]) : (f[2] - f[1]) / (x[2] - x[1]);
+}
+
 // Traveltime of one contribution from the pair terms e/f (LDS) and the
 // (pillar, sample) terms.  y1/y2 return 1/t_src, 1/t_rec for anti-aliasing.
 template <int TT, bool AA>
@@ -956,6 +983,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     const int tw = wave % kTW;
     const int pg = wave / kTW;
     const int win = 64 * kTW * MW;
+    const bool pipe = !learned_tt(TT) && P.pipe;
     unsigned long long cnt = 0;
     __shared__ int s_item;
     __shared__ int s_row[kMaxBatch];  // block rows of the current batch
@@ -1012,13 +1040,21 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         const int kwin = li / T.n_pgroups;
         const int pil0 = pgroup * kNPil;
         const int kbase = kwin * win;
+        // learned traveltime: the network is evaluated at three pillars of the
+        // group, 0, p_mid and p_hi (the last valid one)
+        const int p_hi = min(kNPil - 1, npil - 1 - pil0);
+        const int p_mid = (p_hi + 1) >> 1;
 
         // register-resident per-(pillar, sample) terms
         float q[MW], cs[kPW][MW], ci[kPW][MW];
         float e2[kPW][MW], e1[kPW][MW], t0r[MW];
         float etar[kPW][MW];  // learned VTI: eta at (pillar, sample)
         float acc[kPW][MW];
-        double* tot = reinterpret_cast<double*>(smem + P.tot_off);  // [kPW*MW][kThreads], lane-private
+        // running totals across batches, lane-private in LDS: f64 for the
+        // exact traveltimes, f32 for the learned one (whose accuracy is that
+        // of the network)
+        double* tot = reinterpret_cast<double*>(smem + P.tot_off);  // [kPW*MW][kThreads]
+        float* totf = reinterpret_cast<float*>(smem + P.tot_off);
 #pragma unroll
         for (int m = 0; m < MW; ++m) {
             const int k = kbase + 64 * (tw + kTW * m) + lane;
@@ -1029,7 +1065,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             for (int p = 0; p < kPW; ++p) {
                 const int pil = pil0 + pg * kPW + p;
                 acc[p][m] = 0.0f;
-                tot[(p * MW + m) * kThreads + tid] = 0.0;
+                if (learned_tt(TT)) totf[(p * MW + m) * kThreads + tid] = 0.0f;
+                else tot[(p * MW + m) * kThreads + tid] = 0.0;
                 e2[p][m] = 0.0f;
                 e1[p][m] = 1.0f;
                 etar[p][m] = 0.0f;
@@ -1052,7 +1089,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         }
 
         // the weight constant kc.y of a slot: constant over the traces, so
-        // applied to the slot's f64 total when the item ends; only early blocks
+        // applied to the slot's total when the item ends; only early blocks
         // (below) need it per contribution, where it is read when used
         auto cw_at = [&](int p, int m) -> float {
             const int k = kbase + 64 * (tw + kTW * m) + lane;
@@ -1074,21 +1111,31 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         const int pt_p = tid / P.batch, pt_j = tid % P.batch;
         const bool pt_thread = tid < kNPil * P.batch;
 
-        // learned traveltime: the lattice sample below each slot, the lerp
-        // fraction, and per (pillar, node) the smallest taper slope (widest
-        // aperture) over the samples that read the node, (j - 1) K .. (j + 1) K
-        // - 1 of the window: a pair farther than that aperture needs no network
-        // evaluation at the node.  The node's own network inputs (t0 / 2,
-        // slowness^2 or vg^2, encoded velocity and eta / gradient) are tabled
-        // here too, so that the evaluation reads LDS only.  A node past the
-        // axis extrapolates t0 linearly and takes v, eta at the axis end.
+        // Learned traveltime on the t0 lattice.  The network's outputs g (the
+        // legs' learned factors) are evaluated at the lattice nodes for three
+        // pillars of the workgroup's group of kNPil (0, p_mid and the last
+        // valid one, p_hi), quadratic in the pillar index in between (a leg's
+        // squared distance is quadratic in it: a line through two pillars
+        // would bias every time the same way); along t0 each slot
+        // takes g from the cubic Hermite polynomial of its segment [n_j,
+        // n_j+1] of the axis lattice (the segment clamped to the window's first
+        // and last), through g at the two nodes with slopes from differences
+        // over nodes j - 1 .. j + 2; the leg times are then formed per sample
+        // with the sample's own velocity and eta.  The window's nodes are jf ..
+        // jl; with one neighbour on each side the item tables "extended" nodes
+        // jf - 1 .. jl + 1 (local index i = j - jf + 1, nx = nn + 2 of them;
+        // nodes off the axis are never evaluated).  Per (pillar, extended
+        // node): the smallest taper slope (widest aperture) over the samples
+        // whose segment reads the node (segments j - 2 .. j + 1): a node no
+        // pillar of the group sees inside its aperture needs no network
+        // evaluation; and the node's network inputs (t0 / 2, slowness^2 or
+        // vg^2, encoded velocity and eta / gradient), so that the evaluation
+        // reads LDS only.
         int nidx[MW];
         float nfr[MW];
         int nn = 0;  // lattice nodes of this window (wave-uniform)
+        int jf = 0;
         if (learned_tt(TT)) {
-            // the window's nodes are jf .. jl of the axis lattice: the last at or
-            // before its first sample, the first at or after its last one; each
-            // slot reads the two nodes of its segment (clamped to the window's)
             const int* npos = reinterpret_cast<const int*>(smem + P.lat_pos_off);
             auto upper = [&](int k) {  // first node past sample k
                 int lo = 0, hi = P.lat_total;
@@ -1100,30 +1147,30 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 return lo;
             };
             const int kend = min(kbase + win, P.nt);
-            const int jf = min(upper(kbase) - 1, P.lat_total - 2);
+            jf = __builtin_amdgcn_readfirstlane(min(upper(kbase) - 1, P.lat_total - 2));
             const int jl = max(upper(kend - 2), jf + 1);
             nn = __builtin_amdgcn_readfirstlane(jl - jf + 1);
 #pragma unroll
             for (int m = 0; m < MW; ++m) {
                 const int k = min(kbase + 64 * (tw + kTW * m) + lane, kend - 1);
                 const int j = max(jf, min(upper(k) - 1, jl - 1));
-                nidx[m] = j - jf;
+                nidx[m] = j - jf;  // segment of the window
                 nfr[m] = (float)(k - npos[j]) / (float)(npos[j + 1] - npos[j]);
             }
+            const int nx = nn + 2;
             float* cimin = reinterpret_cast<float*>(smem + P.lat_cimin_off);
             float4* ntab = reinterpret_cast<float4*>(smem + P.node_off);
-            for (int e = tid; e < kNPil * nn; e += kThreads) {
-                const int p = e / nn, i = e - p * nn;
+            for (int e = tid; e < kNPil * nx; e += kThreads) {
+                const int p = e / nx, i = e - p * nx;
                 const int pil = pil0 + p;
-                const int j = jf + i;
-                // samples that read node j: segments j - 1 and j inside the
-                // window, and the window's last sample (clamped to its last
-                // segment, which may end at it)
-                const int k0 = i == 0 ? kbase : max(kbase, npos[j - 1]);
-                const int k1 = i == nn - 1 ? kend : min(kend, npos[j + 1] + 1);
+                const int j = jf - 1 + i;
+                // window segments that read node j: j - 2 .. j + 1
+                const int sa = max(j - 2, jf), sb = min(j + 1, jl - 1);
                 float c = INFINITY;
                 float4 nd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (pil < npil) {
+                if (pil < npil && j >= 0 && j < P.lat_total && sa <= sb) {
+                    const int k0 = sa == jf ? kbase : npos[sa];
+                    const int k1 = sb == jl - 1 ? kend : npos[sb + 1];
                     for (int k = k0; k < k1; ++k) c = fminf(c, T.kc[(size_t)pil * T.kc_pstride + k].z);
                     const int kk = npos[j];
                     const float tau = 0.5f * P.t0k[kk];
@@ -1144,7 +1191,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
         }
 
-        // Pipelined staging (P.pipe): while a batch computes, each thread holds
+        // Pipelined staging (pipe; never for the learned traveltime, whose rows
+        // go by direct loads so that no registers are held across the network
+        // evaluation): while a batch computes, each thread holds
         // up to kPF dwords of the next batch's rows in flight in registers; they
         // are written to LDS after the batch's last barrier.
         uint32_t pf[kPF];
@@ -1191,7 +1240,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // waves w0.. issue the loads (chunk c of 64 dwords by wave w0 + c % (kWaves - w0),
         // the same wave for a given LDS chunk at every call)
         auto dma = [&](int jb, int buf, int rsrc, int w0) {
-            if constexpr (AA) {
+            if constexpr (AA || learned_tt(TT)) {
                 const int nbb = min(P.batch, T.ntr - jb);
                 if (nbb <= 0 || wave < w0) return;
                 const int nw = kWaves - w0;
@@ -1218,7 +1267,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 }
             }
         };
-        if (P.pipe) {
+        if (pipe) {
             if (AA) dma(0, 0, kRowsGlobal, 0);
             else prefetch(0, kRowsGlobal);
         }
@@ -1231,7 +1280,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // per element).  The row indices come from s_rownext, written by the
         // batch before.  The waves that build the pair table issue none: their
         // own global reads would wait behind the row loads.
-        const bool early_dma = AA && !P.pipe;
+        const bool early_dma = (AA || learned_tt(TT)) && !pipe;
         const int w_pairs = (kNPil * P.batch + 63) / 64;  // waves holding pair-table threads
         for (int j0 = 0, par = 0; j0 < T.ntr; j0 += P.batch, par ^= 1) {
             const int nb = min(P.batch, T.ntr - j0);
@@ -1242,7 +1291,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             // so that those registers are free for the rest of the batch (the
             // network evaluation above all); visible after the pair-table
             // barrier.  A skipped batch stores rows nobody reads.
-            if (!AA && P.pipe) {
+            if (!AA && pipe) {
                 uint32_t* dst = reinterpret_cast<uint32_t*>(rows);
                 const int n = nb * rl_dw;
 #pragma unroll
@@ -1255,7 +1304,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             if (learned_tt(TT) && tid == 0) s_nrows = 0;
             if (tid < P.batch) {  // wave 0, which builds pairs and issues no row loads
                 if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
-                if (P.pipe || early_dma) {
+                if (pipe || early_dma) {
                     const int jn = j0 + P.batch + tid;
                     s_rownext[par][tid] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
                 }
@@ -1271,8 +1320,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     const double dx64 = __dsub_rn(pox64, P.mx64[trow]);
                     const double dy64 = __dsub_rn(poy64, P.my64[trow]);
                     const double dm64 = dist64(dx64, dy64);
-                    if (!(dm64 > P.max_ap64)) {
-                        inap = 1;
+                    // learned traveltimes: the leg terms of every pillar, also
+                    // outside the aperture (an end pillar's network inputs)
+                    const bool in_ap64 = !(dm64 > P.max_ap64);
+                    if (in_ap64 || learned_tt(TT)) {
+                        inap = in_ap64 ? 1 : 0;
                         const float4 g0 = T.geo[2 * (j0 + pt_j)];
                         const float4 g1 = T.geo[2 * (j0 + pt_j) + 1];
                         const float dxs = pox - g0.x, dys = poy - g0.y;
@@ -1288,7 +1340,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             e = make_float4(a_s, a_r, dm, g1.z);
                             f = make_float4(1.0f / (den_s * den_r), 0.0f, 0.0f, 0.0f);
                         } else if (TT == PSTM_TT_CURVED_LEARNED) {
-                            e = make_float4(P.ck2 * ds2, P.ck2 * dr2, dm, g1.z);
+                            e = make_float4(P.ck2 * ds2, P.ck2 * dr2, in_ap64 ? dm : INFINITY, g1.z);
+                        } else if (TT == PSTM_TT_VTI_LEARNED) {
+                            e = make_float4(ds2, dr2, in_ap64 ? dm : INFINITY, g1.z);
                         } else {
                             e = make_float4(ds2, dr2, dm, g1.z);
                             // AA: offsets pre-scaled by bin size / dt, so the operator
@@ -1301,7 +1355,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 if (kPairVec == 2) pairs[(pt_p * P.batch + pt_j) * kPairVec + 1] = f;
             }
             if (!__syncthreads_or(inap)) {
-                if (P.pipe) {  // this batch's rows are not needed
+                if (pipe) {  // this batch's rows are not needed
                     if (AA) {
                         dma(j0 + P.batch, cur ^ 1, kRowsNext + par, 0);
                         cur ^= 1;
@@ -1313,11 +1367,15 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
 
             if (learned_tt(TT)) {
-                // network on the t0 lattice: the (pair, node) rows whose node is
-                // read by some slot inside the pair's time-dependent aperture
-                // are listed (in any order), the others set to 0; then 32 listed
+                // network on the t0 lattice at the group's three evaluated
+                // pillars (pe = 0, 1, 2: pillars 0, p_mid, p_hi; pe = 1 only for
+                // groups of three or more, pe = 2 for two or more): the rows
+                // (pe, trace, extended node) whose node some pillar of the
+                // group reads inside its time-dependent aperture are listed (in
+                // any order), the others marked missing (NaN); then 32 listed
                 // rows per wavefront step
-                const int R = kNPil * P.batch * nn;
+                const int nx = nn + 2;
+                const int R = 3 * P.batch * nx;
                 float2* lat = reinterpret_cast<float2*>(smem + P.lat_off);
                 const float* cimin = reinterpret_cast<const float*>(smem + P.lat_cimin_off);
                 unsigned short* rlist = reinterpret_cast<unsigned short*>(smem + P.lat_list_off);
@@ -1325,11 +1383,15 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     const int r = r0 + tid;
                     bool valid = false;
                     if (r < R) {
-                        const int pp = r / nn, node = r - pp * nn;
-                        const float4 e = pairs[pp * kPairVec];
-                        valid = e.z != INFINITY &&
-                                fmaf(e.z, cimin[(pp / P.batch) * nn + node], P.taper_off) <= 1.0f + P.eps_x;
-                        if (!valid) lat[r] = make_float2(0.0f, 0.0f);
+                        const int pe = r / (P.batch * nx), jn = r - pe * (P.batch * nx);
+                        const int jt = jn / nx, node = jn - jt * nx;
+                        if (pe == 0 || (pe == 1 && p_hi >= 2) || (pe == 2 && p_hi >= 1)) {
+                            for (int p = 0; p <= p_hi && !valid; ++p) {
+                                const float ez = pairs[(p * P.batch + jt) * kPairVec].z;
+                                valid = ez != INFINITY && fmaf(ez, cimin[p * nx + node], P.taper_off) <= 1.0f + P.eps_x;
+                            }
+                        }
+                        if (!valid) lat[r] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
                     }
                     const unsigned long long bal = __ballot(valid);
                     int base = 0;
@@ -1348,9 +1410,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     const int r = valid ? (int)rlist[li] : 0;
                     float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                     if (valid) {
-                        const int pp = r / nn, node = r - pp * nn;
-                        const float4 e = pairs[pp * kPairVec];
-                        const float4 nd = ntab[(pp / P.batch) * nn + node];  // {tau, s, v enc., eta enc.}
+                        const int pe = r / (P.batch * nx), jn = r - pe * (P.batch * nx);
+                        const int jt = jn / nx, node = jn - jt * nx;
+                        const int pend = pe == 0 ? 0 : (pe == 1 ? p_mid : p_hi);
+                        const float4 e = pairs[(pend * P.batch + jt) * kPairVec];
+                        const float4 nd = ntab[pend * nx + node];  // {tau, s, v enc., eta enc.}
                         const float tau = nd.x;
                         float sn_s, sn_r;
                         if (TT == PSTM_TT_VTI_LEARNED) {
@@ -1380,12 +1444,12 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
 
             // stage the batch of trace rows in LDS (coalesced)
-            if (AA && P.pipe) {
+            if (AA && pipe) {
                 rows = reinterpret_cast<Row*>(smem + (size_t)cur * P.rows_bytes);
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
-            } else if (P.pipe) {
+            } else if (pipe) {
                 // f32 rows: stored at the batch start
-            } else if (AA) {
+            } else if (early_dma) {
                 // the batch's rows, loaded at the batch start (early_dma)
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
             } else {
@@ -1419,10 +1483,75 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
             // pipelined f32 rows are visible since the pair-table barrier: only
             // DMA'd or loaded rows and the lattice need this one
-            if (AA || !P.pipe || learned_tt(TT)) __syncthreads();
-            if (P.pipe) {  // in flight during the batch
+            if (AA || !pipe || learned_tt(TT)) __syncthreads();
+            if (pipe) {  // in flight during the batch
                 if (AA) dma(j0 + P.batch, cur ^ 1, kRowsNext + par, 0);
                 else prefetch(j0 + P.batch, kRowsNext + par);
+            }
+            if (learned_tt(TT)) {
+                // per (pillar, trace, window segment) the cubic Hermite
+                // coefficients of both legs' g in the segment's fraction f,
+                // g = a + f (b + f (c + f d)): g at the four nodes j - 1 .. j + 2
+                // from the three evaluated pillars (Lagrange weights in the
+                // pillar index: quadratic; a line for a group of two, the value
+                // itself for one), values at n_j and n_j+1 with slopes by
+                // node_derivative; a segment with a node not evaluated is read
+                // by no slot inside the aperture (zeros)
+                const int nx = nn + 2, nseg = nn - 1;
+                const int* npos = reinterpret_cast<const int*>(smem + P.lat_pos_off);
+                const float2* lat = reinterpret_cast<const float2*>(smem + P.lat_off);
+                float4* coef = reinterpret_cast<float4*>(smem + P.coef_off);
+                for (int e = tid; e < kNPil * P.batch * nseg; e += kThreads) {
+                    const int pp = e / nseg, sg = e - pp * nseg;
+                    const int p = pp / P.batch, jt = pp - p * P.batch;
+                    const int j = jf + sg;
+                    float4 cs4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cr4 = cs4;
+                    if (p <= p_hi) {
+                        // weights of pillars 0, p_mid, p_hi for pillar p
+                        float w0, w1, w2;
+                        const float fp = (float)p;
+                        if (p_hi >= 2) {
+                            const float b = (float)p_mid, c = (float)p_hi;
+                            w0 = (fp - b) * (fp - c) / (b * c);
+                            w1 = fp * (fp - c) / (b * (b - c));
+                            w2 = fp * (fp - b) / (c * (c - b));
+                        } else {
+                            w2 = p_hi == 1 ? fp : 0.0f;
+                            w0 = 1.0f - w2;
+                            w1 = 0.0f;
+                        }
+                        const float2* ra = lat + jt * nx + sg;                  // pe 0, nodes j - 1 .. j + 2
+                        const float2* rb = lat + (P.batch + jt) * nx + sg;      // pe 1
+                        const float2* rc = lat + (2 * P.batch + jt) * nx + sg;  // pe 2
+                        float gs[4], gr[4], x[4];
+                        bool ok[4];
+#pragma unroll
+                        for (int n = 0; n < 4; ++n) {
+                            const int jn = j - 1 + n;
+                            const float2 ga = ra[n];
+                            const float2 gb = p_hi >= 2 ? rb[n] : make_float2(0.0f, 0.0f);
+                            const float2 gc = p_hi >= 1 ? rc[n] : make_float2(0.0f, 0.0f);
+                            ok[n] = !__builtin_isnan(ga.x) && !__builtin_isnan(gb.x) && !__builtin_isnan(gc.x);
+                            gs[n] = fmaf(w0, ga.x, fmaf(w1, gb.x, w2 * gc.x));
+                            gr[n] = fmaf(w0, ga.y, fmaf(w1, gb.y, w2 * gc.y));
+                            x[n] = (float)npos[min(max(jn, 0), P.lat_total - 1)] +
+                                   (float)(jn < 0 ? jn : (jn >= P.lat_total ? jn - P.lat_total + 1 : 0));
+                        }
+                        if (ok[1] && ok[2]) {
+                            const float L = x[2] - x[1];
+                            auto herm = [&](const float (&g)[4]) {
+                                const float b0 = L * node_derivative(g, ok, x, 1), b1 = L * node_derivative(g, ok, x, 2);
+                                return make_float4(g[1], b0, 3.0f * (g[2] - g[1]) - 2.0f * b0 - b1,
+                                                   2.0f * (g[1] - g[2]) + b0 + b1);
+                            };
+                            cs4 = herm(gs);
+                            cr4 = herm(gr);
+                        }
+                    }
+                    coef[2 * e] = cs4;
+                    coef[2 * e + 1] = cr4;
+                }
+                __syncthreads();
             }
 
             // the traces of the batch; EARLY: this wave holds t0 <= 2 ms samples,
@@ -1472,31 +1601,32 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             any_in |= blk[m];
                         }
                         if (!any_in) continue;  // time-dependent aperture excludes every slot
-                        if (learned_tt(TT)) {
-                            const float2* lp = reinterpret_cast<const float2*>(smem + P.lat_off) + (size_t)pi * nn;
-    #pragma unroll
-                            for (int m = 0; m < MW; ++m) {
-                                const float2 c0 = lp[nidx[m]], c1 = lp[nidx[m] + 1];
-                                e2[p][m] = fmaf(nfr[m], c1.x - c0.x, c0.x);
-                                e1[p][m] = fmaf(nfr[m], c1.y - c0.y, c0.y);
-                            }
-                        }
+                        // learned: the pair's Hermite coefficients of g per window segment
+                        const float4* cfp = reinterpret_cast<const float4*>(smem + P.coef_off) + (size_t)pi * 2 * (nn - 1);
                         // one pass per sample block, so that only one block's
                         // temporaries are live at a time (fewer registers)
     #pragma unroll
                         for (int m = 0; m < MW; ++m) {
                             if (!blk[m]) continue;
                             float y1, y2;
+                            if constexpr (learned_tt(TT)) {
+                                // both legs' g: the segment's cubics in the slot's fraction
+                                const float4 ca = cfp[2 * nidx[m]], cb = cfp[2 * nidx[m] + 1];
+                                const float fr = nfr[m];
+                                e2[p][m] = fmaf(fmaf(fmaf(ca.w, fr, ca.z), fr, ca.y), fr, ca.x);
+                                e1[p][m] = fmaf(fmaf(fmaf(cb.w, fr, cb.z), fr, cb.y), fr, cb.x);
+                            }
                             const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m],
                                                                  etar[p][m], y1, y2);
                             const float u = fmaf(t, P.idt, uoff_v);
                             // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
                             const bool in_ap = xt[m] <= ((COUNT && !learned_tt(TT)) ? 1.0f - P.eps_x : 1.0f);
                             const bool below = u < P.umax_lo;
-                            // thresholds that can hold lanes below umax - eps
+                            // thresholds that can hold lanes below umax - eps (the learned
+                            // time has no f64 counterpart: none)
                             bool near_o = false;
-                            if (LOB) near_o = near_o || (in_ap && u < P.lo_eps && u > -P.lo_eps);
-                            if (COUNT) near_o = near_o || (fabsf(xt[m] - 1.0f) < P.eps_x);
+                            if (LOB && !learned_tt(TT)) near_o = near_o || (in_ap && u < P.lo_eps && u > -P.lo_eps);
+                            if (COUNT && !learned_tt(TT)) near_o = near_o || (fabsf(xt[m] - 1.0f) < P.eps_x);
                             // nearest sample and windowed sinc jump at fraction 0.5 (the
                             // sample / the tap window moves by one): decided in f64 there
                             if (kHalfJump<INTERP>)
@@ -1524,7 +1654,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             // band lanes are never below umax - eps: only near_o takes lanes
                             // out of the ones below
                             bool ok = in_ap && (learned_tt(TT) ? u < P.umax : below);
-                            if (LOB || COUNT || kHalfJump<INTERP>) ok = ok && !near_o;
+                            if ((LOB || COUNT || kHalfJump<INTERP>) && !learned_tt(TT)) ok = ok && !near_o;
                             if (LOB) ok = ok && u >= 0.0f;
                             const float us = ok ? u : 0.0f;
                             const int i0 = (int)us;
@@ -1615,11 +1745,12 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             for (int p = 0; p < kPW; ++p)
 #pragma unroll
                 for (int m = 0; m < MW; ++m) {
-                    tot[(p * MW + m) * kThreads + tid] += (double)acc[p][m];
+                    if (learned_tt(TT)) totf[(p * MW + m) * kThreads + tid] += acc[p][m];
+                    else tot[(p * MW + m) * kThreads + tid] += (double)acc[p][m];
                     acc[p][m] = 0.0f;
                 }
             __syncthreads();
-            if (AA && P.pipe) cur ^= 1;
+            if (AA && pipe) cur ^= 1;
         }
         // a skipped last batch leaves its row loads in flight: none may outlive
         // the item (nor the workgroup, whose LDS is then reassigned)
@@ -1637,7 +1768,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     // the total times cw (early blocks carry it per contribution)
                     const double cwb = (early & (1u << m)) ? 1.0 : (double)cw_at(p, m);
                     float* dst = T.image + (size_t)pil * P.nt + k;
-                    *dst = *dst + (float)(tot[(p * MW + m) * kThreads + tid] * cwb);
+                    const double sum = learned_tt(TT) ? (double)totf[(p * MW + m) * kThreads + tid]
+                                                      : tot[(p * MW + m) * kThreads + tid];
+                    *dst = *dst + (float)(sum * cwb);
                 }
             }
         }
@@ -1948,18 +2081,24 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     if (pipe < 0) pipe = sh.aa ? 0 : 1;
     const int rl_dw = sh.rowlen * (int)(sh.elem / 4);
     const int pipe_batch = sh.aa ? kMaxBatch : kPF * kThreads / rl_dw;
-    sh.pipe = pipe == 1 && pipe_batch >= 1;
+    // (the learned traveltime always stages by direct loads at the batch start)
+    sh.pipe = pipe == 1 && pipe_batch >= 1 && !sh.learned;
     const int nbuf = (sh.pipe && sh.aa) ? 2 : 1;
     // LDS: trace rows + pair table + f64 totals; learned: + lattice, widest
     // apertures, row list, node inputs, network operands (kept for the
     // workgroup's life)
+    // learned: per (evaluated pillar, trace, extended node) the network
+    // outputs (8 B) and the row list entry (2 B), per (pillar, trace, segment) the
+    // Hermite coefficients of both legs (32 B), per (pillar, extended node)
+    // the widest aperture (4 B) and the network inputs (16 B)
     auto lds_for = [&](int b) {
+        const size_t nx = (size_t)sh.lat_nodes + 2;
         return align_up((size_t)nbuf * align_up((size_t)b * sh.rowlen * sh.elem, 16) +
                             (size_t)kNPil * b * sh.pair_vec * 16, 16) +
-               sh.tot_bytes + align_up((size_t)kNPil * b * sh.lat_nodes * 8, 16) +
-               (sh.learned ? align_up((size_t)kNPil * sh.lat_nodes * 4, 16) +
-                                 align_up((size_t)kNPil * b * sh.lat_nodes * 2, 16) +
-                                 (size_t)kNPil * sh.lat_nodes * 16 + (size_t)sh.mlp_terms * kAVec8 * 16 +
+               sh.tot_bytes +
+               (sh.learned ? align_up((size_t)3 * b * nx * 8, 16) + (size_t)kNPil * b * sh.lat_nodes * 32 +
+                                 align_up((size_t)kNPil * nx * 4, 16) + align_up((size_t)3 * b * nx * 2, 16) +
+                                 (size_t)kNPil * nx * 16 + (size_t)sh.mlp_terms * kAVec8 * 16 +
                                  (size_t)kBVals * 4 + align_up((size_t)sh.lat_total * 4, 16)
                            : 0);
     };
@@ -1970,7 +2109,8 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     }
     // sample blocks per wave: the learned layout must fit one trace per batch
     for (;;) {
-        sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * sizeof(double);
+        // running totals: f64, f32 for the learned traveltime
+        sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * (sh.learned ? sizeof(float) : sizeof(double));
         sh.lat_nodes = sh.learned ? lattice_window_nodes(npos, nt, 256 * sh.mw) : 0;
         if (sh.mw == 1 || lds_for(1) <= kLdsBudget) break;
         --sh.mw;
@@ -2050,12 +2190,14 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.cv0 = a.curved_v0;
     kp.ck = a.curved_k;
     kp.eta_const = a.eta_constant;
+    const int nx = sh.lat_nodes + 2;  // learned: extended nodes of a window
     kp.lat_off = (int)(kp.tot_off + sh.tot_bytes);
     kp.lat_nodes = sh.lat_nodes;
-    kp.lat_cimin_off = kp.lat_off + (int)align_up((size_t)kNPil * sh.batch * sh.lat_nodes * 8, 16);
-    kp.lat_list_off = kp.lat_cimin_off + (int)align_up((size_t)kNPil * sh.lat_nodes * 4, 16);
-    kp.node_off = kp.lat_list_off + (int)align_up((size_t)kNPil * sh.batch * sh.lat_nodes * 2, 16);
-    kp.mlp_off = kp.node_off + kNPil * sh.lat_nodes * 16;
+    kp.coef_off = kp.lat_off + (int)align_up((size_t)3 * sh.batch * nx * 8, 16);
+    kp.lat_cimin_off = kp.coef_off + kNPil * sh.batch * sh.lat_nodes * 32;
+    kp.lat_list_off = kp.lat_cimin_off + (int)align_up((size_t)kNPil * nx * 4, 16);
+    kp.node_off = kp.lat_list_off + (int)align_up((size_t)3 * sh.batch * nx * 2, 16);
+    kp.mlp_off = kp.node_off + kNPil * nx * 16;
     kp.lat_pos_off = kp.mlp_off + (int)((size_t)sh.mlp_terms * kAVec8 * 16 + (size_t)kBVals * 4);
     kp.lat_total = sh.lat_total;
     kp.lat_kmin = a.mlp_node_spacing;

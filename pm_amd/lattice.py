@@ -4,14 +4,16 @@ Nodes are output-sample indices n_0 = 0, n_{j+1} = min(n_j + min(max((n_j g)
 >> 8, K), K_max), nt - 1), up to the last sample nt - 1: spacing K at shallow
 times, growing by g / 256 of the node's own index, capped at K_max.  g = 0
 gives the uniform lattice K, 2K, ... (the last interval shorter).  Each output
-sample reads the two nodes around it and interpolates the network's outputs
-linearly.  The same integer law runs in the gather (lattice_next in
-pstm_hip.hip) and here, for the held-out check and the tests.
+sample takes the network's outputs from the cubic Hermite polynomial of its
+segment [n_j, n_{j+1}] (values at the two nodes, slopes from differences
+over nodes j - 1 .. j + 2).  The same integer law runs in the gather
+(lattice_next in pstm_hip.hip) and here, for the held-out check and the
+tests.
 
-Why a growing spacing: the learned factor c of a leg changes fastest where
+Why a growing spacing: the learned factor of a leg changes fastest where
 t0 / 2 is comparable to the leg's horizontal time d / v, which for the short
-legs that dominate the interpolation error is at shallow times; deeper, c is
-smooth on the scale of tens of samples.  A lattice dense at shallow times
+legs that dominate the interpolation error is at shallow times; deeper, it
+is smooth on the scale of tens of samples.  A lattice dense at shallow times
 and sparse below reaches the interpolation error of a uniform lattice with
 far fewer network evaluations per (pillar, trace) pair.
 """
@@ -61,7 +63,13 @@ def as_lattice(node_spacing) -> Lattice:
     return Lattice(spacing=int(node_spacing))
 
 
-# the lattice the benchmark and the full-size tests use: 8 samples at shallow
-# times, growing by 77/256 (~0.3) of the node index, at most 48 samples (40
-# nodes over 1500 samples where the uniform 16-sample lattice has 95)
+# a growing lattice for general use: 8 samples at shallow times, growing by
+# 77/256 (~0.3) of the node index, at most 48 samples (40 nodes over 1500
+# samples where the uniform 16-sample lattice has 95)
 GEOMETRIC = Lattice(spacing=8, growth=77, max_spacing=48)
+
+# the benchmark's lattice (and the full-size VTI test's): growing by half
+# the node index, at most 96 samples (23 nodes over 1500 samples); on a
+# short output axis a larger share of the samples sits at the coarsest
+# spacing, where GEOMETRIC is the safer choice
+SPARSE = Lattice(spacing=8, growth=128, max_spacing=96)

@@ -193,7 +193,8 @@ class TileExecutor:
         traveltime.  Returns the per-tile maxima."""
         import torch
 
-        from pm_amd.surrogate import curved_leg_times, curved_velocity_at, lattice_times, vti_leg_times
+        from pm_amd.surrogate import (curved_leg_times, curved_velocity_at, lattice_times, pillar_group_ends,
+                                      vti_leg_times)
 
         g = torch.Generator(device=self.device)
         g.manual_seed(seed)
@@ -214,6 +215,12 @@ class TileExecutor:
             pil, row, k, ox, oy = pil[keep][:n], row[keep][:n], k[keep][:n], ox[keep][:n], oy[keep][:n]
             ds = ((ox - tr.source_x[row]) ** 2 + (oy - tr.source_y[row]) ** 2).sqrt()
             dr = ((ox - tr.receiver_x[row]) ** 2 + (oy - tr.receiver_y[row]) ** 2).sqrt()
+            # the gather evaluates the network at three pillars of each
+            # pillar's group and interpolates between them
+            pe, w = pillar_group_ends(pil, tnx * tny)
+            ex_, ey_ = t.x_axis[pe // tny], t.y_axis[pe % tny]
+            ds_e = ((ex_ - tr.source_x[row][:, None]) ** 2 + (ey_ - tr.source_y[row][:, None]) ** 2).sqrt()
+            dr_e = ((ex_ - tr.receiver_x[row][:, None]) ** 2 + (ey_ - tr.receiver_y[row][:, None]) ** 2).sqrt()
             if getattr(surrogate, "kind", "vti") == "curved":
                 if float(surrogate.curved_v0) != float(self.config.curved_ray_v0):
                     raise ValueError("the curved surrogate was trained for another surface velocity")
@@ -222,7 +229,9 @@ class TileExecutor:
                 vg_axis = (v0 * curved_velocity_at(tau_axis, v0, kg)).sqrt()
                 v_at = lambda kk: vg_axis[kk]  # noqa: E731
                 e_at = lambda kk: torch.full(kk.shape, kg, dtype=torch.float64, device=self.device)  # noqa: E731
-                learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing)
+                ends = {"ds": ds_e, "dr": dr_e, "v_at": lambda kk, e: v_at(kk), "eta_at": lambda kk, e: e_at(kk),
+                        "w": w}
+                learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing, ends)
                 tau = tau_axis[k]
                 exact = curved_leg_times(tau, ds, v0, kg) + curved_leg_times(tau, dr, v0, kg)
             else:
@@ -234,7 +243,10 @@ class TileExecutor:
                 else:
                     e_at = lambda kk: torch.full(kk.shape, float(self.config.vti_eta_constant),  # noqa: E731
                                                  dtype=torch.float64, device=self.device)
-                learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing)
+                v_e = (lambda kk, e: vflat[pe[:, e], kk]) if vflat is not None else (lambda kk, e: t.vrms[kk])
+                e_e = (lambda kk, e: eflat[pe[:, e], kk]) if eflat is not None else (lambda kk, e: e_at(kk))
+                ends = {"ds": ds_e, "dr": dr_e, "v_at": v_e, "eta_at": e_e, "w": w}
+                learned = lattice_times(surrogate, tau_axis, k, ds, dr, v_at, e_at, node_spacing, ends)
                 tau = tau_axis[k]
                 exact = vti_leg_times(tau, ds, v_at(k), e_at(k)) + vti_leg_times(tau, dr, v_at(k), e_at(k))
             inside = exact / dt_in < ns - 1

@@ -473,35 +473,121 @@ class TraveltimeSurrogate:
         return self.params.to(torch.bfloat16)
 
 
-def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, node_spacing):
-    """Learned two-leg time of output samples k as the gather forms it: the
-    network (the fused inference's own code) on the lattice nodes n_j <= k <
-    n_{j+1} around each sample (pm_amd.lattice; ``node_spacing`` a Lattice or
-    an int, the uniform spacing), its outputs interpolated linearly, then the
-    sum of the legs: VTI sqrt(tau^2 + (d / v)^2 max(1 + eta g, 0)) with g the
-    interpolated output and v, eta at the sample (t = t0 where t0 <= 0);
-    curved ray z / sqrt(vg^2 + k^2 d^2) max(1 + c, 0).
+def node_derivative(f, ok, x, i):
+    """Derivative at node i (1 or 2) of values f[0..3] at positions x[0..3]
+    (only the nodes with ok set), as the gather forms it: the three-point
+    difference over i - 1, i, i + 1 where both neighbours are there, else the
+    one-sided three-point difference over i, i -+ 1, i -+ 2, else the
+    two-point one; all second-order, exact for a quadratic (f64 tensors)."""
+    import torch
+
+    def d3(a, b, c, xa, xb, xc):
+        # derivative at xa of the parabola through (xa, a), (xb, b), (xc, c)
+        h1, h2 = xb - xa, xc - xb
+        return -(2.0 * h1 + h2) / (h1 * (h1 + h2)) * a + (h1 + h2) / (h1 * h2) * b - h1 / (h2 * (h1 + h2)) * c
+
+    lo, hi = i - 1, i + 1
+    h1, h2 = x[i] - x[lo], x[hi] - x[i]
+    central = (h1 * h1 * (f[hi] - f[i]) + h2 * h2 * (f[i] - f[lo])) / (h1 * h2 * (h1 + h2))
+    if i == 1:
+        one = torch.where(ok[3], d3(f[1], f[2], f[3], x[1], x[2], x[3]), (f[2] - f[1]) / (x[2] - x[1]))
+        return torch.where(ok[0], central, one)
+    # backward: the parabola through nodes 2, 1, 0 (negative spacings)
+    one = torch.where(ok[0], d3(f[2], f[1], f[0], x[2], x[1], x[0]), (f[2] - f[1]) / (x[2] - x[1]))
+    return torch.where(ok[3], central, one)
+
+
+# pillars per workgroup of the fused gather: the network is evaluated at
+# three pillars of each group of consecutive (tile-local, row-major) pillars
+# and interpolated quadratically in between
+GROUP_PILLARS = 8
+
+
+def pillar_group_ends(pil, npil: int, group: int = GROUP_PILLARS):
+    """For tile-local pillar indices pil (int64 tensor) of a tile of npil
+    pillars: the pillars of each one's group at which the gather evaluates the
+    network, (n, 3) = (first, middle, last valid pillar of the group), and the
+    weights (n, 3) that make the pillar's value from theirs, as the gather
+    interpolates the network's outputs: the quadratic through the three in
+    the pillar index (a leg's squared distance is quadratic in it), the line
+    through two for a group of two pillars, the value itself for one."""
+    import torch
+
+    e0 = (pil // group) * group
+    p_hi = torch.clamp(npil - 1 - e0, max=group - 1)
+    p_mid = (p_hi + 1) // 2
+    p = (pil - e0).double()
+    a, b, c = torch.zeros_like(p), p_mid.double(), p_hi.double()
+    quad = p_hi >= 2
+    bs, cs = torch.where(quad, b, 1.0), torch.where(quad, c, 2.0)
+    w0 = (p - bs) * (p - cs) / ((a - bs) * (a - cs))
+    w1 = (p - a) * (p - cs) / ((bs - a) * (bs - cs))
+    w2 = (p - a) * (p - bs) / ((cs - a) * (cs - bs))
+    lin = p / c.clamp_min(1.0)
+    W = torch.stack([torch.where(quad, w0, torch.where(p_hi == 1, 1.0 - lin, 1.0)),
+                     torch.where(quad, w1, 0.0),
+                     torch.where(quad, w2, torch.where(p_hi == 1, lin, 0.0))], 1)
+    E = torch.stack([e0, e0 + p_mid, e0 + p_hi], 1)
+    return E, W
+
+
+def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, node_spacing, ends=None):
+    """Learned two-leg time of output samples k as the gather forms it.  The
+    network (the fused inference's own code) gives the legs' factors g at the
+    lattice nodes (pm_amd.lattice; ``node_spacing`` a Lattice or an int, the
+    uniform spacing) for three pillars of the sample pillar's group,
+    interpolated quadratically in the pillar index (``ends``); along t0 the
+    sample's segment n_j <= k < n_{j+1} takes the cubic Hermite polynomial in
+    the fraction (k - n_j) / (n_{j+1} - n_j) through g at n_j and n_{j+1}
+    with slopes by node_derivative over nodes j - 1 .. j + 2.  The legs then
+    use the sample's own v and eta: VTI sqrt(tau^2 + (d / v)^2 max(1 + eta g,
+    0)) (t = t0 where t0 <= 0); curved ray z / sqrt(vg^2 + k^2 d^2) max(1 + c,
+    0).
 
     tau_axis: (nt,) one-way times t0 / 2 (f64); k: (n,) int64 sample indices;
     ds, dr: (n,) leg distances; v_at(kk), eta_at(kk): (n,) f64 values at the
-    sample's own pillar and samples kk."""
+    sample's own pillar and samples kk.  ends: None (the sample's own
+    pillar) or a dict with 'ds', 'dr' ((n, 3) leg distances from the group's
+    three evaluated pillars), 'v_at', 'eta_at' (callables (kk, e) -> (n,)
+    values at pillar e) and 'w' ((n, 3) weights, pillar_group_ends)."""
     import torch
 
     from pm_amd.lattice import as_lattice
 
     nt = int(tau_axis.shape[0])
     nodes = torch.as_tensor(as_lattice(node_spacing).nodes(nt), device=k.device)
-    j = (torch.searchsorted(nodes, k, right=True) - 1).clamp(0, nodes.shape[0] - 2)
-    k0, k1 = nodes[j], nodes[j + 1]
-    frac = ((k - k0).double() / (k1 - k0).double())[:, None]
+    nj = int(nodes.shape[0])
+    j = (torch.searchsorted(nodes, k, right=True) - 1).clamp(0, nj - 2)
+    if ends is None:
+        ends = {"ds": ds[:, None], "dr": dr[:, None], "v_at": lambda kk, e: v_at(kk),
+                "eta_at": lambda kk, e: eta_at(kk), "w": torch.ones_like(ds)[:, None]}
 
-    def at_node(kn):
+    def g_at(kn, e):
+        """Network outputs (n, 2) at nodes kn for end pillar e."""
+        tau = tau_axis[kn]
+        v, et = ends["v_at"](kn, e), ends["eta_at"](kn, e)
         enc = sur.norm.encode_curved if sur.kind == "curved" else sur.norm.encode
-        x = enc(tau_axis[kn], ds, dr, v_at(kn), eta_at(kn))
+        x = enc(tau, ends["ds"][:, e], ends["dr"][:, e], v, et)
         return sur.forward_fused(x).double() / sur.norm.out_scale
 
-    c0, c1 = at_node(k0), at_node(k1)
-    c = c0 + frac * (c1 - c0)
+    # the four nodes j - 1 .. j + 2 (positions of off-axis ones 1 apart)
+    jj = [j - 1, j, j + 1, j + 2]
+    ok = [(q >= 0) & (q < nj) for q in jj]
+    x = [nodes[q.clamp(0, nj - 1)].double() + torch.where(q < 0, q.double(), 0.0) +
+         torch.where(q >= nj, (q - nj + 1).double(), 0.0) for q in jj]
+    W = ends["w"]
+    g = []
+    for q in jj:
+        kn = nodes[q.clamp(0, nj - 1)]
+        g.append(sum(W[:, e:e + 1] * g_at(kn, e) for e in range(W.shape[1])))
+    L = x[2] - x[1]
+    f = (k - nodes[j]).double() / L
+    gk = []
+    for leg in (0, 1):
+        gl = [gn[:, leg] for gn in g]
+        b0 = L * node_derivative(gl, ok, x, 1)
+        b1 = L * node_derivative(gl, ok, x, 2)
+        gk.append(gl[1] + f * (b0 + f * ((3.0 * (gl[2] - gl[1]) - 2.0 * b0 - b1) + f * (2.0 * (gl[1] - gl[2]) + b0 + b1))))
     tau = tau_axis[k]
     v = v_at(k)
     if sur.kind == "curved":
@@ -510,8 +596,7 @@ def lattice_times(sur: TraveltimeSurrogate, tau_axis, k, ds, dr, v_at, eta_at, n
         vg = v
         z = (vg * vg / float(sur.curved_v0) - float(sur.curved_v0)) / kg
         leg = lambda d, cc: z / (vg * vg + (kg * d) ** 2).sqrt() * (1.0 + cc).clamp_min(0.0)  # noqa: E731
-        return leg(ds, c[:, 0]) + leg(dr, c[:, 1])
-    # VTI: the lattice carries g = c / eta; the sample's own eta makes c
+        return leg(ds, gk[0]) + leg(dr, gk[1])
     eta = eta_at(k)
-    t = learned_leg_time(tau, ds, v, eta * c[:, 0]) + learned_leg_time(tau, dr, v, eta * c[:, 1])
+    t = learned_leg_time(tau, ds, v, eta * gk[0]) + learned_leg_time(tau, dr, v, eta * gk[1])
     return torch.where(tau > 0, t, 2.0 * tau)

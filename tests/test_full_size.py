@@ -18,7 +18,7 @@ F32_FLOOR_*; the 1e-5 bar holds at 1001 samples, tests/test_gpu_parity.py).
   fold and every tile's trace list); migrated: the stated central region of
   32 x 32 pillars, exact VTI against the oracle and the learned traveltime
   (f32-trained network, bf16 weights, the growing t0 lattice of the
-  benchmark, pm_amd.lattice.GEOMETRIC) against exact to image relative L2
+  benchmark, pm_amd.lattice.SPARSE) against exact to image relative L2
   1e-3, with its per-tile held-out check.
 
 Progress lines name the job and step (each job takes tens of seconds).
@@ -181,7 +181,7 @@ def test_vti_full_size_region(kernel):
     import torch
 
     from pm_amd.autotune import region_tiles
-    from pm_amd.lattice import GEOMETRIC
+    from pm_amd.lattice import SPARSE
     from pm_amd.pipeline.executor import TileExecutor
     from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set
     from pm_amd.synthetic import vti_diffractor_job
@@ -209,9 +209,9 @@ def test_vti_full_size_region(kernel):
     torch.cuda.synchronize()
     say(f"vti: surrogate trained in {time.perf_counter() - t0:.1f} s")
     del tr
-    errs = ex.check_surrogate(sur, node_spacing=GEOMETRIC, tiles=sub)
+    errs = ex.check_surrogate(sur, node_spacing=SPARSE, tiles=sub)
     assert errs[0] <= 0.25
-    dt = _run_plan(ex, tiles=sub, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=GEOMETRIC)
+    dt = _run_plan(ex, tiles=sub, traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=SPARSE)
     learned = ex.tiles[sub[0]].image.double()
     rl = float(torch.linalg.norm(learned - exact) / torch.linalg.norm(exact))
     say(f"vti: learned traveltime in {dt:.1f} s, held-out {errs[0]:.3f} sample, image rel-L2 vs exact {rl:.3e}")

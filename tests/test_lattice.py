@@ -4,9 +4,11 @@ The node law is the gather's own (lattice_next in pstm_hip.hip): the GPU
 read-back tests (tests/test_learned.py, tests/test_learned_curved.py) compare
 the gather's learned times with pm_amd.surrogate.lattice_times, which uses
 these nodes, for uniform and growing lattices.  Here: the law's closed form
-for uniform spacing, its invariants, the argument checks, and that the
-growing lattice interpolates the exact VTI factor (anisotropy_vti.py:181-220)
-to the stated error with far fewer nodes than the uniform one.
+for uniform spacing, its invariants, the argument checks, the orders of the
+node differences behind the gather's cubic Hermite slopes (exact for a
+quadratic with three nodes on a side), and that the cubic on the growing
+lattice interpolates the exact VTI factor (anisotropy_vti.py:181-220) better
+than lines on a uniform lattice with four times the nodes.
 """
 
 from __future__ import annotations
@@ -14,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from pm_amd.lattice import GEOMETRIC, Lattice, as_lattice
+from pm_amd.lattice import GEOMETRIC, SPARSE, Lattice, as_lattice
 
 
 @pytest.mark.parametrize("nt", [2, 3, 17, 500, 1500, 2001])
@@ -24,7 +26,7 @@ def test_uniform_lattice_closed_form(nt, k):
     np.testing.assert_array_equal(Lattice(k).nodes(nt), want)
 
 
-@pytest.mark.parametrize("lat", [GEOMETRIC, Lattice(4, 256, 1024), Lattice(5, 300, 7), Lattice(16, 77, 128)])
+@pytest.mark.parametrize("lat", [GEOMETRIC, SPARSE, Lattice(4, 256, 1024), Lattice(5, 300, 7), Lattice(16, 77, 128)])
 @pytest.mark.parametrize("nt", [2, 9, 100, 1500, 2001])
 def test_growing_lattice_invariants(lat, nt):
     n = lat.nodes(nt)
@@ -47,30 +49,76 @@ def test_lattice_arguments():
     assert as_lattice(12) == Lattice(12) and as_lattice(GEOMETRIC) is GEOMETRIC
 
 
-def _leg_error(nodes, nt=1500, dt=0.002):
-    """Largest one-leg time error (samples) of linear interpolation of the
-    exact VTI factor c = -2 eta x / (tau^2 + (1 + 2 eta) x), x = (d / v)^2,
-    over the nodes, across leg distances, velocities and eta = 0.2 (the
-    factor's largest excursion)."""
+def _hermite(gn, nodes, k):
+    """g at samples k from node values gn as the gather interpolates it: the
+    cubic Hermite polynomial of each segment with node_derivative slopes."""
+    import torch
+
+    from pm_amd.surrogate import node_derivative
+
+    nj = len(nodes)
+    nodes_t = torch.as_tensor(nodes)
+    kt = torch.as_tensor(k)
+    j = (torch.searchsorted(nodes_t, kt, right=True) - 1).clamp(0, nj - 2)
+    jj = [j - 1, j, j + 1, j + 2]
+    ok = [(q >= 0) & (q < nj) for q in jj]
+    x = [nodes_t[q.clamp(0, nj - 1)].double() for q in jj]
+    g = [torch.as_tensor(gn)[q.clamp(0, nj - 1)] for q in jj]
+    L = x[2] - x[1]
+    b0, b1 = L * node_derivative(g, ok, x, 1), L * node_derivative(g, ok, x, 2)
+    f = (kt - nodes_t[j]).double() / L
+    return (g[1] + f * (b0 + f * ((3 * (g[2] - g[1]) - 2 * b0 - b1) + f * (2 * (g[1] - g[2]) + b0 + b1)))).numpy()
+
+
+def _leg_error(nodes, cubic, nt=1500, dt=0.002):
+    """Largest one-leg time error (samples) of interpolating the exact VTI
+    factor g = -2 x / (tau^2 + (1 + 2 eta) x), x = (d / v)^2, over the nodes
+    (cubic: the gather's Hermite; else linear), across leg distances,
+    velocities and eta = 0.2 (the factor's largest excursion)."""
     k = np.arange(nt)
     j = np.clip(np.searchsorted(nodes, k, side="right") - 1, 0, len(nodes) - 2)
     f = (k - nodes[j]) / (nodes[j + 1] - nodes[j])
     worst = 0.0
-    for d in np.r_[np.linspace(2, 200, 60), np.linspace(200, 4000, 40)]:
+    for d in (50.0, 150.0, 400.0, 1000.0, 2500.0):
         for v in (1500.0, 2500.0, 4500.0):
             x, eta = (d / v) ** 2, 0.2
-            c = lambda tau: -2 * eta * x / (tau ** 2 + (1 + 2 * eta) * x)  # noqa: E731
+            g = lambda tau: -2 * x / (tau ** 2 + (1 + 2 * eta) * x)  # noqa: E731
             tau, taun = k * dt / 2, nodes * dt / 2
-            ci = c(taun)[j] + f * (c(taun)[j + 1] - c(taun)[j])
-            t = np.sqrt(tau ** 2 + x * (1 + c(tau)))
-            ti = np.sqrt(tau ** 2 + x * np.maximum(1 + ci, 0))
+            gi = _hermite(g(taun), nodes, k) if cubic else g(taun)[j] + f * (g(taun)[j + 1] - g(taun)[j])
+            t = np.sqrt(tau ** 2 + x * (1 + eta * g(tau)))
+            ti = np.sqrt(tau ** 2 + x * np.maximum(1 + eta * gi, 0))
             worst = max(worst, float(np.abs(t - ti).max() / dt))
     return worst
 
 
-def test_growing_lattice_beats_uniform_on_the_exact_factor():
-    uni, geo = Lattice(16).nodes(1500), GEOMETRIC.nodes(1500)
-    e_uni, e_geo = _leg_error(uni), _leg_error(geo)
-    # well under half the nodes at half the interpolation error
-    assert len(geo) * 2.3 < len(uni)
-    assert e_geo < 0.6 * e_uni < 0.06
+def test_growing_cubic_lattice_beats_uniform_linear_on_the_exact_factor():
+    """The gather's cubic on the benchmark's growing lattice (23 nodes over
+    1500 samples) interpolates the factor to half the worst error of lines
+    through the 95 nodes of the uniform 16-sample lattice, and of lines
+    through its own nodes."""
+    uni, geo = Lattice(16).nodes(1500), SPARSE.nodes(1500)
+    e_uni, e_geo, e_geo_lin = _leg_error(uni, False), _leg_error(geo, True), _leg_error(geo, False)
+    assert len(geo) * 4 < len(uni)
+    assert e_geo < 0.5 * e_uni < 0.05
+    assert e_geo < 0.5 * e_geo_lin
+
+
+@pytest.mark.parametrize("ok", [(True, True, True, True), (False, True, True, True), (True, True, True, False),
+                                (False, True, True, False)])
+def test_node_derivative_orders(ok):
+    """Slopes at nodes 1 and 2 of four nonuniform nodes: exact for a quadratic
+    with three nodes on a side (central or one-sided), for a line with two."""
+    import torch
+
+    from pm_amd.surrogate import node_derivative
+
+    x = [torch.tensor([v], dtype=torch.float64) for v in (3.0, 11.0, 16.0, 31.0)]
+    okt = [torch.tensor([o]) for o in ok]
+    a, b, c = 0.7, -1.3, 0.045
+    quad = [a + b * xi + c * xi * xi for xi in x]
+    line = [a + b * xi for xi in x]
+    for i in (1, 2):
+        three = ok[i - 1] and ok[i + 1] or (i == 1 and ok[3]) or (i == 2 and ok[0])
+        f = quad if three else line
+        want = b + (2 * c * x[i] if three else torch.zeros(1, dtype=torch.float64))
+        assert torch.allclose(node_derivative(f, okt, x, i), want, rtol=0, atol=1e-12)

@@ -26,7 +26,7 @@ import numpy as np
 import pytest
 
 from pm_amd.kernels import _abi
-from pm_amd.lattice import Lattice
+from pm_amd.lattice import GEOMETRIC, Lattice
 
 V0, KG = 1700.0, 0.55
 
@@ -113,7 +113,8 @@ def _ramp_config():
 def test_learned_curved_readback(kernel, spacing, terms):
     import torch
 
-    from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, curved_velocity_at, lattice_times
+    from pm_amd.surrogate import (Normalizer, TraveltimeSurrogate, curved_velocity_at, lattice_times,
+                                  pillar_group_ends)
 
     c = _ramp_case(4)
     norm = Normalizer(tau_max=1.0, d_max=2000.0, v_lo=1500.0, v_hi=4500.0, eta_max=1.0, out_scale=4.0)
@@ -137,8 +138,14 @@ def test_learned_curved_readback(kernel, spacing, terms):
     dr = ((ox - c["rec"][0]) ** 2 + (oy - c["rec"][1]) ** 2).sqrt()
     tau_axis = c["t_d"] / 2000.0
     vg = (V0 * curved_velocity_at(tau_axis, V0, KG)).sqrt()
-    exp = lattice_times(sur, tau_axis, k, ds, dr, lambda kk: vg[kk],
-                        lambda kk: torch.full(kk.shape, KG, dtype=torch.float64, device="cuda"), spacing)
+    kconst = lambda kk: torch.full(kk.shape, KG, dtype=torch.float64, device="cuda")  # noqa: E731
+    # the network at three pillars of each pillar's group of 8
+    pe, w = pillar_group_ends(pil, nx * ny)
+    ex_, ey_ = c["xa_d"][pe // ny], c["ya_d"][pe % ny]
+    ends = {"ds": ((ex_ - c["src"][0]) ** 2 + (ey_ - c["src"][1]) ** 2).sqrt(),
+            "dr": ((ex_ - c["rec"][0]) ** 2 + (ey_ - c["rec"][1]) ** 2).sqrt(),
+            "v_at": lambda kk, e: vg[kk], "eta_at": lambda kk, e: kconst(kk), "w": w}
+    exp = lattice_times(sur, tau_axis, k, ds, dr, lambda kk: vg[kk], kconst, spacing, ends)
     exp = exp.cpu().numpy().reshape(nx, ny, nt)
     u = exp / 2e-3
     inside = (u > 1.0) & (u < 1199.0 - 1.0)
@@ -186,11 +193,11 @@ def test_learned_curved_job_against_exact(kernel, curved_small):
     import torch
 
     job, ex, sur = curved_small
-    errs = ex.check_surrogate(sur, node_spacing=16)
+    errs = ex.check_surrogate(sur, node_spacing=GEOMETRIC)
     print("per-tile held-out max error (samples)", errs)
     assert all(e <= 0.25 for e in errs)
     exact = _image(ex)
-    learned = _image(ex, surrogate=sur, node_spacing=16)
+    learned = _image(ex, surrogate=sur, node_spacing=GEOMETRIC)
     rel = float(torch.linalg.norm(learned - exact) / torch.linalg.norm(exact))
     saved = ex.config
     ex.config = replace(saved, kernel_type="straight_ray", curved_ray_enabled=False)
@@ -209,6 +216,6 @@ def test_learned_curved_fallback_is_exact(kernel, curved_small):
     import torch
 
     job, ex, sur = curved_small
-    ex.check_surrogate(sur, node_spacing=16, bar=1e-9)
+    ex.check_surrogate(sur, node_spacing=GEOMETRIC, bar=1e-9)
     assert not any(t.learned_ok for t in ex.tiles)
-    assert torch.equal(_image(ex), _image(ex, surrogate=sur, node_spacing=16))
+    assert torch.equal(_image(ex), _image(ex, surrogate=sur, node_spacing=GEOMETRIC))
