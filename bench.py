@@ -283,10 +283,10 @@ def vti_headline(kernel, args) -> dict:
     # term) or splits them in two (--weight-terms 2)
     if USE_GRAPH:
         sur.train(tr, steps=args.train_steps, batch=args.train_batch, log_every=max(args.train_steps // 8, 1),
-                  verbose=True)
+                  qat_fraction=args.qat, verbose=True)
     else:
         sur.train_stepwise(tr, steps=args.train_steps, batch=args.train_batch,
-                           log_every=max(args.train_steps // 8, 1), verbose=True)
+                           log_every=max(args.train_steps // 8, 1), qat_fraction=args.qat, verbose=True)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     held = sur.error_samples(tr, job.dt_ms, bf16=True) if args.train_samples <= 2_000_000 else None
@@ -420,7 +420,7 @@ def vti_headline(kernel, args) -> dict:
                           "heldout_max_samples_per_tile": [round(e, 4) for e in errs],
                           "tiles_learned": len(learned), "tiles_exact_fallback": len(fallback),
                           "training_set_error_samples": held, "forced": bool(args.force_learned),
-                          "weight_terms": args.weight_terms},
+                          "weight_terms": args.weight_terms, "qat_fraction": args.qat},
         },
         "plain_straight_ray": dict(roofline_entry(roofline.FLOPS_PER_CONTRIBUTION["straight"], e_plain, plain_ms,
                                                   "k_migrate (straight ray, weights, linear) on the same subset"),
@@ -717,7 +717,10 @@ def main() -> None:
     ap.add_argument("--weight-terms", type=int, default=1, choices=[1, 2],
                     help="fused inference weights of the f32-trained network: 1 rounded to bf16, 2 as two bf16 "
                          "terms (one more MFMA per product)")
-    ap.add_argument("--lattice", default="8,128,96",
+    ap.add_argument("--qat", type=float, default=0.25,
+                    help="fraction of the training steps (the last ones) run through the fused inference's bf16 "
+                         "numerics (quantisation-aware)")
+    ap.add_argument("--lattice", default="8,100,64",
                     help="t0 lattice of the learned traveltime: smallest spacing, growth (1/256), largest spacing")
     ap.add_argument("--force-learned", action="store_true",
                     help="use the learned traveltime on every subset tile whatever its held-out error "

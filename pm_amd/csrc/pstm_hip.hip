@@ -1304,19 +1304,23 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             if (learned_tt(TT) && tid == 0) s_nrows = 0;
             if (tid < P.batch) {  // wave 0, which builds pairs and issues no row loads
                 if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
-                if (pipe || early_dma) {
-                    const int jn = j0 + P.batch + tid;
-                    s_rownext[par][tid] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
-                }
+                // the next batch's rows: its row loads and its pair table read
+                // them from here
+                const int jn = j0 + P.batch + tid;
+                s_rownext[par][tid] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
             }
             if (pt_thread) {
                 const int pil = pil0 + pt_p;
                 float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
                 float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (pt_j < nb && pil < npil) {
+                    // every global load of the pair issued at once (one memory
+                    // latency): the block row from the previous batch's table
+                    const int trow = j0 == 0 ? (T.tidx ? T.tidx[pt_j] : pt_j) : s_rownext[par ^ 1][pt_j];
+                    const float4 g0 = T.geo[2 * (j0 + pt_j)];
+                    const float4 g1 = T.geo[2 * (j0 + pt_j) + 1];
                     const double pox64 = T.xax[pil / T.ny], poy64 = T.yax[pil % T.ny];
                     const float pox = (float)(pox64 - T.xax[0]), poy = (float)(poy64 - T.yax[0]);
-                    const int trow = T.tidx ? T.tidx[j0 + pt_j] : j0 + pt_j;
                     const double dx64 = __dsub_rn(pox64, P.mx64[trow]);
                     const double dy64 = __dsub_rn(poy64, P.my64[trow]);
                     const double dm64 = dist64(dx64, dy64);
@@ -1325,8 +1329,6 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     const bool in_ap64 = !(dm64 > P.max_ap64);
                     if (in_ap64 || learned_tt(TT)) {
                         inap = in_ap64 ? 1 : 0;
-                        const float4 g0 = T.geo[2 * (j0 + pt_j)];
-                        const float4 g1 = T.geo[2 * (j0 + pt_j) + 1];
                         const float dxs = pox - g0.x, dys = poy - g0.y;
                         const float dxr = pox - g0.z, dyr = poy - g0.w;
                         const float dxm = pox - g1.x, dym = poy - g1.y;

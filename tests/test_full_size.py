@@ -205,7 +205,7 @@ def test_vti_full_size_region(kernel):
     sur = TraveltimeSurrogate.create(norm, seed=0)
     sur.weight_terms = 1
     t0 = time.perf_counter()
-    sur.train(tr, steps=100_000, batch=65536)
+    sur.train(tr, steps=100_000, batch=65536, qat_fraction=0.25)  # as the benchmark trains it
     torch.cuda.synchronize()
     say(f"vti: surrogate trained in {time.perf_counter() - t0:.1f} s")
     del tr

