@@ -11,11 +11,11 @@ FMA = 2, add/sub/mul = 1, sqrt/rsq/rcp/log/exp = 1, compares/floor/convert = 0.
   curved ray 25, VTI 31, triangle anti-aliasing 47, MLP 5-64-64-64-64-2
     25,472 per evaluation: fixed before their first measurement and kept
     regardless of how the kernels evaluate them.
-  learned VTI traveltime, VALU part of a contribution: the anisotropic
-    factor of each leg interpolated between lattice samples (sub + FMA, 3
-    per leg: 6) + per leg h s (1), 1 + c (1), FMA radicand (2) (8) + 2 sqrt +
+  learned VTI traveltime, VALU part of a contribution: the learned factor
+    of each leg from the cubic of its lattice segment (Horner, 3 FMA per leg:
+    12) + per leg h s (1), 1 + eta g (2), FMA radicand (2) (10) + 2 sqrt +
     add (3) + the straight-ray tail (sample index FMA 2, frac 1, lerp 3,
-    rcp 1, t0/(v t^2) 2, pair weight 1, accumulate FMA 2: 12) = 29; the
+    rcp 1, t0/(v t^2) 2, pair weight 1, accumulate FMA 2: 12) = 37; the
     network evaluations on the lattice are MFMA work, counted apart.
 """
 
@@ -27,7 +27,7 @@ FLOPS_PER_CONTRIBUTION = {
     "curved": 25,
     "vti": 31,
     "triangle_aa": 47,
-    "vti_learned": 29,
+    "vti_learned": 37,
 }
 MLP_FLOPS_PER_EVAL = 2 * (5 * 64 + 3 * 64 * 64 + 64 * 2)  # 25,472
 
