@@ -48,8 +48,13 @@ def group_bounds(hdr: dict, key: str | None, n: int, rows_per_group: int) -> lis
 def migrate_streamed(kernel, traces_path, headers_path, x_axis, y_axis, t_axis_ms, config, *, cube=None,
                      vrms_1d=None, eta_cube=None, tile: int = 64, group_key: str | None = "BIN",
                      rows_per_group: int = 100_000, traveltime: int | None = None, surrogate=None,
-                     node_spacing: int = 16, n_buffers: int = 8, n_threads: int = 4) -> StreamedResult:
-    """Stream a Zarr trace array to the GPU and migrate it, group by group."""
+                     node_spacing: int = 16, n_buffers: int = 8, n_threads: int = 4,
+                     region: tuple[int, int, int, int] | None = None) -> StreamedResult:
+    """Stream a Zarr trace array to the GPU and migrate it, group by group.
+    With a learned traveltime, every tile is checked first
+    (TileExecutor.check_surrogate) and a tile that fails migrates with the
+    exact one.  region = (x0, x1, y0, y1) pillars: migrate only the tiles
+    that cover it (the whole image and fold are still allocated)."""
     import torch
 
     from pm_amd.pipeline.executor import TileExecutor
@@ -73,19 +78,35 @@ def migrate_streamed(kernel, traces_path, headers_path, x_axis, y_axis, t_axis_m
     t_start = time.perf_counter()
     ex = TileExecutor(kernel, job, spatial_sort=False)   # rows stay in file order
     groups = group_bounds(hdr, group_key, n, rows_per_group)
-    tt = traveltime
-    plans = [ex.plan(row_range=g, traveltime=tt, surrogate=surrogate, node_spacing=node_spacing)
-             for g in groups]
+    if region is not None:
+        from pm_amd.autotune import region_tiles
+
+        tiles = region_tiles(ex, *region)
+    else:
+        tiles = list(range(len(ex.tiles)))
+    learned, exact = [], tiles
+    if surrogate is not None:
+        # geometry and the model only: the amplitudes have not arrived yet
+        ex.check_surrogate(surrogate, node_spacing=node_spacing, tiles=tiles)
+        learned = [i for i in tiles if ex.tiles[i].learned_ok]
+        exact = [i for i in tiles if not ex.tiles[i].learned_ok]
+    plans = []
+    for g in groups:
+        if learned:
+            plans.append((g, ex.plan(row_range=g, tiles=learned, traveltime=traveltime, surrogate=surrogate,
+                                     node_spacing=node_spacing)))
+        if exact:
+            plans.append((g, ex.plan(row_range=g, tiles=exact)))
     ex.reset()
     streamer = ChunkStreamer(arr, dest, n_buffers=n_buffers, n_threads=n_threads).start()
     try:
-        for g, p in zip(groups, plans):
+        for g, p in plans:
             streamer.wait_rows(*g)
             p.run(use_graph=False)
         torch.cuda.synchronize()
     finally:
         streamer.close()
-        for p in plans:
+        for _, p in plans:
             p.destroy()
     wall = time.perf_counter() - t_start
     img, fold = ex.assemble()

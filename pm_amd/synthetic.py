@@ -305,6 +305,60 @@ def vti_diffractor_job(seed: int = 4, n_traces: int = 500_000, n_samples: int = 
     return job
 
 
+def vti_bins_job(seed: int = 7, n_bins: int = 60, total: int = 2_400_000, n_samples: int = 2001,
+                 n_out: int = 1024, nt: int = 2000, d_out: float = 12.5, tile: int = 16, bin_width: float = 50.0,
+                 n_diffractors: int = 300, device: str = "cpu") -> SyntheticJob:
+    """The largest single-GPU job: common-offset bins b = 0 .. n_bins-1
+    (offsets in [50 b, 50 b + 50) m, uniform midpoints and azimuths) with
+    lognormal counts (about `total` traces, stored bin by bin,
+    meta["bin_start"]), the VTI model of vti_diffractor_job over the survey,
+    random diffractors at their exact two-leg VTI times (30 Hz Ricker,
+    amplitudes accumulated in f32); output n_out x n_out bins x nt samples
+    at 2 ms."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    scale = total / 1_600_000
+    counts = bin_counts(rng, n_bins, total, median=max(1, int(30_000 * scale)), lo=max(1, int(5_000 * scale)),
+                        hi=max(1, int(120_000 * scale)))
+    x_hi = d_out * (n_out - 1)
+    lo, hi = -500.0, x_hi + 500.0
+    parts = []
+    for b, n in enumerate(counts):
+        mx, my = rng.uniform(lo, hi, n), rng.uniform(lo, hi, n)
+        h = rng.uniform(bin_width * b, bin_width * (b + 1), n)
+        az = rng.uniform(0.0, 2.0 * np.pi, n)
+        parts.append((mx - 0.5 * h * np.cos(az), my - 0.5 * h * np.sin(az),
+                      mx + 0.5 * h * np.cos(az), my + 0.5 * h * np.sin(az)))
+    sx, sy, rx, ry = (np.concatenate([p[i] for p in parts]) for i in range(4))
+    vc, ec = vti_model(x_lo=lo, x_hi=hi)
+    dt_s = 0.002
+    tr = torch.zeros((len(sx), n_samples), dtype=torch.float32, device=device)
+    S = [torch.as_tensor(a, device=device) for a in (sx, sy, rx, ry)]
+    xd = rng.uniform(0.0, x_hi, n_diffractors)
+    yd = rng.uniform(0.0, x_hi, n_diffractors)
+    t0d = rng.uniform(0.2, 0.98 * nt * dt_s, n_diffractors)
+    ad = rng.uniform(0.5, 1.0, n_diffractors)
+    vd = vc.at(xd, yd, 1000.0 * t0d)
+    ed = ec.at(xd, yd, 1000.0 * t0d)
+    for i in range(n_diffractors):
+        tau = torch.full_like(S[0], t0d[i] / 2)
+        hs = ((S[0] - xd[i]) ** 2 + (S[1] - yd[i]) ** 2).sqrt()
+        hr = ((S[2] - xd[i]) ** 2 + (S[3] - yd[i]) ** 2).sqrt()
+        vv = torch.full_like(tau, vd[i])
+        ee = torch.full_like(tau, ed[i])
+        t = _vti_leg(tau, hs, vv, ee) + _vti_leg(tau, hr, vv, ee)
+        add_events(tr, t, torch.full_like(t, ad[i]), 30.0, dt_s)
+    x = d_out * np.arange(n_out)
+    cfg = KernelConfig(max_aperture_m=2500.0, min_aperture_m=500.0, max_dip_degrees=60.0,
+                       taper_fraction=0.1, apply_spreading=True, apply_obliquity=True,
+                       grid_dx=d_out, grid_dy=d_out, kernel_type="anisotropic_vti", vti_enabled=True)
+    return SyntheticJob("vti_bins", tr, sx, sy, rx, ry, 2.0, 0.0, x, x.copy(), 2.0 * np.arange(nt), cfg,
+                        cube=vc, tile=tile,
+                        meta={"eta_cube": ec, "bin_start": np.concatenate([[0], np.cumsum(counts)]),
+                              "diffractors": np.stack([xd, yd, t0d], 1)})
+
+
 def curved_diffractor_job(seed: int = 6, n_traces: int = 200_000, n_samples: int = 1501, n_out: int = 128,
                           nt: int = 1000, d_out: float = 12.5, tile: int = 32, n_diffractors: int = 120,
                           v0: float = 1600.0, k: float = 0.6, device: str = "cpu") -> SyntheticJob:

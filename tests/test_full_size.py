@@ -21,6 +21,13 @@ F32_FLOOR_*; the 1e-5 bar holds at 1001 samples, tests/test_gpu_parity.py).
   benchmark, pm_amd.lattice.SPARSE) against exact to image relative L2
   1e-3, with its per-tile held-out check.
 
+* streamed (the largest single-GPU job): 60 common-offset bins, ~2.4M
+  traces x 2001 samples (19.2 GB) written as Zarr v2 + Parquet, evicted from
+  the page cache and streamed to HBM bin by bin while each bin migrates into
+  a 1024 x 1024 x 2000 image (8.4 GB) with the learned VTI traveltime on the
+  stated central 32 x 32 pillars; fold against the oracle on every pillar of
+  the region, image against the exact migration of the streamed traces.
+
 Progress lines name the job and step (each job takes tens of seconds).
 """
 
@@ -215,4 +222,86 @@ def test_vti_full_size_region(kernel):
     learned = ex.tiles[sub[0]].image.double()
     rl = float(torch.linalg.norm(learned - exact) / torch.linalg.norm(exact))
     say(f"vti: learned traveltime in {dt:.1f} s, held-out {errs[0]:.3f} sample, image rel-L2 vs exact {rl:.3e}")
+    assert rl <= 1e-3
+
+
+def test_streamed_vti_bins_full_size(kernel, tmp_path):
+    """The largest single-GPU job: 60 common-offset bins with lognormal counts
+    (~2.4M traces x 2001 samples, 19.2 GB) written as a Zarr v2 array + a
+    Parquet header table, evicted from the page cache, then streamed to HBM
+    through pinned buffers while the bins migrate one by one, as they arrive,
+    into a 1024 x 1024 x 2000 image (8.4 GB f32, the whole image and fold
+    allocated) with the learned VTI traveltime and its per-tile check, on
+    the stated central 32 x 32 pillars.  Fold equal on every pillar of the
+    region to the oracle's over all traces; the learned image within relative
+    L2 1e-3 of the exact-traveltime migration of the streamed traces."""
+    import os
+
+    import torch
+
+    from pm_amd.io.zarr_v2 import write_headers, write_zarr_v2
+    from pm_amd.lattice import SPARSE
+    from pm_amd.pipeline.streaming import migrate_streamed
+    from pm_amd.surrogate import Normalizer, TraveltimeSurrogate, make_training_set
+    from pm_amd.synthetic import vti_bins_job
+
+    say("streamed: building 60 bins, ~2.4M x 2001, 1024 x 1024 x 2000")
+    t0 = time.perf_counter()
+    job = vti_bins_job(device="cuda")
+    torch.cuda.synchronize()
+    n, ns = job.n_traces, job.n_samples
+    assert job.meta["bin_start"][-1] == n and 2_300_000 < n < 2_500_000 and ns == 2001
+    say(f"streamed: {n} traces built in {time.perf_counter() - t0:.1f} s")
+    bins = np.repeat(np.arange(60), np.diff(job.meta["bin_start"]))
+    t0 = time.perf_counter()
+    arr = write_zarr_v2(tmp_path / "traces.zarr", job.amplitudes, chunk_rows=10_000,
+                        attrs={"sample_rate_ms": job.dt_ms, "start_time_ms": job.t_start_ms})
+    write_headers(tmp_path / "headers.parquet", job.source_x, job.source_y, job.receiver_x, job.receiver_y,
+                  extra={"BIN": bins})
+    os.sync()
+    for i in range(arr.n_chunks):   # the reads below come from the disk, not the page cache
+        fd = os.open(tmp_path / "traces.zarr" / f"{i}.0", os.O_RDONLY)
+        os.posix_fadvise(fd, 0, 0, os.POSIX_FADV_DONTNEED)
+        os.close(fd)
+    say(f"streamed: {n * ns * 4 / 1e9:.1f} GB written in {time.perf_counter() - t0:.1f} s")
+    d_max = job.config.max_aperture_m + 1500.0
+    norm = Normalizer(tau_max=2.0, d_max=d_max, v_lo=1500.0, v_hi=4500.0, eta_max=0.2)
+    tr = make_training_set(job.cube, job.meta["eta_cube"], 10_000_000, d_max, norm, seed=1, t_max_ms=4000.0)
+    sur = TraveltimeSurrogate.create(norm, seed=0)
+    t0 = time.perf_counter()
+    sur.train(tr, steps=100_000, batch=65536, qat_fraction=0.25)
+    torch.cuda.synchronize()
+    say(f"streamed: surrogate trained in {time.perf_counter() - t0:.1f} s")
+    del tr
+    x_axis, y_axis, t_axis, cfg, cube, ecube = (job.x_axis, job.y_axis, job.t_axis_ms, job.config, job.cube,
+                                                job.meta["eta_cube"])
+    mx, my = job.midpoint_x, job.midpoint_y
+    del job
+    torch.cuda.empty_cache()
+    region = (496, 528, 496, 528)
+    res = migrate_streamed(kernel, tmp_path / "traces.zarr", tmp_path / "headers.parquet", x_axis, y_axis,
+                           t_axis, cfg, cube=cube, eta_cube=ecube, tile=16, traveltime=_abi.TT_VTI_LEARNED,
+                           surrogate=sur, node_spacing=SPARSE, region=region, n_buffers=8, n_threads=4)
+    ex = res.executor
+    sub = sorted((i for i, t in enumerate(ex.tiles) if t.x0 >= 496 and t.x1 <= 528 and t.y0 >= 496 and t.y1 <= 528),
+                 key=lambda i: (ex.tiles[i].x0, ex.tiles[i].y0))
+    assert len(sub) == 4
+    errs = [ex.tiles[i].heldout_error for i in sub]
+    say(f"streamed: {res.groups} bins, {res.stats.chunks} chunks, {res.stats.bytes_read / 1e9:.1f} GB in "
+        f"{res.wall_s:.1f} s end to end ({res.stats.bytes_read / 1e9 / res.wall_s:.2f} GB/s), held-out "
+        f"{[round(e, 3) for e in errs]}, errors {res.stats.errors}")
+    assert res.groups == 60 and res.stats.chunks == arr.n_chunks and not res.stats.errors
+    assert ex.shape == (1024, 1024, 2000) and ex.image_storage.numel() == 1024 * 1024 * 2000
+    assert all(ex.tiles[i].learned_ok for i in sub)
+    learned = torch.cat([torch.cat([ex.tiles[i].image.double() for i in sub[r:r + 2]], 1)
+                         for r in range(0, 4, 2)], 0)
+    fold = torch.cat([torch.cat([ex.tiles[i].fold for i in sub[r:r + 2]], 1) for r in range(0, 4, 2)], 0)
+    want = orc.fold(x_axis[496:528], y_axis[496:528], mx, my, cfg.max_aperture_m)
+    np.testing.assert_array_equal(fold.cpu().numpy(), want)
+    t0 = time.perf_counter()
+    _run_plan(ex, tiles=sub)   # exact traveltime, the streamed traces
+    exact = torch.cat([torch.cat([ex.tiles[i].image.double() for i in sub[r:r + 2]], 1)
+                       for r in range(0, 4, 2)], 0)
+    rl = float(torch.linalg.norm(learned - exact) / torch.linalg.norm(exact))
+    say(f"streamed: exact pass {time.perf_counter() - t0:.1f} s; learned image rel-L2 vs exact {rl:.3e}")
     assert rl <= 1e-3
