@@ -93,6 +93,9 @@ constexpr int kNPil = kPW * kPG;       // 8 pillars per workgroup
 constexpr int kMaxBatch = 16;          // traces per LDS batch (upper bound)
 constexpr int kMaxHalfWidth = 64;      // largest anti-aliasing half width (aa_max_half_width)
 constexpr int kPF = 12;                // dwords of the next batch's rows each thread holds in flight
+constexpr int kStageBatch = 64 / kNPil;  // batches whose pair table one wave holds
+constexpr int kStageDw = kStageBatch * 12;  // staged pair inputs: 12 dwords per trace
+constexpr size_t kStageBytes = (size_t)(kStageDw + kStageBatch) * 4;  // + the block rows after
 constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS uses the rest
 
 // traveltimes read from the fused network's t0 lattice
@@ -153,6 +156,7 @@ struct KParams {
     int rows_bytes;              // bytes of one row buffer
     int pair_off;                // byte offset of the pair table in LDS
     int tot_off;                 // byte offset of the f64 running totals in LDS
+    int stage_off;               // byte offset of the staged pair inputs in LDS
     float taper_off;             // x_t = d * slope + taper_off (taper_off = -(1-tf)/tf)
     int n_items;                 // work items over all tiles (pillar groups x time windows)
     int wpow;                    // 0, 1 or 2: power of 1/t in the weight
@@ -866,6 +870,14 @@ __device__ __forceinline__ float interp_at(const float* row, int i0, float f0, i
 // a compare per call)
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 __device__ __forceinline__ unsigned long long wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+// Workgroup barrier for data in LDS: this wave's LDS accesses complete
+// (lgkmcnt(0)), then s_barrier, and no memory access moves across it.  Unlike
+// __syncthreads it does not wait for vector memory operations: the fence of
+// __syncthreads counts direct global -> LDS loads as LDS writes and waits
+// for them all (vmcnt(0)), so row loads issued early would land before the
+// next barrier of every phase.  A direct load's own wave waits for it
+// (vmcnt(0)) before a barrier after which other waves read its data.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Trace amplitude at sample position i0 + f0: linear interpolation
 // (interpolation.py:68-92) or another interpolator (interp_at), or, with
@@ -994,6 +1006,18 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     __shared__ int s_rownext[2][kMaxBatch];
     __shared__ float2 s_invsq[kMaxHalfWidth + 2];  // AA: {1 / l^2, 1 / (l + 1)^2}
     __shared__ int s_nrows;  // learned traveltime: lattice rows listed for evaluation
+    // per batch parity: the batch's first trace j0 when some pair of the batch
+    // is inside the maximum aperture (written by the waves holding such a pair)
+    __shared__ int s_any[2];
+    // the next batch's pair inputs, loaded during the batch before by direct
+    // global -> LDS loads of wave 0 (batches whose pair table wave 0 holds
+    // alone): per trace 12 dwords, the geometry {sx, sy, rx, ry}, {mx, my, w, 0}
+    // and the f64 midpoint (x, y); and the block rows of the batch after it
+    // (in the dynamic LDS, after the pair table: arrays written by direct
+    // loads alone must not be static ones, whose reads the compiler may fold
+    // as never written)
+    uint32_t* s_stage = reinterpret_cast<uint32_t*>(smem + P.stage_off);
+    int* s_stage_row = reinterpret_cast<int*>(smem + P.stage_off + kStageDw * 4);
     if (AA) {
         for (int i = tid; i < kMaxHalfWidth + 2; i += kThreads)
             s_invsq[i] = make_float2(i > 0 ? 1.0f / (float)(i * i) : 0.0f, 1.0f / (float)((i + 1) * (i + 1)));
@@ -1198,19 +1222,19 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // are written to LDS after the batch's last barrier.
         uint32_t pf[kPF];
         const int rl_dw = P.rowlen * (int)(sizeof(Row) / 4);
-        // row jrow of the tile's list (r-th of its batch); its block row from
-        // (wave-uniform) rsrc: kRowsGlobal the tile's list in global memory,
-        // kRowsNext + parity s_rownext.  The LDS array is indexed by name so
-        // its reads stay LDS reads: through a plain pointer they would be flat
-        // loads, whose wait also waits for the row loads in flight
-        constexpr int kRowsGlobal = 0, kRowsNext = 1;
-        auto row_dw = [&](int jrow, int rsrc, int r) -> const uint32_t* {
+        // row jrow of the tile's list (r-th of its batch), its first dword in
+        // global memory: the block row from s_rownext[rpar] (written by the
+        // batch before; for the first batch at the item start), or the tile's
+        // own order for AA rows read in place.  The LDS array is indexed by
+        // name so its reads stay LDS reads: through a plain pointer they would
+        // be flat loads, whose wait also waits for the row loads in flight
+        auto row_dw = [&](int jrow, int rpar, int r) -> const uint32_t* {
             if (AA && !P.prow_global) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)jrow * P.rowlen);
-            const int rw = rsrc >= kRowsNext ? s_rownext[rsrc - kRowsNext][r] : (T.tidx ? T.tidx[jrow] : jrow);
+            const int rw = s_rownext[rpar][r];
             if constexpr (AA) return reinterpret_cast<const uint32_t*>(P.prow + (size_t)rw * P.rowlen);
             else return reinterpret_cast<const uint32_t*>(P.amp + (size_t)rw * P.ns);
         };
-        auto prefetch = [&](int jb, int rsrc) {
+        auto prefetch = [&](int jb, int rpar) {
             const int nbb = min(P.batch, T.ntr - jb);
             if (nbb <= 0) return;
             const int n = nbb * rl_dw;
@@ -1219,7 +1243,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 c -= rl_dw;
                 ++r;
             }
-            const uint32_t* src = row_dw(jb + min(r, nbb - 1), rsrc, min(r, nbb - 1));
+            const uint32_t* src = row_dw(jb + min(r, nbb - 1), rpar, min(r, nbb - 1));
 #pragma unroll
             for (int i = 0; i < kPF; ++i) {
                 if (tid + i * kThreads < n) pf[i] = src[c];
@@ -1230,46 +1254,55 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     ++r;
                     moved = true;
                 }
-                if (moved && r < nbb) src = row_dw(jb + r, rsrc, r);
+                if (moved && r < nbb) src = row_dw(jb + r, rpar, r);
             }
         };
-        // AA rows: direct global -> LDS loads, 64 dwords per wave instruction
-        // (wave-uniform LDS destination, per-lane source), into the second of
-        // two row buffers when pipelined, else into the one buffer
+        // Rows by direct global -> LDS loads (AA rows; the learned traveltime's
+        // f32 rows), into the second of two row buffers when pipelined, else
+        // into the one buffer: (row, 64-dword chunk) units dealt round-robin
+        // over waves w0.., one wave instruction each (wave-uniform LDS
+        // destination, one dword per lane).  The batch's block rows are read
+        // from LDS once per call and the unit's row taken from lane r, so
+        // that the loop holds no memory read: a read whose result the loop
+        // waited for would also wait for every row load issued before it.
         int cur = 0;
-        // waves w0.. issue the loads (chunk c of 64 dwords by wave w0 + c % (kWaves - w0),
-        // the same wave for a given LDS chunk at every call)
-        auto dma = [&](int jb, int buf, int rsrc, int w0) {
+        auto dma = [&](int jb, int buf, int rpar, int w0) {
             if constexpr (AA || learned_tt(TT)) {
                 const int nbb = min(P.batch, T.ntr - jb);
                 if (nbb <= 0 || wave < w0) return;
                 const int nw = kWaves - w0;
-                const int n = nbb * rl_dw;
+                const int nck = (rl_dw + 63) >> 6;
                 uint32_t* dst0 = reinterpret_cast<uint32_t*>(smem + (size_t)buf * P.rows_bytes);
-                int e = (wave - w0) * 64 + lane;
-                int r = 0, col = e;
-                while (col >= rl_dw) {
-                    col -= rl_dw;
+                const bool listed = !(AA && !P.prow_global);
+                const int lrow = listed && lane < nbb ? s_rownext[rpar][lane] : jb + lane;
+                int r = 0, ck = wave - w0;
+                while (ck >= nck) {
+                    ck -= nck;
                     ++r;
                 }
-                const uint32_t* src = row_dw(jb + min(r, nbb - 1), rsrc, min(r, nbb - 1));
-                for (int c = wave - w0; c * 64 < n; c += nw) {
-                    if (e < n) __builtin_amdgcn_global_load_lds(src + col, dst0 + c * 64, 4, 0, 0);
-                    e += nw * 64;
-                    col += nw * 64;
-                    bool moved = false;
-                    while (col >= rl_dw) {
-                        col -= rl_dw;
+                while (r < nbb) {
+                    const int rw = __builtin_amdgcn_readlane(lrow, r);
+                    const uint32_t* src;
+                    if constexpr (AA) src = reinterpret_cast<const uint32_t*>(P.prow + (size_t)rw * P.rowlen);
+                    else src = reinterpret_cast<const uint32_t*>(P.amp + (size_t)rw * P.ns);
+                    const int col = ck * 64 + lane;
+                    if (col < rl_dw) __builtin_amdgcn_global_load_lds(src + col, dst0 + r * rl_dw + ck * 64, 4, 0, 0);
+                    ck += nw;
+                    while (ck >= nck) {
+                        ck -= nck;
                         ++r;
-                        moved = true;
                     }
-                    if (moved && r < nbb) src = row_dw(jb + r, rsrc, r);
                 }
             }
         };
+        // the first batch's block rows where every later batch finds them
+        // (s_rownext[1], parity of the batch before the first)
+        if (tid < P.batch) s_rownext[1][tid] = tid < T.ntr ? (T.tidx ? T.tidx[tid] : tid) : 0;
+        if (tid == 0) s_any[0] = s_any[1] = -1;
+        __syncthreads();
         if (pipe) {
-            if (AA) dma(0, 0, kRowsGlobal, 0);
-            else prefetch(0, kRowsGlobal);
+            if (AA) dma(0, 0, 1, 0);
+            else prefetch(0, 1);
         }
 
         // Unpipelined AA staging (early_dma): the batch's f64 rows go by direct
@@ -1282,9 +1315,43 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // own global reads would wait behind the row loads.
         const bool early_dma = (AA || learned_tt(TT)) && !pipe;
         const int w_pairs = (kNPil * P.batch + 63) / 64;  // waves holding pair-table threads
+        // Pair inputs one batch ahead (w_pairs == 1): wave 0 loads batch jb's
+        // geometry and midpoints (rows from s_rownext[rpar]) and batch jb +
+        // batch's block rows into s_stage / s_stage_row after the pair-table
+        // barrier of the batch before, so that the pair table (on which every
+        // wave waits) reads LDS only; the loads land during that batch.  Only
+        // wave 0 reads them, after its own vmcnt wait.
+        const bool staged_pairs = w_pairs == 1;
+        auto stage = [&](int jb, int rpar) {
+            const int nbb = min(P.batch, T.ntr - jb);
+            if (nbb <= 0) return;
+            for (int c = 0; c * 64 < nbb * 12; ++c) {
+                const int d = c * 64 + lane;
+                if (d < nbb * 12) {
+                    const int jt = d / 12, f = d - jt * 12;
+                    const uint32_t* src;
+                    if (f < 8) {
+                        src = reinterpret_cast<const uint32_t*>(T.geo + 2 * (jb + jt)) + f;
+                    } else {
+                        const int trow = s_rownext[rpar][jt];
+                        src = reinterpret_cast<const uint32_t*>((f < 10 ? P.mx64 : P.my64) + trow) + (f & 1);
+                    }
+                    __builtin_amdgcn_global_load_lds(src, s_stage + c * 64, 4, 0, 0);
+                }
+            }
+            const int jn = jb + P.batch + lane;
+            if (T.tidx && lane < P.batch && jn < T.ntr) __builtin_amdgcn_global_load_lds(T.tidx + jn, s_stage_row, 4, 0, 0);
+        };
         for (int j0 = 0, par = 0; j0 < T.ntr; j0 += P.batch, par ^= 1) {
             const int nb = min(P.batch, T.ntr - j0);
-            if (early_dma) dma(j0, 0, j0 == 0 ? kRowsGlobal : kRowsNext + (par ^ 1), w_pairs);
+            const bool staged = staged_pairs && j0 > 0;
+            if (staged && wave == 0) {
+                // this wave's staging loads (issued a batch ago) have landed;
+                // no LDS read of them may move above the wait
+                __builtin_amdgcn_s_waitcnt(0x0F70);
+                asm volatile("" ::: "memory");
+            }
+            if (early_dma) dma(j0, 0, par ^ 1, w_pairs);
             // pipelined f32 rows, loaded into registers during the batch
             // before (or before the first): to LDS at the batch start, while
             // the row buffer is free (the batch before ended with a barrier),
@@ -1303,26 +1370,40 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             int inap = 0;
             if (learned_tt(TT) && tid == 0) s_nrows = 0;
             if (tid < P.batch) {  // wave 0, which builds pairs and issues no row loads
-                if (tid < nb) s_row[tid] = T.tidx ? T.tidx[j0 + tid] : j0 + tid;
+                if (tid < nb) s_row[tid] = s_rownext[par ^ 1][tid];
                 // the next batch's rows: its row loads and its pair table read
                 // them from here
                 const int jn = j0 + P.batch + tid;
-                s_rownext[par][tid] = jn < T.ntr ? (T.tidx ? T.tidx[jn] : jn) : 0;
+                s_rownext[par][tid] = jn < T.ntr ? (T.tidx ? (staged ? s_stage_row[tid] : T.tidx[jn]) : jn) : 0;
             }
             if (pt_thread) {
                 const int pil = pil0 + pt_p;
                 float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
                 float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (pt_j < nb && pil < npil) {
-                    // every global load of the pair issued at once (one memory
-                    // latency): the block row from the previous batch's table
-                    const int trow = j0 == 0 ? (T.tidx ? T.tidx[pt_j] : pt_j) : s_rownext[par ^ 1][pt_j];
-                    const float4 g0 = T.geo[2 * (j0 + pt_j)];
-                    const float4 g1 = T.geo[2 * (j0 + pt_j) + 1];
+                    // staged inputs from LDS; else every global load of the pair
+                    // issued at once (one memory latency), the block row from
+                    // the previous batch's table
+                    float4 g0, g1;
+                    double mx64, my64;
+                    if (staged) {
+                        const float4* s4 = reinterpret_cast<const float4*>(s_stage) + 3 * pt_j;
+                        const double* s8 = reinterpret_cast<const double*>(s_stage) + 6 * pt_j;
+                        g0 = s4[0];
+                        g1 = s4[1];
+                        mx64 = s8[4];
+                        my64 = s8[5];
+                    } else {
+                        const int trow = s_rownext[par ^ 1][pt_j];
+                        g0 = T.geo[2 * (j0 + pt_j)];
+                        g1 = T.geo[2 * (j0 + pt_j) + 1];
+                        mx64 = P.mx64[trow];
+                        my64 = P.my64[trow];
+                    }
                     const double pox64 = T.xax[pil / T.ny], poy64 = T.yax[pil % T.ny];
                     const float pox = (float)(pox64 - T.xax[0]), poy = (float)(poy64 - T.yax[0]);
-                    const double dx64 = __dsub_rn(pox64, P.mx64[trow]);
-                    const double dy64 = __dsub_rn(poy64, P.my64[trow]);
+                    const double dx64 = __dsub_rn(pox64, mx64);
+                    const double dy64 = __dsub_rn(poy64, my64);
                     const double dm64 = dist64(dx64, dy64);
                     // learned traveltimes: the leg terms of every pillar, also
                     // outside the aperture (an end pillar's network inputs)
@@ -1356,13 +1437,18 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 pairs[(pt_p * P.batch + pt_j) * kPairVec] = e;
                 if (kPairVec == 2) pairs[(pt_p * P.batch + pt_j) * kPairVec + 1] = f;
             }
-            if (!__syncthreads_or(inap)) {
+            // the written j0 tells this batch from the last one of its parity
+            if (wave_any(inap) && lane == 0) s_any[par] = j0;
+            lds_barrier();
+            const bool any_inap = __builtin_amdgcn_readfirstlane(s_any[par]) == j0;
+            if (staged_pairs && wave == 0) stage(j0 + P.batch, par);
+            if (!any_inap) {
                 if (pipe) {  // this batch's rows are not needed
                     if (AA) {
-                        dma(j0 + P.batch, cur ^ 1, kRowsNext + par, 0);
+                        dma(j0 + P.batch, cur ^ 1, par, 0);
                         cur ^= 1;
                     } else {
-                        prefetch(j0 + P.batch, kRowsNext + par);
+                        prefetch(j0 + P.batch, par);
                     }
                 }
                 continue;
@@ -1401,7 +1487,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     base = __shfl(base, 0, 64);
                     if (valid) rlist[base + (int)__popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)r;
                 }
-                __syncthreads();
+                lds_barrier();
                 const int nrows = s_nrows;
                 const float4* ntab = reinterpret_cast<const float4*>(smem + P.node_off);
                 LdsA* wa = (LdsA*)(smem + P.mlp_off);
@@ -1485,10 +1571,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
             // pipelined f32 rows are visible since the pair-table barrier: only
             // DMA'd or loaded rows and the lattice need this one
-            if (AA || !pipe || learned_tt(TT)) __syncthreads();
+            if (AA || !pipe || learned_tt(TT)) lds_barrier();
             if (pipe) {  // in flight during the batch
-                if (AA) dma(j0 + P.batch, cur ^ 1, kRowsNext + par, 0);
-                else prefetch(j0 + P.batch, kRowsNext + par);
+                if (AA) dma(j0 + P.batch, cur ^ 1, par, 0);
+                else prefetch(j0 + P.batch, par);
             }
             if (learned_tt(TT)) {
                 // per (pillar, trace, window segment) the cubic Hermite
@@ -1553,7 +1639,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     coef[2 * e] = cs4;
                     coef[2 * e + 1] = cr4;
                 }
-                __syncthreads();
+                lds_barrier();
             }
 
             // the traces of the batch; EARLY: this wave holds t0 <= 2 ms samples,
@@ -1751,12 +1837,12 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     else tot[(p * MW + m) * kThreads + tid] += (double)acc[p][m];
                     acc[p][m] = 0.0f;
                 }
-            __syncthreads();
+            lds_barrier();
             if (AA && pipe) cur ^= 1;
         }
         // a skipped last batch leaves its row loads in flight: none may outlive
         // the item (nor the workgroup, whose LDS is then reassigned)
-        if (early_dma) __builtin_amdgcn_s_waitcnt(0x0F70);
+        if (early_dma || staged_pairs) __builtin_amdgcn_s_waitcnt(0x0F70);
 
         // the workgroup is the only writer of these cells
 #pragma unroll
@@ -2086,7 +2172,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     // (the learned traveltime always stages by direct loads at the batch start)
     sh.pipe = pipe == 1 && pipe_batch >= 1 && !sh.learned;
     const int nbuf = (sh.pipe && sh.aa) ? 2 : 1;
-    // LDS: trace rows + pair table + f64 totals; learned: + lattice, widest
+    // LDS: trace rows + pair table + staged pair inputs + f64 totals; learned: + lattice, widest
     // apertures, row list, node inputs, network operands (kept for the
     // workgroup's life)
     // learned: per (evaluated pillar, trace, extended node) the network
@@ -2097,7 +2183,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
         const size_t nx = (size_t)sh.lat_nodes + 2;
         return align_up((size_t)nbuf * align_up((size_t)b * sh.rowlen * sh.elem, 16) +
                             (size_t)kNPil * b * sh.pair_vec * 16, 16) +
-               sh.tot_bytes +
+               kStageBytes + sh.tot_bytes +
                (sh.learned ? align_up((size_t)3 * b * nx * 8, 16) + (size_t)kNPil * b * sh.lat_nodes * 32 +
                                  align_up((size_t)kNPil * nx * 4, 16) + align_up((size_t)3 * b * nx * 2, 16) +
                                  (size_t)kNPil * nx * 16 + (size_t)sh.mlp_terms * kAVec8 * 16 +
@@ -2148,7 +2234,8 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.pipe = sh.pipe ? 1 : 0;
     kp.rows_bytes = (int)align_up((size_t)sh.batch * sh.rowlen * sh.elem, 16);
     kp.pair_off = kp.rows_bytes * ((sh.pipe && sh.aa) ? 2 : 1);
-    kp.tot_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * sh.batch * sh.pair_vec * 16, 16);
+    kp.stage_off = (int)align_up((size_t)kp.pair_off + (size_t)kNPil * sh.batch * sh.pair_vec * 16, 16);
+    kp.tot_off = kp.stage_off + (int)kStageBytes;
     kp.taper_off = a.taper_fraction > 0.0 ? (float)(-(1.0 - a.taper_fraction) / a.taper_fraction) : -1e30f;
     kp.wpow = (a.apply_spreading ? 1 : 0) + (a.apply_obliquity ? 1 : 0);
     kp.interp = a.interpolation;
