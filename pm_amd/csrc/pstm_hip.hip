@@ -877,6 +877,17 @@ __device__ __forceinline__ unsigned long long wave_mask(bool p) { return __built
 // for them all (vmcnt(0)), so row loads issued early would land before the
 // next barrier of every phase.  A direct load's own wave waits for it
 // (vmcnt(0)) before a barrier after which other waves read its data.
+// a / d for 0 <= a < 2^16 and 2 <= d < 2^16 by one multiply-high with
+// m = div_magic(d) = floor((2^32 - 1) / d) + 1: a m / 2^32 exceeds a / d by
+// less than a / 2^32 < 2^-16, less than the distance 1 / d from a / d's
+// fraction to the next integer, so the floor is exact.  The magic number of
+// a wave-uniform divisor is kept in a scalar register (a divisor held in a
+// vector register across the batch loop was spilled to scratch memory, whose
+// reload waits for every vector memory load in flight).
+__device__ __forceinline__ unsigned div_magic(int d) {
+    return __builtin_amdgcn_readfirstlane(d > 1 ? 0xFFFFFFFFu / (unsigned)d + 1u : 0u);
+}
+__device__ __forceinline__ int div_by(int a, unsigned m, int d) { return d > 1 ? (int)__umulhi((unsigned)a, m) : a; }
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Trace amplitude at sample position i0 + f0: linear interpolation
@@ -1009,6 +1020,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     // per batch parity: the batch's first trace j0 when some pair of the batch
     // is inside the maximum aperture (written by the waves holding such a pair)
     __shared__ int s_any[2];
+    // learned traveltime: per pillar of the group its Lagrange weights on
+    // pillars 0, p_mid and p_hi (the item's, written at the item start)
+    __shared__ float4 s_lw[kNPil];
     // the next batch's pair inputs, loaded during the batch before by direct
     // global -> LDS loads of wave 0 (batches whose pair table wave 0 holds
     // alone): per trace 12 dwords, the geometry {sx, sy, rx, ry}, {mx, my, w, 0}
@@ -1299,6 +1313,20 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // (s_rownext[1], parity of the batch before the first)
         if (tid < P.batch) s_rownext[1][tid] = tid < T.ntr ? (T.tidx ? T.tidx[tid] : tid) : 0;
         if (tid == 0) s_any[0] = s_any[1] = -1;
+        if (learned_tt(TT) && tid < kNPil) {
+            float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+            const float fp = (float)tid;
+            if (p_hi >= 2) {
+                const float b = (float)p_mid, c = (float)p_hi;
+                w0 = (fp - b) * (fp - c) / (b * c);
+                w1 = fp * (fp - c) / (b * (b - c));
+                w2 = fp * (fp - b) / (c * (c - b));
+            } else {
+                w2 = p_hi == 1 ? fp : 0.0f;
+                w0 = 1.0f - w2;
+            }
+            s_lw[tid] = make_float4(w0, w1, w2, 0.0f);
+        }
         __syncthreads();
         if (pipe) {
             if (AA) dma(0, 0, 1, 0);
@@ -1464,6 +1492,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 // rows per wavefront step
                 const int nx = nn + 2;
                 const int R = 3 * P.batch * nx;
+                const unsigned m_bnx = div_magic(P.batch * nx), m_nx = div_magic(nx);
                 float2* lat = reinterpret_cast<float2*>(smem + P.lat_off);
                 const float* cimin = reinterpret_cast<const float*>(smem + P.lat_cimin_off);
                 unsigned short* rlist = reinterpret_cast<unsigned short*>(smem + P.lat_list_off);
@@ -1471,8 +1500,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     const int r = r0 + tid;
                     bool valid = false;
                     if (r < R) {
-                        const int pe = r / (P.batch * nx), jn = r - pe * (P.batch * nx);
-                        const int jt = jn / nx, node = jn - jt * nx;
+                        const int pe = div_by(r, m_bnx, P.batch * nx), jn = r - pe * (P.batch * nx);
+                        const int jt = div_by(jn, m_nx, nx), node = jn - jt * nx;
                         if (pe == 0 || (pe == 1 && p_hi >= 2) || (pe == 2 && p_hi >= 1)) {
                             for (int p = 0; p <= p_hi && !valid; ++p) {
                                 const float ez = pairs[(p * P.batch + jt) * kPairVec].z;
@@ -1498,8 +1527,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     const int r = valid ? (int)rlist[li] : 0;
                     float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                     if (valid) {
-                        const int pe = r / (P.batch * nx), jn = r - pe * (P.batch * nx);
-                        const int jt = jn / nx, node = jn - jt * nx;
+                        const int pe = div_by(r, m_bnx, P.batch * nx), jn = r - pe * (P.batch * nx);
+                        const int jt = div_by(jn, m_nx, nx), node = jn - jt * nx;
                         const int pend = pe == 0 ? 0 : (pe == 1 ? p_mid : p_hi);
                         const float4 e = pairs[(pend * P.batch + jt) * kPairVec];
                         const float4 nd = ntab[pend * nx + node];  // {tau, s, v enc., eta enc.}
@@ -1509,12 +1538,12 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                             const float q = tau * tau;
                             const float hs = e.x * nd.y, hr = e.y * nd.y;
                             const float ts2 = q + hs, tr2 = q + hr;
-                            sn_s = ts2 > 0.0f ? fsqrt(hs / ts2) : 0.0f;
-                            sn_r = tr2 > 0.0f ? fsqrt(hr / tr2) : 0.0f;
+                            sn_s = ts2 > 0.0f ? fsqrt(hs * frcp(ts2)) : 0.0f;
+                            sn_r = tr2 > 0.0f ? fsqrt(hr * frcp(tr2)) : 0.0f;
                         } else {
                             // curved: sines k h / sqrt(vg^2 + k^2 h^2)
-                            sn_s = fsqrt(e.x / (e.x + nd.y));
-                            sn_r = fsqrt(e.y / (e.y + nd.y));
+                            sn_s = fsqrt(e.x * frcp(e.x + nd.y));
+                            sn_r = fsqrt(e.y * frcp(e.y + nd.y));
                         }
                         x[0] = fmaf(tau, P.m_tau_sc, -1.0f);
                         x[1] = fmaf(2.0f, sn_s, -1.0f);
@@ -1589,25 +1618,16 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 const int* npos = reinterpret_cast<const int*>(smem + P.lat_pos_off);
                 const float2* lat = reinterpret_cast<const float2*>(smem + P.lat_off);
                 float4* coef = reinterpret_cast<float4*>(smem + P.coef_off);
+                const unsigned m_nseg = div_magic(nseg), m_b = div_magic(P.batch);
                 for (int e = tid; e < kNPil * P.batch * nseg; e += kThreads) {
-                    const int pp = e / nseg, sg = e - pp * nseg;
-                    const int p = pp / P.batch, jt = pp - p * P.batch;
+                    const int pp = div_by(e, m_nseg, nseg), sg = e - pp * nseg;
+                    const int p = div_by(pp, m_b, P.batch), jt = pp - p * P.batch;
                     const int j = jf + sg;
                     float4 cs4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cr4 = cs4;
                     if (p <= p_hi) {
                         // weights of pillars 0, p_mid, p_hi for pillar p
-                        float w0, w1, w2;
-                        const float fp = (float)p;
-                        if (p_hi >= 2) {
-                            const float b = (float)p_mid, c = (float)p_hi;
-                            w0 = (fp - b) * (fp - c) / (b * c);
-                            w1 = fp * (fp - c) / (b * (b - c));
-                            w2 = fp * (fp - b) / (c * (c - b));
-                        } else {
-                            w2 = p_hi == 1 ? fp : 0.0f;
-                            w0 = 1.0f - w2;
-                            w1 = 0.0f;
-                        }
+                        const float4 lw = s_lw[p];
+                        const float w0 = lw.x, w1 = lw.y, w2 = lw.z;
                         const float2* ra = lat + jt * nx + sg;                  // pe 0, nodes j - 1 .. j + 2
                         const float2* rb = lat + (P.batch + jt) * nx + sg;      // pe 1
                         const float2* rc = lat + (2 * P.batch + jt) * nx + sg;  // pe 2
