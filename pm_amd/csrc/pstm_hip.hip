@@ -462,20 +462,24 @@ __device__ __forceinline__ bool exact_decision(const KParams& P, const TileView&
 // ---------------------------------------------------------------------------
 // fused traveltime surrogate: BF16 MFMA, f32 accumulation
 // ---------------------------------------------------------------------------
-// One wavefront evaluates the network for 32 rows with gfx950's
-// v_mfma_f32_32x32x16_bf16 in the transposed form D = W . X^T: A = weights
-// (32 outputs x 16 inputs), B = activations (16 inputs x 32 rows), D = 32
-// outputs x 32 rows.  Lane l holds row l % 32 and, of an operand, the 8
-// inputs 8 (l / 32) .. 8 (l / 32) + 7 of the K step; its accumulator register
-// r holds output 8 (r / 4) + 4 (l / 32) + r % 4 of the M tile.  The 8
-// registers 8 s .. 8 s + 7 of M tile mt are therefore taken as the B operand
-// of K step j = 2 mt + s of the next layer, and the next layer's weights are
+// One wavefront evaluates the network for 16 rows with gfx950's
+// v_mfma_f32_16x16x32_bf16 in the transposed form D = W . X^T: A = weights
+// (16 outputs x 32 inputs), B = activations (32 inputs x 16 rows), D = 16
+// outputs x 16 rows.  Lane l holds row l % 16 and, of an operand, 8 of the
+// K step's 32 inputs (lane group g = l / 16: the same 8 in A and B, so the
+// products pair up whatever order the hardware sums them in); its
+// accumulator register r holds output 4 g + r of the M tile.  The registers
+// of M tiles 2 j and 2 j + 1 are therefore taken, as they are, as the B
+// operand of K step j of the next layer, and the next layer's weights are
 // laid out for that order of its inputs (mlp_unit): activations never leave
-// registers.  The input and every hidden activation enter as a bf16 pair
-// hi + lo (two products per K step); the weights as bf16, or as two bf16
-// terms (one more product).
+// registers.  Sixteen rows (not 32 with the 32 x 32 form) keep the
+// accumulators and operands of a layer in 32 registers, so that the gather
+// around the evaluation holds its own per-sample terms without spilling.
+// The input and every hidden activation enter as a bf16 pair hi + lo (two
+// products per K step); the weights as bf16, or as two bf16 terms (one more
+// product).
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kMlpLayers = 5;
 __host__ __device__ constexpr int mlp_in(int l) { return l == 0 ? PSTM_MLP_IN : PSTM_MLP_HIDDEN; }
@@ -486,21 +490,19 @@ __host__ __device__ constexpr int mlp_w_off(int l) {
     return off;
 }
 static_assert(mlp_w_off(kMlpLayers) == PSTM_MLP_PARAMS, "network layout");
+static_assert(PSTM_MLP_HIDDEN == 64 && PSTM_MLP_IN <= 8 && PSTM_MLP_OUT <= 4, "operand tiling");
 
-// weight operands, in bf16x8 units: layer 0 [mt 2][lane 64]; layers 1-3
-// [mt 2][j 4][lane 64]; layer 4 (outputs padded to 32) [j 4][lane 64]; the
+// weight operands, in bf16x8 units: layer 0 [mt 4][lane 64]; layers 1-3
+// [mt 4][j 2][lane 64]; layer 4 (outputs padded to 16) [j 2][lane 64]; the
 // remainders of split weights follow as a second image of kAVec8 vectors
-constexpr int kA1 = 2 * 64, kA4 = kA1 + 3 * 8 * 64, kAVec8 = kA4 + 4 * 64, kAVals = kAVec8 * 8;
-// biases in accumulator layout, in f32x16 units: vector v (layers 0-3:
-// 2 layer + mt, layer 4: 8) for lane half h at 2 v + h; the accumulator
-// layout depends on the lane only through its half, so every lane of a half
-// reads the same 64 bytes
-constexpr int kBVec = 4 * 2 + 1, kBVals = kBVec * 2 * 16;
+constexpr int kA1 = 4 * 64, kA4 = kA1 + 3 * 8 * 64, kAVec8 = kA4 + 2 * 64, kAVals = kAVec8 * 8;
+// biases in accumulator layout, in f32x4 units: vector v (layers 0-3:
+// 4 layer + mt, layer 4: 16) for lane group g at 4 v + g; the accumulator
+// layout depends on the lane only through its group
+constexpr int kBVec = 4 * 4 + 1, kBVals = kBVec * 4 * 4;
 
-// hidden unit carried by element e of lane half h of K step j (see above)
-__host__ __device__ constexpr int mlp_unit(int j, int h, int e) {
-    return 32 * (j / 2) + 16 * (j % 2) + (e < 4 ? 4 * h + e : 8 + 4 * h + (e - 4));
-}
+// hidden unit carried by element e of lane group g of K step j (see above)
+__host__ __device__ constexpr int mlp_unit(int j, int g, int e) { return 32 * j + 16 * (e >> 2) + 4 * g + (e & 3); }
 
 __host__ __device__ inline unsigned short bf16_bits(float f) {
     unsigned u = __builtin_bit_cast(unsigned, f);
@@ -518,21 +520,21 @@ __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restri
             l = 0; mt = v / 64; j = 0; lane = v % 64;
         } else if (v < kA4) {
             const int r = v - kA1;
-            l = 1 + r / (8 * 64); mt = (r / (4 * 64)) % 2; j = (r / 64) % 4; lane = r % 64;
+            l = 1 + r / (8 * 64); mt = (r / (2 * 64)) % 4; j = (r / 64) % 2; lane = r % 64;
         } else {
             l = 4; mt = 0; j = (v - kA4) / 64; lane = (v - kA4) % 64;
         }
-        const int o = mt * 32 + lane % 32, h = lane / 32;
-        const int i = l == 0 ? 8 * h + el : mlp_unit(j, h, el);
+        const int o = mt * 16 + lane % 16, g = lane / 16;
+        const int i = l == 0 ? 8 * g + el : mlp_unit(j, g, el);
         const float w = (o < mlp_out(l) && i < mlp_in(l)) ? P[mlp_w_off(l) + o * mlp_in(l) + i] : 0.0f;
         const unsigned short hi = bf16_bits(w);
         A[e] = hi;
         // the weight's remainder as a second bf16 term (split-weight inference)
         A[kAVals + e] = bf16_bits(w - __builtin_bit_cast(float, (unsigned)hi << 16));
     } else if (e < kAVals + kBVals) {
-        const int f = e - kAVals, v = f / 32, h = (f / 16) % 2, r = f % 16;
-        const int l = v < 8 ? v / 2 : 4, mt = v < 8 ? v % 2 : 0;
-        const int o = mt * 32 + 8 * (r / 4) + 4 * h + r % 4;
+        const int f = e - kAVals, v = f / 16, g = (f / 4) % 4, r = f % 4;
+        const int l = v < 16 ? v / 4 : 4, mt = v < 16 ? v % 4 : 0;
+        const int o = mt * 16 + 4 * g + r;
         B[f] = o < mlp_out(l) ? P[mlp_w_off(l) + mlp_in(l) * mlp_out(l) + o] : 0.0f;
     }
 }
@@ -549,13 +551,13 @@ __device__ __forceinline__ void split8(const float (&a)[8], bf16x8_t& hi, bf16x8
     }
 }
 
-// SiLU of the two M tiles, re-packed as the next layer's four B operands
-__device__ __forceinline__ void act_pack(const f32x16_t (&acc)[2], bf16x8_t (&h)[4], bf16x8_t (&l)[4]) {
+// SiLU of the four M tiles, re-packed as the next layer's two B operands
+__device__ __forceinline__ void act_pack(const f32x4_t (&acc)[4], bf16x8_t (&h)[2], bf16x8_t (&l)[2]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 2; ++j) {
         float a[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) a[i] = silu(acc[j / 2][8 * (j % 2) + i]);
+        for (int i = 0; i < 8; ++i) a[i] = silu(acc[2 * j + (i >> 2)][i & 3]);
         split8(a, h[j], l[j]);
     }
 }
@@ -563,73 +565,73 @@ __device__ __forceinline__ void act_pack(const f32x16_t (&acc)[2], bf16x8_t (&h)
 // one K step: W (x_hi + x_lo), plus w_lo x_hi for split weights (the w_lo
 // x_lo term, about 2^-16 of the product, is left out)
 template <bool SPLIT, typename PA>
-__device__ __forceinline__ f32x16_t mfma_w(PA WA, int idx, bf16x8_t bh, bf16x8_t bl, f32x16_t c) {
+__device__ __forceinline__ f32x4_t mfma_w(PA WA, int idx, bf16x8_t bh, bf16x8_t bl, f32x4_t c) {
     const bf16x8_t a = WA[idx];
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl, c, 0, 0, 0);
-    if (SPLIT) c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(WA[kAVec8 + idx], bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, c, 0, 0, 0);
+    if (SPLIT) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(WA[kAVec8 + idx], bh, c, 0, 0, 0);
     return c;
 }
 
 // Network outputs (y0, y1) of the row held by this lane (valid in lanes
-// 0-31; lanes l and l + 32 carry the same row).  Wave-uniform call, all
+// 0-15; the four lanes l % 16 carry the same row).  Wave-uniform call, all
 // lanes active.  SPLIT: each weight carried as two bf16 terms (WA holds the
 // high terms, WA + kAVec8 the remainders).  WA, WB: the operand images in
 // global memory or (the fused gather) in LDS.
 template <bool SPLIT, typename PA, typename PB>
-__device__ __forceinline__ void mlp_eval32(PA WA, PB WB, const float (&x)[5], float& y0, float& y1) {
+__device__ __forceinline__ void mlp_eval16(PA WA, PB WB, const float (&x)[5], float& y0, float& y1) {
     const int lane = threadIdx.x & 63;
-    const bool upper = lane >= 32;
-    const int hb = lane >> 5;
-    bf16x8_t h[4], l[4];
+    const int g = lane >> 4;
+    bf16x8_t h[2], l[2];
     {
-        // layer 0: the 5 inputs are elements 0-4 of the lower lane half
-        const float xin[8] = {upper ? 0.0f : x[0], upper ? 0.0f : x[1], upper ? 0.0f : x[2],
-                              upper ? 0.0f : x[3], upper ? 0.0f : x[4], 0.0f, 0.0f, 0.0f};
+        // layer 0: the 5 inputs are elements 0-4 of lane group 0
+        const bool g0 = g == 0;
+        const float xin[8] = {g0 ? x[0] : 0.0f, g0 ? x[1] : 0.0f, g0 ? x[2] : 0.0f,
+                              g0 ? x[3] : 0.0f, g0 ? x[4] : 0.0f, 0.0f, 0.0f, 0.0f};
         bf16x8_t xh, xl;
         split8(xin, xh, xl);
-        f32x16_t acc[2];
+        f32x4_t acc[4];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma_w<SPLIT>(WA, mt * 64 + lane, xh, xl, WB[mt * 2 + hb]);
+        for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma_w<SPLIT>(WA, mt * 64 + lane, xh, xl, WB[mt * 4 + g]);
         act_pack(acc, h, l);
     }
 #pragma unroll 1
     for (int layer = 1; layer < 4; ++layer) {
         const int wa = kA1 + (layer - 1) * 8 * 64;
-        f32x16_t acc[2];
+        f32x4_t acc[4];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-            acc[mt] = WB[(layer * 2 + mt) * 2 + hb];
+        for (int mt = 0; mt < 4; ++mt) {
+            acc[mt] = WB[(layer * 4 + mt) * 4 + g];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[mt] = mfma_w<SPLIT>(WA, wa + (mt * 4 + j) * 64 + lane, h[j], l[j], acc[mt]);
+            for (int j = 0; j < 2; ++j) acc[mt] = mfma_w<SPLIT>(WA, wa + (mt * 2 + j) * 64 + lane, h[j], l[j], acc[mt]);
         }
         act_pack(acc, h, l);
     }
-    f32x16_t o = WB[8 * 2 + hb];
+    f32x4_t o = WB[16 * 4 + g];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) o = mfma_w<SPLIT>(WA, kA4 + j * 64 + lane, h[j], l[j], o);
+    for (int j = 0; j < 2; ++j) o = mfma_w<SPLIT>(WA, kA4 + j * 64 + lane, h[j], l[j], o);
     y0 = o[0];
     y1 = o[1];
 }
 
 #ifndef PSTM_GATHER_PART
 // Batched inference through the fused path's device code: one wavefront per
-// 32 rows of x (n, 5), outputs y (n, 2).
+// 16 rows of x (n, 5), outputs y (n, 2).
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x8_t* __restrict__ WA,
-                                                          const f32x16_t* __restrict__ WB,
+                                                          const f32x4_t* __restrict__ WB,
                                                           const float* __restrict__ x, int n, float* __restrict__ y) {
     const int lane = threadIdx.x & 63;
     const long wv = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long r = wv * 32 + (lane & 31);
+    const long r = wv * 16 + (lane & 15);
     float xi[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (r < n) {
 #pragma unroll
         for (int i = 0; i < 5; ++i) xi[i] = x[r * 5 + i];
     }
     float y0, y1;
-    mlp_eval32<SPLIT>(WA, WB, xi, y0, y1);
-    if (lane < 32 && r < n) {
+    mlp_eval16<SPLIT>(WA, WB, xi, y0, y1);
+    if (lane < 16 && r < n) {
         y[2 * r] = y0;
         y[2 * r + 1] = y1;
     }
@@ -1040,7 +1042,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     // workgroup's life (a weight operand per MFMA from global memory would
     // wait out the L2 latency once per product)
     using LdsA = __attribute__((address_space(3))) const bf16x8_t;
-    using LdsB = __attribute__((address_space(3))) const f32x16_t;
+    using LdsB = __attribute__((address_space(3))) const f32x4_t;
     const int mlp_nv = learned_tt(TT) ? kAVec8 * (P.mlp_split ? 2 : 1) : 0;
     if (learned_tt(TT)) {
         if (tid == 0) {
@@ -1488,7 +1490,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 // groups of three or more, pe = 2 for two or more): the rows
                 // (pe, trace, extended node) whose node some pillar of the
                 // group reads inside its time-dependent aperture are listed (in
-                // any order), the others marked missing (NaN); then 32 listed
+                // any order), the others marked missing (NaN); then 16 listed
                 // rows per wavefront step
                 const int nx = nn + 2;
                 const int R = 3 * P.batch * nx;
@@ -1521,8 +1523,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 const float4* ntab = reinterpret_cast<const float4*>(smem + P.node_off);
                 LdsA* wa = (LdsA*)(smem + P.mlp_off);
                 LdsB* wb = (LdsB*)(smem + P.mlp_off + mlp_nv * 16);
-                for (int c0 = wave * 32; c0 < nrows; c0 += kWaves * 32) {
-                    const int li = c0 + (lane & 31);
+                for (int c0 = wave * 16; c0 < nrows; c0 += kWaves * 16) {
+                    const int li = c0 + (lane & 15);
                     const bool valid = li < nrows;
                     const int r = valid ? (int)rlist[li] : 0;
                     float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -1553,10 +1555,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     }
                     float y0, y1;
                     if (P.mlp_split)
-                        mlp_eval32<true>(wa, wb, x, y0, y1);
+                        mlp_eval16<true>(wa, wb, x, y0, y1);
                     else
-                        mlp_eval32<false>(wa, wb, x, y0, y1);
-                    if (lane < 32 && valid) lat[r] = make_float2(y0 * P.m_inv_out, y1 * P.m_inv_out);
+                        mlp_eval16<false>(wa, wb, x, y0, y1);
+                    if (lane < 16 && valid) lat[r] = make_float2(y0 * P.m_inv_out, y1 * P.m_inv_out);
                 }
             }
 
@@ -3003,9 +3005,9 @@ int pstm_mlp_forward_mfma_terms(const float* params, const float* x, int n, floa
     hipLaunchKernelGGL(k_prep_mlp, dim3((np + 255) / 256), dim3(256), 0, st, params,
                        reinterpret_cast<unsigned short*>(ws), reinterpret_cast<float*>(ws + (size_t)kAVals * 4));
     HIP_TRY(hipGetLastError());
-    const long waves = ((long)n + 31) / 32;
+    const long waves = ((long)n + 15) / 16;
     const auto* wa = reinterpret_cast<const bf16x8_t*>(ws);
-    const auto* wb = reinterpret_cast<const f32x16_t*>(ws + (size_t)kAVals * 4);
+    const auto* wb = reinterpret_cast<const f32x4_t*>(ws + (size_t)kAVals * 4);
     if (weight_terms == 2)
         hipLaunchKernelGGL(k_mlp_forward_mfma<true>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, wa, wb, x,
                            n, y);
