@@ -93,9 +93,8 @@ constexpr int kNPil = kPW * kPG;       // 8 pillars per workgroup
 constexpr int kMaxBatch = 16;          // traces per LDS batch (upper bound)
 constexpr int kMaxHalfWidth = 64;      // largest anti-aliasing half width (aa_max_half_width)
 constexpr int kPF = 12;                // dwords of the next batch's rows each thread holds in flight
-constexpr int kStageBatch = 64 / kNPil;  // batches whose pair table one wave holds
-constexpr int kStageDw = kStageBatch * 12;  // staged pair inputs: 12 dwords per trace
-constexpr size_t kStageBytes = (size_t)(kStageDw + kStageBatch) * 4;  // + the block rows after
+constexpr int kStageDw = kMaxBatch * 12;  // staged pair inputs: 12 dwords per trace
+constexpr size_t kStageBytes = (size_t)(kStageDw + kMaxBatch) * 4;  // + the block rows after
 constexpr size_t kLdsBudget = 160 * 1024 - 2048;  // dynamic LDS; static LDS uses the rest
 
 // traveltimes read from the fused network's t0 lattice
@@ -1026,8 +1025,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     // pillars 0, p_mid and p_hi (the item's, written at the item start)
     __shared__ float4 s_lw[kNPil];
     // the next batch's pair inputs, loaded during the batch before by direct
-    // global -> LDS loads of wave 0 (batches whose pair table wave 0 holds
-    // alone): per trace 12 dwords, the geometry {sx, sy, rx, ry}, {mx, my, w, 0}
+    // global -> LDS loads of wave 0, which builds the pair table: per trace 12
+    // dwords, the geometry {sx, sy, rx, ry}, {mx, my, w, 0}
     // and the f64 midpoint (x, y); and the block rows of the batch after it
     // (in the dynamic LDS, after the pair table: arrays written by direct
     // loads alone must not be static ones, whose reads the compiler may fold
@@ -1090,11 +1089,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         float e2[kPW][MW], e1[kPW][MW], t0r[MW];
         float etar[kPW][MW];  // learned VTI: eta at (pillar, sample)
         float acc[kPW][MW];
-        // running totals across batches, lane-private in LDS: f64 for the
-        // exact traveltimes, f32 for the learned one (whose accuracy is that
-        // of the network)
+        // running totals across batches: f64 lane-private in LDS for the
+        // exact traveltimes, f32 in registers for the learned one (whose
+        // accuracy is that of the network; the LDS goes to larger batches)
         double* tot = reinterpret_cast<double*>(smem + P.tot_off);  // [kPW*MW][kThreads]
-        float* totf = reinterpret_cast<float*>(smem + P.tot_off);
+        float totf[kPW][MW];
 #pragma unroll
         for (int m = 0; m < MW; ++m) {
             const int k = kbase + 64 * (tw + kTW * m) + lane;
@@ -1105,7 +1104,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             for (int p = 0; p < kPW; ++p) {
                 const int pil = pil0 + pg * kPW + p;
                 acc[p][m] = 0.0f;
-                if (learned_tt(TT)) totf[(p * MW + m) * kThreads + tid] = 0.0f;
+                if (learned_tt(TT)) totf[p][m] = 0.0f;
                 else tot[(p * MW + m) * kThreads + tid] = 0.0;
                 e2[p][m] = 0.0f;
                 e1[p][m] = 1.0f;
@@ -1148,8 +1147,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         if (TT == PSTM_TT_CURVED || TT == PSTM_TT_CURVED_LEARNED) early = (1u << MW) - 1u;  // t may be < t0 on a curved ray
         early = __builtin_amdgcn_readfirstlane(early);
 
-        const int pt_p = tid / P.batch, pt_j = tid % P.batch;
-        const bool pt_thread = tid < kNPil * P.batch;
+        // the pair table, pair pt = p * batch + j: by thread pt, or for the
+        // learned traveltime (the larger batches) by wave 0 alone, lane pt % 64
+        const int n_pairs = kNPil * P.batch;
+        const unsigned m_batch = div_magic(P.batch);
+        const int pt_p0 = tid / P.batch, pt_j0 = tid % P.batch;
 
         // Learned traveltime on the t0 lattice.  The network's outputs g (the
         // legs' learned factors) are evaluated at the lattice nodes for three
@@ -1344,7 +1346,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // batch before.  The waves that build the pair table issue none: their
         // own global reads would wait behind the row loads.
         const bool early_dma = (AA || learned_tt(TT)) && !pipe;
-        const int w_pairs = (kNPil * P.batch + 63) / 64;  // waves holding pair-table threads
+        const int w_pairs = learned_tt(TT) ? 1 : (n_pairs + 63) / 64;  // waves holding the pair table
         // Pair inputs one batch ahead (w_pairs == 1): wave 0 loads batch jb's
         // geometry and midpoints (rows from s_rownext[rpar]) and batch jb +
         // batch's block rows into s_stage / s_stage_row after the pair-table
@@ -1406,7 +1408,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 const int jn = j0 + P.batch + tid;
                 s_rownext[par][tid] = jn < T.ntr ? (T.tidx ? (staged ? s_stage_row[tid] : T.tidx[jn]) : jn) : 0;
             }
-            if (pt_thread) {
+            auto build_pair = [&](int pt_p, int pt_j) {
                 const int pil = pil0 + pt_p;
                 float4 e = make_float4(0.0f, 0.0f, INFINITY, 0.0f);
                 float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1439,7 +1441,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     // outside the aperture (an end pillar's network inputs)
                     const bool in_ap64 = !(dm64 > P.max_ap64);
                     if (in_ap64 || learned_tt(TT)) {
-                        inap = in_ap64 ? 1 : 0;
+                        if (in_ap64) inap = 1;
                         const float dxs = pox - g0.x, dys = poy - g0.y;
                         const float dxr = pox - g0.z, dyr = poy - g0.w;
                         const float dxm = pox - g1.x, dym = poy - g1.y;
@@ -1466,6 +1468,16 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 }
                 pairs[(pt_p * P.batch + pt_j) * kPairVec] = e;
                 if (kPairVec == 2) pairs[(pt_p * P.batch + pt_j) * kPairVec + 1] = f;
+            };
+            if constexpr (learned_tt(TT)) {
+                if (wave == 0) {
+                    for (int pt = lane; pt < n_pairs; pt += 64) {
+                        const int pp = div_by(pt, m_batch, P.batch);
+                        build_pair(pp, pt - pp * P.batch);
+                    }
+                }
+            } else if (tid < n_pairs) {
+                build_pair(pt_p0, pt_j0);
             }
             // the written j0 tells this batch from the last one of its parity
             if (wave_any(inap) && lane == 0) s_any[par] = j0;
@@ -1855,7 +1867,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             for (int p = 0; p < kPW; ++p)
 #pragma unroll
                 for (int m = 0; m < MW; ++m) {
-                    if (learned_tt(TT)) totf[(p * MW + m) * kThreads + tid] += acc[p][m];
+                    if (learned_tt(TT)) totf[p][m] += acc[p][m];
                     else tot[(p * MW + m) * kThreads + tid] += (double)acc[p][m];
                     acc[p][m] = 0.0f;
                 }
@@ -1878,7 +1890,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                     // the total times cw (early blocks carry it per contribution)
                     const double cwb = (early & (1u << m)) ? 1.0 : (double)cw_at(p, m);
                     float* dst = T.image + (size_t)pil * P.nt + k;
-                    const double sum = learned_tt(TT) ? (double)totf[(p * MW + m) * kThreads + tid]
+                    const double sum = learned_tt(TT) ? (double)totf[p][m]
                                                       : tot[(p * MW + m) * kThreads + tid];
                     *dst = *dst + (float)(sum * cwb);
                 }
@@ -2206,7 +2218,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
         return align_up((size_t)nbuf * align_up((size_t)b * sh.rowlen * sh.elem, 16) +
                             (size_t)kNPil * b * sh.pair_vec * 16, 16) +
                kStageBytes + sh.tot_bytes +
-               (sh.learned ? align_up((size_t)3 * b * nx * 8, 16) + (size_t)kNPil * b * sh.lat_nodes * 32 +
+               (sh.learned ? align_up((size_t)3 * b * nx * 8, 16) + (size_t)kNPil * b * (sh.lat_nodes - 1) * 32 +
                                  align_up((size_t)kNPil * nx * 4, 16) + align_up((size_t)3 * b * nx * 2, 16) +
                                  (size_t)kNPil * nx * 16 + (size_t)sh.mlp_terms * kAVec8 * 16 +
                                  (size_t)kBVals * 4 + align_up((size_t)sh.lat_total * 4, 16)
@@ -2219,8 +2231,8 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     }
     // sample blocks per wave: the learned layout must fit one trace per batch
     for (;;) {
-        // running totals: f64, f32 for the learned traveltime
-        sh.tot_bytes = (size_t)kPW * sh.mw * kThreads * (sh.learned ? sizeof(float) : sizeof(double));
+        // running totals: f64 in LDS (the learned traveltime keeps f32 totals in registers)
+        sh.tot_bytes = sh.learned ? 0 : (size_t)kPW * sh.mw * kThreads * sizeof(double);
         sh.lat_nodes = sh.learned ? lattice_window_nodes(npos, nt, 256 * sh.mw) : 0;
         if (sh.mw == 1 || lds_for(1) <= kLdsBudget) break;
         --sh.mw;
@@ -2305,7 +2317,7 @@ void fill_common(KParams& kp, const PstmMigrateArgs& a, const Shape& sh, double 
     kp.lat_off = (int)(kp.tot_off + sh.tot_bytes);
     kp.lat_nodes = sh.lat_nodes;
     kp.coef_off = kp.lat_off + (int)align_up((size_t)3 * sh.batch * nx * 8, 16);
-    kp.lat_cimin_off = kp.coef_off + kNPil * sh.batch * sh.lat_nodes * 32;
+    kp.lat_cimin_off = kp.coef_off + kNPil * sh.batch * (sh.lat_nodes - 1) * 32;
     kp.lat_list_off = kp.lat_cimin_off + (int)align_up((size_t)kNPil * nx * 4, 16);
     kp.node_off = kp.lat_list_off + (int)align_up((size_t)3 * sh.batch * nx * 2, 16);
     kp.mlp_off = kp.node_off + kNPil * nx * 16;
