@@ -665,9 +665,11 @@ __device__ __forceinline__ float node_derivative(const float (&f)[4], const bool
 
 // Traveltime of one contribution from the pair terms e/f (LDS) and the
 // (pillar, sample) terms.  y1/y2 return 1/t_src, 1/t_rec for anti-aliasing.
+// t0z: the sample's t0 may be <= 0 (the VTI times are then t0).
 template <int TT, bool AA>
 __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e, const float4& f, float q, float cs,
-                                              float e2v, float e1v, float t0v, float etav, float& y1, float& y2) {
+                                              float e2v, float e1v, float t0v, float etav, float& y1, float& y2,
+                                              bool t0z = true) {
     float t;
     y1 = 0.f;
     y2 = 0.f;
@@ -704,7 +706,7 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         const float a1 = fmaf(e.x * cs, fmaxf(fmaf(etav, e2v, 1.0f), 0.0f), q);
         const float a2 = fmaf(e.y * cs, fmaxf(fmaf(etav, e1v, 1.0f), 0.0f), q);
         t = fsqrt(a1) + fsqrt(a2);
-        if (!(t0v > 0.f)) t = t0v;
+        if (t0z && !(t0v > 0.f)) t = t0v;
     } else {
         // VTI one-way legs (anisotropy_vti.py:181-220) with s = 1/v^2:
         // t^2 = tau^2 + h^2 s - 2 eta (h^2 s)^2 / (tau^2 + (1 + 2 eta) h^2 s)
@@ -712,7 +714,7 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         const float ts2 = q + hs - e2v * hs * hs * frcp(fmaf(e1v, hs, q));
         const float tr2 = q + hr - e2v * hr * hr * frcp(fmaf(e1v, hr, q));
         t = fsqrt(fmaxf(ts2, 0.f)) + fsqrt(fmaxf(tr2, 0.f));
-        if (!(t0v > 0.f)) t = t0v;
+        if (t0z && !(t0v > 0.f)) t = t0v;
     }
     return t;
 }
@@ -1738,8 +1740,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                                 e2[p][m] = fmaf(fmaf(fmaf(ca.w, fr, ca.z), fr, ca.y), fr, ca.x);
                                 e1[p][m] = fmaf(fmaf(fmaf(cb.w, fr, cb.z), fr, cb.y), fr, cb.x);
                             }
+                            // (t0 <= 0 only in a block whose first t0 is <= 2 ms: early)
                             const float t = contrib_time<TT, AA>(P, e, f, q[m], cs[p][m], e2[p][m], e1[p][m], t0r[m],
-                                                                 etar[p][m], y1, y2);
+                                                                 etar[p][m], y1, y2, EARLY && ((early >> m) & 1u));
                             const float u = fmaf(t, P.idt, uoff_v);
                             // u < umax - eps: inside; [umax - eps, umax + eps): decided in f64
                             const bool in_ap = xt[m] <= ((COUNT && !learned_tt(TT)) ? 1.0f - P.eps_x : 1.0f);
@@ -2249,8 +2252,8 @@ Shape launch_shape(const PstmMigrateArgs& a) {
 }
 
 bool lo_check(const PstmMigrateArgs& a, const Shape& sh) {
-    return a.t_start_in_ms > 0.0 || (sh.tt == PSTM_TT_CURVED && a.curved_k < 0.0) || sh.vti_like ||
-           sh.tt == PSTM_TT_CURVED_LEARNED;
+    // (the learned times are never below t0 >= 0, nor below 0: u >= 0 unless the input starts late)
+    return a.t_start_in_ms > 0.0 || (sh.tt == PSTM_TT_CURVED && a.curved_k < 0.0) || (sh.vti_like && !sh.learned);
 }
 
 // Kernel parameters common to every tile (everything but the tile table).
