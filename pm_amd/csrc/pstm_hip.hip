@@ -642,25 +642,28 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const bf16x8_t* __rest
 // i, i + 1 where both neighbours are there, else the one-sided three-point
 // difference over i, i -+ 1, i -+ 2, else the two-point one (second order:
 // exact for a quadratic).
+// (the quotients through one hardware reciprocal each: within ~2 ulp of
+// the divided forms, far inside the network's own accuracy)
 __device__ __forceinline__ float d3(float a, float b, float c, float xa, float xb, float xc) {
-    // derivative at xa of the parabola through (xa, a), (xb, b), (xc, c)
-    const float h1 = xb - xa, h2 = xc - xb;
-    return -(2.0f * h1 + h2) / (h1 * (h1 + h2)) * a + (h1 + h2) / (h1 * h2) * b - h1 / (h2 * (h1 + h2)) * c;
+    // derivative at xa of the parabola through (xa, a), (xb, b), (xc, c):
+    // [-(2 h1 + h2) h2 a + (h1 + h2)^2 b - h1^2 c] / (h1 h2 (h1 + h2))
+    const float h1 = xb - xa, h2 = xc - xb, s = h1 + h2;
+    return (-(2.0f * h1 + h2) * h2 * a + s * s * b - h1 * h1 * c) * frcp(h1 * h2 * s);
 }
 
 __device__ __forceinline__ float node_derivative(const float (&f)[4], const bool (&ok)[4], const float (&x)[4], int i) {
     if (i == 1) {
         if (ok[0]) {
             const float h1 = x[1] - x[0], h2 = x[2] - x[1];
-            return (h1 * h1 * (f[2] - f[1]) + h2 * h2 * (f[1] - f[0])) / (h1 * h2 * (h1 + h2));
+            return (h1 * h1 * (f[2] - f[1]) + h2 * h2 * (f[1] - f[0])) * frcp(h1 * h2 * (h1 + h2));
         }
-        return ok[3] ? d3(f[1], f[2], f[3], x[1], x[2], x[3]) : (f[2] - f[1]) / (x[2] - x[1]);
+        return ok[3] ? d3(f[1], f[2], f[3], x[1], x[2], x[3]) : (f[2] - f[1]) * frcp(x[2] - x[1]);
     }
     if (ok[3]) {
         const float h1 = x[2] - x[1], h2 = x[3] - x[2];
-        return (h1 * h1 * (f[3] - f[2]) + h2 * h2 * (f[2] - f[1])) / (h1 * h2 * (h1 + h2));
+        return (h1 * h1 * (f[3] - f[2]) + h2 * h2 * (f[2] - f[1])) * frcp(h1 * h2 * (h1 + h2));
     }
-    return ok[0] ? d3(f[2], f[1], f[0], x[2], x[1], x[0]) : (f[2] - f[1]) / (x[2] - x[1]);
+    return ok[0] ? d3(f[2], f[1], f[0], x[2], x[1], x[0]) : (f[2] - f[1]) * frcp(x[2] - x[1]);
 }
 
 // Traveltime of one contribution from the pair terms e/f (LDS) and the
