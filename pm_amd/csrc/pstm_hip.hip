@@ -1354,10 +1354,11 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         const int w_pairs = learned_tt(TT) ? 1 : (n_pairs + 63) / 64;  // waves holding the pair table
         // Pair inputs one batch ahead (w_pairs == 1): wave 0 loads batch jb's
         // geometry and midpoints (rows from s_rownext[rpar]) and batch jb +
-        // batch's block rows into s_stage / s_stage_row after the pair-table
-        // barrier of the batch before, so that the pair table (on which every
-        // wave waits) reads LDS only; the loads land during that batch.  Only
-        // wave 0 reads them, after its own vmcnt wait.
+        // batch's block rows into s_stage / s_stage_row during the batch
+        // before (after that batch's wait for its row loads, or, skipped,
+        // after its pair-table barrier), so that the pair table (on which
+        // every wave waits) reads LDS only; the loads land during the trace
+        // loop.  Only wave 0 reads them, after its own vmcnt wait.
         const bool staged_pairs = w_pairs == 1;
         auto stage = [&](int jb, int rpar) {
             const int nbb = min(P.batch, T.ntr - jb);
@@ -1488,8 +1489,10 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             if (wave_any(inap) && lane == 0) s_any[par] = j0;
             lds_barrier();
             const bool any_inap = __builtin_amdgcn_readfirstlane(s_any[par]) == j0;
-            if (staged_pairs && wave == 0) stage(j0 + P.batch, par);
+            // the next batch's pair inputs: issued after this batch's last wait
+            // for row loads (a wait would also wait for them)
             if (!any_inap) {
+                if (staged_pairs && wave == 0) stage(j0 + P.batch, par);
                 if (pipe) {  // this batch's rows are not needed
                     if (AA) {
                         dma(j0 + P.batch, cur ^ 1, par, 0);
@@ -1620,6 +1623,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             // pipelined f32 rows are visible since the pair-table barrier: only
             // DMA'd or loaded rows and the lattice need this one
             if (AA || !pipe || learned_tt(TT)) lds_barrier();
+            if (staged_pairs && wave == 0) stage(j0 + P.batch, par);
             if (pipe) {  // in flight during the batch
                 if (AA) dma(j0 + P.batch, cur ^ 1, par, 0);
                 else prefetch(j0 + P.batch, par);
