@@ -87,25 +87,43 @@ def bf16_pair(a):
     return hi + lo
 
 
-def forward_bf16(p, x):
+# the fused inference carries every pre-activation and activation times
+# c = -log2(e) (so that its SiLU needs no multiply before the exponential):
+# inputs c x, biases c b (f32), c SiLU(z) = z' / (1 + 2^z') with z' = c z,
+# outputs divided by c; the trainer's bf16 mode evaluates unscaled (scale 1)
+SILU_C = float(np.float32(-1.44269504088896341))
+SILU_INV_C = float(np.float32(-0.693147180559945309))
+
+
+def _silu_scaled(zc, scale):
+    return silu(zc) if scale == 1.0 else zc / (1.0 + np.exp2(zc))
+
+
+def _bias(b, scale):
+    return b if scale == 1.0 else (np.float32(scale) * np.asarray(b, np.float32)).astype(np.float64)
+
+
+def forward_bf16(p, x, scale=1.0):
     """Forward pass with the numerics of the fused inference: weights rounded to
     bfloat16, the input and each hidden activation carried as a bfloat16 pair,
     products and sums exact (f64 here; f32 accumulation on the device),
-    biases in f32."""
+    biases in f32.  scale: 1 for the trainer's bf16 mode, SILU_C for the
+    fused inference (mlp_eval16's scaled evaluation)."""
     ls = unpack(np.asarray(p, np.float32).astype(np.float64))
-    a = bf16_pair(x)
+    a = bf16_pair(scale * np.asarray(x, np.float32) if scale != 1.0 else x)
     for i, (W, b) in enumerate(ls):
-        z = a @ bf16_round(W).T + b
-        a = bf16_pair(silu(z)) if i < len(ls) - 1 else z
-    return a
+        z = a @ bf16_round(W).T + _bias(b, scale)
+        a = bf16_pair(_silu_scaled(z, scale)) if i < len(ls) - 1 else z
+    return a if scale == 1.0 else a * SILU_INV_C
 
 
-def forward_bf16_split(p, x):
+def forward_bf16_split(p, x, scale=SILU_C):
     """Forward pass with the split-weight numerics of the fused inference
     (mlp_weight_terms = 2): each weight as a high bfloat16 term plus the
     bfloat16 of its remainder, the input and each hidden activation carried
     as a bfloat16 pair, and the product of the two remainders left out (the
-    MFMA sequence w_hi x_hi + w_hi x_lo + w_lo x_hi); biases in f32."""
+    MFMA sequence w_hi x_hi + w_hi x_lo + w_lo x_hi); biases in f32; values
+    scaled as in forward_bf16."""
     ls = unpack(np.asarray(p, np.float32).astype(np.float64))
 
     def split(a):
@@ -113,11 +131,11 @@ def forward_bf16_split(p, x):
         hi = bf16_round(a32)
         return hi, bf16_round((a32.astype(np.float64) - hi).astype(np.float32))
 
-    xh, xl = split(x)
+    xh, xl = split(scale * np.asarray(x, np.float32) if scale != 1.0 else x)
     for i, (W, b) in enumerate(ls):
         wh, wl = split(W)
-        z = (xh + xl) @ wh.T + xh @ wl.T + b
+        z = (xh + xl) @ wh.T + xh @ wl.T + _bias(b, scale)
         if i < len(ls) - 1:
-            xh, xl = split(silu(z))
+            xh, xl = split(_silu_scaled(z, scale))
         else:
-            return z
+            return z if scale == 1.0 else z * SILU_INV_C

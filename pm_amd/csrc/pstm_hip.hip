@@ -509,6 +509,10 @@ __host__ __device__ inline unsigned short bf16_bits(float f) {
     return (unsigned short)(u >> 16);
 }
 
+// the evaluation's scale c (mlp_eval16)
+constexpr float kSiluC = -1.44269504088896341f;     // -log2(e)
+constexpr float kSiluInvC = -0.693147180559945309f;  // 1 / c = -ln(2)
+
 #ifndef PSTM_GATHER_PART
 __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restrict__ A, float* __restrict__ B) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -534,12 +538,21 @@ __global__ void k_prep_mlp(const float* __restrict__ P, unsigned short* __restri
         const int f = e - kAVals, v = f / 16, g = (f / 4) % 4, r = f % 4;
         const int l = v < 16 ? v / 4 : 4, mt = v < 16 ? v % 4 : 0;
         const int o = mt * 16 + 4 * g + r;
-        B[f] = o < mlp_out(l) ? P[mlp_w_off(l) + mlp_in(l) * mlp_out(l) + o] : 0.0f;
+        // biases times c = -log2(e) (the scaled evaluation of mlp_eval16)
+        B[f] = o < mlp_out(l) ? kSiluC * P[mlp_w_off(l) + mlp_in(l) * mlp_out(l) + o] : 0.0f;
     }
 }
 #endif  // PSTM_GATHER_PART
 
-__device__ __forceinline__ float silu(float z) { return z * frcp(1.0f + __expf(-z)); }
+// The evaluation carries every pre-activation and activation scaled by
+// c = -log2(e): with z' = c z, SiLU(z) = z / (1 + e^-z) gives
+// c SiLU(z) = z' / (1 + 2^z') (no multiply before the exponential).  The
+// inputs enter as c x and every bias as c b (k_prep_mlp), so that each
+// layer's product with its unchanged weights is c times the true one; the
+// outputs are divided by c at the end.  bf16 hi + lo splitting is relative,
+// so the scale costs no precision, and no weight is re-rounded.  (kSiluC,
+// kSiluInvC above.)
+__device__ __forceinline__ float silu_c(float zc) { return zc * frcp(1.0f + __builtin_amdgcn_exp2f(zc)); }
 
 // 8 values as a bf16 pair: hi = bf16(a), lo = bf16(a - hi)
 __device__ __forceinline__ void split8(const float (&a)[8], bf16x8_t& hi, bf16x8_t& lo) {
@@ -556,7 +569,7 @@ __device__ __forceinline__ void act_pack(const f32x4_t (&acc)[4], bf16x8_t (&h)[
     for (int j = 0; j < 2; ++j) {
         float a[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) a[i] = silu(acc[2 * j + (i >> 2)][i & 3]);
+        for (int i = 0; i < 8; ++i) a[i] = silu_c(acc[2 * j + (i >> 2)][i & 3]);
         split8(a, h[j], l[j]);
     }
 }
@@ -583,10 +596,10 @@ __device__ __forceinline__ void mlp_eval16(PA WA, PB WB, const float (&x)[5], fl
     const int g = lane >> 4;
     bf16x8_t h[2], l[2];
     {
-        // layer 0: the 5 inputs are elements 0-4 of lane group 0
+        // layer 0: the 5 inputs (times c) are elements 0-4 of lane group 0
         const bool g0 = g == 0;
-        const float xin[8] = {g0 ? x[0] : 0.0f, g0 ? x[1] : 0.0f, g0 ? x[2] : 0.0f,
-                              g0 ? x[3] : 0.0f, g0 ? x[4] : 0.0f, 0.0f, 0.0f, 0.0f};
+        const float xin[8] = {g0 ? kSiluC * x[0] : 0.0f, g0 ? kSiluC * x[1] : 0.0f, g0 ? kSiluC * x[2] : 0.0f,
+                              g0 ? kSiluC * x[3] : 0.0f, g0 ? kSiluC * x[4] : 0.0f, 0.0f, 0.0f, 0.0f};
         bf16x8_t xh, xl;
         split8(xin, xh, xl);
         f32x4_t acc[4];
@@ -609,8 +622,8 @@ __device__ __forceinline__ void mlp_eval16(PA WA, PB WB, const float (&x)[5], fl
     f32x4_t o = WB[16 * 4 + g];
 #pragma unroll
     for (int j = 0; j < 2; ++j) o = mfma_w<SPLIT>(WA, kA4 + j * 64 + lane, h[j], l[j], o);
-    y0 = o[0];
-    y1 = o[1];
+    y0 = o[0] * kSiluInvC;
+    y1 = o[1] * kSiluInvC;
 }
 
 #ifndef PSTM_GATHER_PART

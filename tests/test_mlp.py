@@ -56,6 +56,28 @@ def test_bf16_forward_close_to_f64():
     np.testing.assert_allclose(omlp.forward_bf16(pb, xb), omlp.forward(pb, xb), rtol=2e-4, atol=1e-5)
 
 
+def test_scaled_bf16_forward_is_the_same_network():
+    """The fused inference's scaled evaluation (every value times
+    c = -log2(e), SiLU as z' / (1 + 2^z'), outputs divided by c) is the same
+    network: with bf16 weights it equals the f64 forward as closely as the
+    unscaled bf16 numerics do, and it differs from the unscaled numerics only
+    by the rounding of the activation pairs (a dropped 1 / c, a sign or the
+    bias scale would be off by O(1))."""
+    rng = np.random.default_rng(15)
+    p = _rand_params(15)
+    x = rng.uniform(-1, 1, (256, 5))
+    pb = omlp.bf16_round(p.astype(np.float32))
+    y64 = omlp.forward(pb, x)
+    ys = omlp.forward_bf16(pb, x, omlp.SILU_C)
+    scale = np.max(np.abs(y64))
+    assert np.max(np.abs(ys - y64)) < 2e-4 * scale
+    assert np.max(np.abs(ys - omlp.forward_bf16(pb, x))) < 1e-4 * scale
+    # split weights, scaled and unscaled
+    ys2 = omlp.forward_bf16_split(p, x)
+    assert np.max(np.abs(ys2 - omlp.forward_bf16_split(p, x, 1.0))) < 1e-4 * np.max(np.abs(ys2))
+    assert np.max(np.abs(ys2 - omlp.forward(p, x))) < 2e-4 * np.max(np.abs(ys2))
+
+
 def test_numpy_gradient_matches_finite_differences():
     rng = np.random.default_rng(3)
     p = _rand_params(3)
@@ -144,7 +166,7 @@ class TestHipSurrogate:
     def test_bf16_mode_and_mfma_match_oracle(self):
         """pstm_mlp_forward in bf16 mode and the fused path's MFMA inference
         (pstm_mlp_forward_mfma) against the bf16 NumPy forward; n = 1000 is
-        not a multiple of the 32-row MFMA tile."""
+        not a multiple of the 16-row MFMA tile."""
         import torch
 
         from pm_amd.surrogate import _stream
@@ -153,7 +175,8 @@ class TestHipSurrogate:
         rng = np.random.default_rng(14)
         p = _rand_params(14).astype(np.float32)
         x = rng.uniform(-1, 1, (1000, 5)).astype(np.float32)
-        ref = omlp.forward_bf16(p, x)
+        ref = omlp.forward_bf16(p, x)  # the trainer's bf16 mode
+        ref_c = omlp.forward_bf16(p, x, omlp.SILU_C)  # the fused inference (scaled)
         pd, xd = torch.as_tensor(p, device="cuda"), torch.as_tensor(x, device="cuda")
         scale = np.max(np.abs(ref))
         ref_split = omlp.forward_bf16_split(p, x)
@@ -176,7 +199,7 @@ class TestHipSurrogate:
             else:
                 assert lib.pstm_mlp_forward_mfma(pd.data_ptr(), xd.data_ptr(), 1000, y.data_ptr(), _stream()) == 0
             got = y.cpu().numpy()
-            err = np.max(np.abs(got - ref)) / scale
+            err = np.max(np.abs(got - (ref if mode == "bf16" else ref_c))) / scale
             print(mode, "max abs error / max |y|", err)
             assert err < 2e-5, mode
 
