@@ -717,10 +717,11 @@ __device__ __forceinline__ float contrib_time(const KParams& P, const float4& e,
         t = cs * fmaf(frsq(e.x + q), fmaxf(1.0f + e2v, 0.0f), frsq(e.y + q) * fmaxf(1.0f + e1v, 0.0f));
     } else if (TT == PSTM_TT_VTI_LEARNED) {
         // learned legs: t^2 = tau^2 + h^2 s (1 + eta g), g = c / eta from the
-        // lattice (e2v, e1v here) and eta at the sample (etav); 1 + eta g >= 0
-        // keeps t >= t0 as the exact VTI time is
-        const float a1 = fmaf(e.x * cs, fmaxf(fmaf(etav, e2v, 1.0f), 0.0f), q);
-        const float a2 = fmaf(e.y * cs, fmaxf(fmaf(etav, e1v, 1.0f), 0.0f), q);
+        // lattice (e2v, e1v here), s = cs and etav = s eta at the sample:
+        // h^2 (s + s eta g); s (1 + eta g) >= 0 keeps t >= t0 as the exact VTI
+        // time is
+        const float a1 = fmaf(e.x, fmaxf(fmaf(etav, e2v, cs), 0.0f), q);
+        const float a2 = fmaf(e.y, fmaxf(fmaf(etav, e1v, cs), 0.0f), q);
         t = fsqrt(a1) + fsqrt(a2);
         if (t0z && !(t0v > 0.f)) t = t0v;
     } else {
@@ -1105,7 +1106,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // register-resident per-(pillar, sample) terms
         float q[MW], cs[kPW][MW], ci[kPW][MW];
         float e2[kPW][MW], e1[kPW][MW], t0r[MW];
-        float etar[kPW][MW];  // learned VTI: eta at (pillar, sample)
+        float etar[kPW][MW];  // learned VTI: slowness^2 x eta at (pillar, sample)
         float acc[kPW][MW];
         // running totals across batches: f64 lane-private in LDS for the
         // exact traveltimes, f32 in registers for the learned one (whose
@@ -1136,7 +1137,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                         e2[p][m] = ee.x;
                         e1[p][m] = ee.y;
                     } else if (TT == PSTM_TT_VTI_LEARNED) {
-                        etar[p][m] = 0.5f * T.ek[(size_t)pil * T.ek_pstride + k].x;
+                        etar[p][m] = c.x * (0.5f * T.ek[(size_t)pil * T.ek_pstride + k].x);  // s eta
                     }
                 } else {
                     cs[p][m] = 0.0f;
