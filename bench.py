@@ -145,6 +145,18 @@ def roofline_entry(f: int, e: int, launch_ms: float, kernel: str) -> dict:
             "frac_of_unpacked_fp32": round(achieved / roofline.MI355X_FP32_VALU_UNPACKED_TFLOPS, 4)}
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(ex, tile_index: int, budget_s: float, traveltime: int) -> dict:
     """The FP64 oracle (-O3 -ffast-math, OpenMP on every host core) on whole
     inlines of one tile: all of that tile's traces, all output samples, the
@@ -192,7 +204,7 @@ def cpu_baseline(ex, tile_index: int, budget_s: float, traveltime: int) -> dict:
     dt, c = run(ny, n_x)
     log(f"cpu baseline: {n_x} x {ny} pillars x {len(rows)} traces in {dt:.1f} s on {threads} threads")
     return {"value": round(c / dt / 1e9, 4), "unit": "Gcontrib/s", "cores": threads, "kind": "port",
-            "sample": (f"{n_x} x {ny} pillars (whole inlines of subset tile {tile_index}) x all {len(rows)} "
+            "cpu": cpu_model(), "sample": (f"{n_x} x {ny} pillars (whole inlines of subset tile {tile_index}) x all {len(rows)} "
                        f"traces of that tile x {len(job.t_axis_ms)} samples, same settings and exact "
                        f"traveltime; C_sample={c}, {dt:.2f} s; oracle C++ FP64 -O3 -ffast-math OpenMP")}
 
