@@ -206,13 +206,22 @@ k_fold_normalize1(const float* image, const int32_t* __restrict__ fold, long lon
 // last xp the last value, on a node its value, else
 // slope * (x - xp[j]) + fp[j] with slope = (fp[j+1] - fp[j]) / (xp[j+1] - xp[j])
 // (and the form from xp[j+1] if that is NaN) — rounded to f32.  The bracket
-// is searched once per output sample and reused for kRsPil pillars.
+// is searched once per output sample and reused for kRsPil pillars; the
+// workgroup first copies xp to LDS (up to kRsLds nodes), so that the
+// bisection's chain of dependent reads stays out of global memory.
 constexpr int kRsPil = 16;
+constexpr int kRsLds = 4096;
 
 __global__ void __launch_bounds__(kBlock)
-k_resample_uniform(const float* __restrict__ tv, const double* __restrict__ xp, int n_tv,
+k_resample_uniform(const float* __restrict__ tv, const double* __restrict__ xp_g, int n_tv,
                    const double* __restrict__ x_out, int n_out, long long n_pillars, float* __restrict__ out) {
 #pragma clang fp contract(off)
+    extern __shared__ double xs[];  // n_tv doubles when in_lds (the launch sizes it)
+    const bool in_lds = n_tv <= kRsLds;
+    if (in_lds)
+        for (int i = threadIdx.x; i < n_tv; i += kBlock) xs[i] = xp_g[i];
+    __syncthreads();
+    const double* xp = in_lds ? xs : xp_g;
     const int k = blockIdx.y * kBlock + threadIdx.x;
     if (k >= n_out) return;
     const long long p0 = static_cast<long long>(blockIdx.x) * kRsPil;
@@ -366,7 +375,8 @@ int pstm_hip_resample_uniform(const float* tv_image, const double* tv_times_ms, 
     const int kblocks = (n_out + kBlock - 1) / kBlock;
     if (groups > 0x7FFFFFFFLL || kblocks > 65535) return fail(1, "pstm_hip_resample_uniform: volume too large");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_resample_uniform, dim3(static_cast<unsigned>(groups), kblocks), dim3(kBlock), 0, st,
+    const size_t lds = n_tv <= kRsLds ? static_cast<size_t>(n_tv) * sizeof(double) : 0;
+    hipLaunchKernelGGL(k_resample_uniform, dim3(static_cast<unsigned>(groups), kblocks), dim3(kBlock), lds, st,
                        tv_image, tv_times_ms, n_tv, out_times_ms, n_out, n_pillars, out);
     OUT_TRY(hipGetLastError());
     return 0;

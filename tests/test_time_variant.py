@@ -117,12 +117,14 @@ def kernel():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n_tv", [1, 2, 37, 308])
+@pytest.mark.parametrize("n_tv", [1, 2, 37, 308, 5000])
 def test_resample_kernel_bit_exact_with_np_interp(kernel, n_tv):
+    """(5000 nodes: more than the kernel copies to LDS, so it searches the
+    axis in global memory)"""
     import torch
 
     rng = np.random.default_rng(n_tv)
-    xp = np.sort(rng.choice(np.arange(0, 2000, 2.0), n_tv, replace=False)) + 4.0
+    xp = np.sort(rng.choice(np.arange(0, max(2000, 4 * n_tv), 2.0), n_tv, replace=False)) + 4.0
     img = rng.normal(size=(7, 5, n_tv)).astype(np.float32)
     x = np.concatenate([[-3.0, xp[0]], np.arange(1000) * 2.0, [xp[-1], xp[-1] + 1.0]])
     got = kernel.resample_uniform(torch.as_tensor(img, device="cuda"), xp, x).cpu().numpy()
