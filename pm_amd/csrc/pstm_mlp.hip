@@ -71,8 +71,10 @@ __device__ __forceinline__ float bf16pair(float a) {
 // Precision of the forward pass.  Q = false: f32.  Q = true: the numerics of
 // the migration kernel's fused inference, i.e. weights rounded to bf16, the
 // network input and every hidden activation carried as a bf16 pair (hi + lo,
-// two MFMA products per layer), f32 accumulation and biases.  Training with
-// Q = true is quantisation-aware (straight-through gradients).
+// two MFMA products per layer), f32 accumulation and biases (the inference
+// carries its values times -log2(e), so its pairs round a scaled value: the
+// same relative precision, other last bits).  Training with Q = true is
+// quantisation-aware (straight-through gradients).
 bool g_q = false;
 
 // Z[s, o] = b[o] + sum_i X[s, i] W[o, i];  A = silu(Z) (or Z for the output layer)
