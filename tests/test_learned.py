@@ -242,3 +242,45 @@ def test_failed_tile_falls_back_to_exact(kernel, vti_small):
     exact = _image(ex)
     fallback = _image(ex, surrogate=sur, node_spacing=GEOMETRIC)
     assert torch.equal(exact, fallback)
+
+
+def test_learned_launch_shapes_agree(kernel, vti_small):
+    """The learned gather under different launch shapes (traces per batch:
+    the pair table built by wave 0 in one or two passes, the next batch's
+    pair inputs staged ahead for up to 16 traces; sample blocks per wave)
+    gives the same image: within one MW a shape changes only how the f32
+    per-batch sums are grouped (every network row, coefficient and time is
+    the same), so the images agree to f32 summation order."""
+    import torch
+
+    job, ex, sur = vti_small
+    lib = _abi.load_library()
+    saved = [t.learned_ok for t in ex.tiles]
+    for t in ex.tiles:
+        t.learned_ok = True
+    try:
+        for mw, caps in ((3, (16, 9, 4, 1)), (1, (16, 12, 3))):
+            ref = None
+            batches = set()
+            for cap in caps:
+                assert lib.pstm_hip_set_tuning(mw, cap) == 0
+                plan = ex.plan(traveltime=_abi.TT_VTI_LEARNED, surrogate=sur, node_spacing=GEOMETRIC)
+                info = plan.info()
+                ex.reset()
+                plan.run(use_graph=False)
+                img, _ = ex.assemble()
+                plan.destroy()
+                img = img.double()
+                batches.add(info["batch"])
+                if ref is None:
+                    ref = img
+                    continue
+                rel = float(torch.linalg.norm(img - ref) / torch.linalg.norm(ref))
+                print(f"mw<={mw} batch<={cap} -> {info}: rel-L2 against the first shape {rel:.2e}")
+                assert rel <= 1e-5, (mw, cap)
+            assert len(batches) >= 2, batches
+    finally:
+        lib.pstm_hip_set_tuning(0, 0)
+        for t, ok in zip(ex.tiles, saved):
+            t.learned_ok = ok
+        ex.reset()
