@@ -514,6 +514,60 @@ def streamed_section(kernel, args) -> dict:
         shutil.rmtree(d, ignore_errors=True)
 
 
+def dropin_section(kernel, ex, lib) -> dict:
+    """The reference's call path on the busiest tile: HipKernel.migrate_tile
+    with host NumPy arrays (TraceBlock, OutputTile, 3-D VelocitySlice), which
+    uploads the tile's traces, migrates and downloads the tile, timed by the
+    host clock; beside it the upload alone and the device-resident plan over
+    the same tile (HIP events)."""
+    import numpy as np
+    import torch
+
+    from pm_amd.kernels.base import OutputTile, VelocitySlice, create_trace_block
+    from pm_amd.kernels.hip_backend import DeviceTraces
+
+    i = max(range(len(ex.tiles)), key=lambda k: int(ex.tiles[k].rows.shape[0]))
+    t = ex.tiles[i]
+    h = ex.host_traces(t.rows.cpu().numpy().astype(np.int64))
+    job = ex.job
+    tb = create_trace_block(h["amplitudes"], h["source_x"], h["source_y"], h["receiver_x"], h["receiver_y"],
+                            sample_rate_ms=job.dt_ms)
+    xa, ya = t.x_axis.cpu().numpy(), t.y_axis.cpu().numpy()
+    nt = len(job.t_axis_ms)
+    vel = VelocitySlice(vrms=t.vrms.cpu().numpy().astype(np.float64), is_1d=False)
+
+    def tile():
+        return OutputTile(image=np.zeros((len(xa), len(ya), nt)), fold=np.zeros((len(xa), len(ya)), np.int32),
+                          x_axis=xa, y_axis=ya, t_axis_ms=np.asarray(job.t_axis_ms, float))
+
+    kernel.migrate_tile(tb, tile(), vel, ex.config)   # warm-up
+    out = tile()
+    t0 = time.perf_counter()
+    kernel.migrate_tile(tb, out, vel, ex.config)
+    call_s = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dt = DeviceTraces.from_block(tb, ex.device)
+    torch.cuda.synchronize()
+    up_s = time.perf_counter() - t0
+    del dt
+    plan = ex.plan(tiles=[i])
+    ex.reset()
+    plan.run(use_graph=USE_GRAPH)
+    torch.cuda.synchronize()
+    plan_ms = plan_launch_ms(lib, [plan])
+    plan.destroy()
+    ex.reset()
+    c = int(out.fold.sum()) * nt
+    gb = tb.amplitudes.nbytes / 1e9
+    log(f"drop-in: tile {i} ({tb.n_traces} traces, {gb:.2f} GB) {call_s:.3f} s per call, upload {up_s:.3f} s, "
+        f"plan {plan_ms:.0f} ms")
+    return {"tile": i, "traces": int(tb.n_traces), "trace_gb": round(gb, 3), "call_s": round(call_s, 4),
+            "upload_s": round(up_s, 4), "upload_gbps": round(gb / up_s, 1), "device_plan_ms": round(plan_ms, 2),
+            "contributions": c, "gcontrib_s_pcie_inclusive": round(c / call_s / 1e9, 2),
+            "gcontrib_s_device": round(c / (plan_ms / 1e3) / 1e9, 2)}
+
+
 def output_section(kernel, ex) -> dict:
     """The output step on the migrated job: per-bin header sums over every
     tile's trace query (HIP atomics), the gather-mode fold normalisation of
@@ -706,6 +760,8 @@ def reflector_section(kernel, args, sections) -> dict:
         del img_u
     if "interpolation" in sections:
         out["interpolation"] = interpolation_section(ex)
+    if "dropin" in sections:
+        out["dropin"] = dropin_section(kernel, ex, lib)
     lib.pstm_hip_set_tuning(0, 0)
     lib.pstm_hip_set_pipeline(-1)
     del ex
@@ -713,7 +769,7 @@ def reflector_section(kernel, args, sections) -> dict:
     return out
 
 
-SECTIONS = ("reflectors", "output", "time_variant", "interpolation", "stream")
+SECTIONS = ("reflectors", "output", "time_variant", "interpolation", "dropin", "stream")
 
 
 def main() -> None:
@@ -780,7 +836,7 @@ def main() -> None:
         calibration_kernel()
     if args.sections_only:
         out = {"sections_only": True}
-        if sections & {"reflectors", "output", "time_variant", "interpolation"}:
+        if sections & {"reflectors", "output", "time_variant", "interpolation", "dropin"}:
             out["reflectors"] = reflector_section(kernel, args, sections)
         if "stream" in sections:
             out["streamed"] = streamed_section(kernel, args)
@@ -815,7 +871,7 @@ def main() -> None:
     if cpu:
         out["gpu_over_cpu"] = {"learned": round(head["value"] / cpu["value"], 1),
                                "exact": round(head["learned_vs_exact"]["exact_gcontrib_s"] / cpu["value"], 1)}
-    if sections & {"reflectors", "output", "time_variant", "interpolation"}:
+    if sections & {"reflectors", "output", "time_variant", "interpolation", "dropin"}:
         out["reflectors"] = reflector_section(kernel, args, sections)
     if "stream" in sections:
         out["streamed"] = streamed_section(kernel, args)
