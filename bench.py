@@ -33,7 +33,8 @@ launch stream), the plain straight-ray gather on the same subset and on the
 4096-trace point-diffractor job (both against the FP32 VALU peak), and the
 CPU baseline (the FP64 oracle, -O3 -ffast-math, OpenMP, on a pillar sample of
 the subset).  Side measurements on the reflector job (output step,
-time-variant sampling, interpolators, streaming) are opt-in: ``--sections``.
+time-variant sampling, interpolators, the drop-in migrate_tile call with host
+arrays, streaming) are opt-in: ``--sections``.
 
 Prints one JSON line.
 """
