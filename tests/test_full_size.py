@@ -44,7 +44,7 @@ from pm_amd.kernels import _abi
 pytestmark = pytest.mark.gpu
 
 # Image bars at full trace length.  The gather computes sample positions
-# u = t / dt in f32 (the north star's FP32 VALU traveltimes): their error is
+# u = t / dt in f32 (traveltimes on the FP32 VALU): their error is
 # about 1e-7 u samples, and for the 30 Hz events of these jobs a position
 # error e costs about 2 pi 30 Hz dt e of an amplitude.  At u ~ 1000 (the
 # scaled and 1001-sample tests) that is below 1e-5 of the image; at 1501 and
