@@ -12,7 +12,7 @@ eta volumes, the full image and fold, every tile's trace list), but a step is
 one migration pass over a FIXED REGION of its output: the central
 ``--region`` x ``--region`` pillars (32 x 32 by default), in the tiles of the
 edge the autotuner picks: a pass over all 512 x 512 pillars takes minutes,
-and the driver's 20 + 5 steps must fit in a few minutes.  C counts exactly
+and a run of 20 + 5 steps must fit in a few minutes.  C counts exactly
 the region's contributions (sum of its 3-D fold), so the rate is that of the
 region.  The autotuner times tile edges (output bins per tile) x launch
 shapes (samples per wave, traces per batch, pipelining) with the exact
