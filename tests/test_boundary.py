@@ -235,3 +235,25 @@ def test_plan_rejects_tiles_that_share_image_or_fold_memory():
 
     if torch.cuda.device_count() == 0:
         assert rc != 0
+
+
+def test_multiply_high_division_rule():
+    """The gather divides lattice-row, node and segment indices by
+    batch-dependent values with one multiply-high, m = floor((2^32 - 1) / d) + 1
+    and a / d = (a m) >> 32 (div_magic / div_by in pstm_hip.hip), claimed exact
+    for 0 <= a < 2^16 and 2 <= d < 2^16; the indices stay below 2^16 because
+    the tables they index fit the 160 KB LDS.  Checked for every a and a range
+    of divisors, the largest included."""
+    a = np.arange(1 << 16, dtype=np.uint64)
+    divisors = list(range(2, 1025)) + [1031, 4093, 4096, 12345, 40000, 65521, 65535]
+    for d in divisors:
+        m = np.uint64((2**32 - 1) // d + 1)
+        assert m < 2**32
+        q = (a * m) >> np.uint64(32)
+        np.testing.assert_array_equal(q, a // np.uint64(d), err_msg=f"d = {d}")
+    # the rule fails past its range (a = 2^16 needs more precision for some d):
+    # the kernel relies on the bound, not on luck
+    a_big = np.arange(1 << 16, 1 << 22, dtype=np.uint64)
+    d = 65535
+    m = np.uint64((2**32 - 1) // d + 1)
+    assert np.any(((a_big * m) >> np.uint64(32)) != a_big // np.uint64(d))
