@@ -272,7 +272,7 @@ def vti_headline(kernel, args) -> dict:
 
         edges = tuple(e for e in (16, 32, 64) if e <= args.region)
         tune = autotune_tiles(kernel, job, edges=edges, region=args.region,
-                              candidates=[(3, 16, 1), (2, 16, 1), (4, 8, 0)], max_edge=args.region)
+                              candidates=[(3, 16, 0), (4, 16, 0), (3, 16, 1)], max_edge=args.region)
         log(f"autotune: edge {tune.edge}, mw {tune.mw}, batch {tune.batch}, pipe {tune.pipe}")
     edge = tune.edge if tune else args.region
     job.tile = edge
@@ -640,7 +640,7 @@ def time_variant_section(kernel, job, uniform_image, uniform_ms: float, uniform_
     table = tv.FrequencyTimeTable.from_pairs(tv.CONFIG_DEFAULT_TABLE)
     w = tv.compute_time_windows(float(t[0]), float(t[-1]), float(t[1] - t[0]), table)
     ex = TileExecutor(kernel, job, time_windows=w)
-    tune = autotune(kernel, ex, candidates=[(2, 8, 0), (2, 16, 1), (1, 8, 0), (1, 16, 1)])
+    tune = autotune(kernel, ex, candidates=[(2, 16, 0), (2, 8, 0), (1, 16, 0), (2, 16, 1)])
     cplan = ex.plan(count_effective=True)
     cplan.run(use_graph=False)
     torch.cuda.synchronize()

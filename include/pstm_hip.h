@@ -207,9 +207,10 @@ int pstm_hip_set_tuning(int mw_cap, int batch_cap);
 /* Pipelined staging of trace rows: the next batch's rows are fetched while
  * the current batch computes (f32 rows through registers, the batch capped so
  * that its rows fit; anti-aliasing rows by direct global-to-LDS loads into a
- * second row buffer).  -1 automatic (on for f32 rows, off with
- * anti-aliasing), 0 off, 1 on.  Used by the autotuner; no counterpart in the
- * reference. */
+ * second row buffer).  Off, each batch's rows go by direct global-to-LDS
+ * loads issued at the batch start into one buffer (the batch as large as the
+ * LDS allows).  -1 automatic (off), 0 off, 1 on.  Used by the autotuner; no
+ * counterpart in the reference. */
 int pstm_hip_set_pipeline(int mode);
 
 /* Shape of the most recent gather launch: mw, traces per batch, workgroups

@@ -132,7 +132,7 @@ def autotune_tiles(kernel, job, edges=(16, 32, 64), region: int = 64, candidates
         lib.pstm_hip_set_tuning(0, 0)
         lib.pstm_hip_set_pipeline(-1)
         _time_plan(lib, ex, tiles[:1])     # warm-up
-        for cand in candidates or [(4, 16, 1), (2, 8, 0)]:
+        for cand in candidates or [(3, 16, 0), (4, 16, 0)]:
             mw, batch, pipe = (tuple(cand) + (-1,))[:3]
             if lib.pstm_hip_set_tuning(mw, batch) != 0 or lib.pstm_hip_set_pipeline(pipe) != 0:
                 continue

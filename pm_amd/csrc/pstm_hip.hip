@@ -1029,8 +1029,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
     const bool pipe = !learned_tt(TT) && P.pipe;
     unsigned long long cnt = 0;
     __shared__ int s_item;
-    __shared__ int s_row[kMaxBatch];  // block rows of the current batch
-    // and of the next one (pipelined staging), in two copies by batch parity: a
+    // block rows of the next batch (its row loads and pair table read them
+    // from here), in two copies by batch parity: a
     // skipped batch has one barrier only, after which the waves read the copy
     // of the batch while a wave that runs ahead writes the other one for the
     // batch after; the copy is written again only after the next barrier
@@ -1252,9 +1252,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             }
         }
 
-        // Pipelined staging (pipe; never for the learned traveltime, whose rows
-        // go by direct loads so that no registers are held across the network
-        // evaluation): while a batch computes, each thread holds
+        // Pipelined staging (pipe, on request; never for the learned
+        // traveltime, whose rows go by direct loads so that no registers are
+        // held across the network evaluation): while a batch computes, each thread holds
         // up to kPF dwords of the next batch's rows in flight in registers; they
         // are written to LDS after the batch's last barrier.
         uint32_t pf[kPF];
@@ -1294,8 +1294,8 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 if (moved && r < nbb) src = row_dw(jb + r, rpar, r);
             }
         };
-        // Rows by direct global -> LDS loads (AA rows; the learned traveltime's
-        // f32 rows), into the second of two row buffers when pipelined, else
+        // Rows by direct global -> LDS loads (every unpipelined batch; pipelined
+        // AA rows too), into the second of two row buffers when pipelined, else
         // into the one buffer: (row, 64-dword chunk) units dealt round-robin
         // over waves w0.., one wave instruction each (wave-uniform LDS
         // destination, one dword per lane).  The batch's block rows are read
@@ -1304,7 +1304,7 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
         // waited for would also wait for every row load issued before it.
         int cur = 0;
         auto dma = [&](int jb, int buf, int rpar, int w0) {
-            if constexpr (AA || learned_tt(TT)) {
+            {
                 const int nbb = min(P.batch, T.ntr - jb);
                 if (nbb <= 0 || wave < w0) return;
                 const int nw = kWaves - w0;
@@ -1356,15 +1356,16 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             else prefetch(0, 1);
         }
 
-        // Unpipelined AA staging (early_dma): the batch's f64 rows go by direct
-        // global -> LDS loads into the one row buffer, issued at the batch start
+        // Unpipelined staging (early_dma, the default): the batch's rows (f32,
+        // or the f64 AA rows) go by direct global -> LDS loads into the one
+        // row buffer, issued at the batch start
         // so that they are in flight while the pair table is built: every load
         // of a wave in flight at once and no registers held (a loop of
         // load-then-store through registers waits out the memory latency once
         // per element).  The row indices come from s_rownext, written by the
         // batch before.  The waves that build the pair table issue none: their
         // own global reads would wait behind the row loads.
-        const bool early_dma = (AA || learned_tt(TT)) && !pipe;
+        const bool early_dma = !pipe;
         const int w_pairs = learned_tt(TT) ? 1 : (n_pairs + 63) / 64;  // waves holding the pair table
         // Pair inputs one batch ahead (w_pairs == 1): wave 0 loads batch jb's
         // geometry and midpoints (rows from s_rownext[rpar]) and batch jb +
@@ -1422,7 +1423,6 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
             int inap = 0;
             if (learned_tt(TT) && tid == 0) s_nrows = 0;
             if (tid < P.batch) {  // wave 0, which builds pairs and issues no row loads
-                if (tid < nb) s_row[tid] = s_rownext[par ^ 1][tid];
                 // the next batch's rows: its row loads and its pair table read
                 // them from here
                 const int jn = j0 + P.batch + tid;
@@ -1602,37 +1602,9 @@ __global__ __launch_bounds__(kThreads) void k_migrate(KParams P) {
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
             } else if (pipe) {
                 // f32 rows: stored at the batch start
-            } else if (early_dma) {
+            } else {
                 // the batch's rows, loaded at the batch start (early_dma)
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row loads have landed
-            } else {
-                // rows picked through the trace list: one flat loop over the
-                // batch's nb x rowlen elements, every thread busy, each row
-                // read coalesced
-                Row* dst = reinterpret_cast<Row*>(rows);
-                const int n = nb * P.rowlen;
-                int r = 0, c = tid;
-                while (c >= P.rowlen) {
-                    c -= P.rowlen;
-                    ++r;
-                }
-                auto row_src = [&](int rr) -> const Row* {
-                    const int rw = rr < nb ? s_row[rr] : 0;
-                    if constexpr (AA) return P.prow + (size_t)rw * P.rowlen;
-                    else return P.amp + (size_t)rw * P.ns;
-                };
-                const Row* src = row_src(r);
-                for (int e = tid; e < n; e += kThreads) {
-                    dst[e] = src[c];
-                    c += kThreads;
-                    if (c >= P.rowlen) {
-                        do {
-                            c -= P.rowlen;
-                            ++r;
-                        } while (c >= P.rowlen);
-                        src = row_src(r);
-                    }
-                }
             }
             // pipelined f32 rows are visible since the pair-table barrier: only
             // DMA'd or loaded rows and the lattice need this one
@@ -2224,7 +2196,10 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     // f32 rows, off for AA rows (measured: the larger single-buffered batch wins)
     int pipe = g_tune.pipe;
     if (pipe < 0 && getenv("PSTM_HIP_PIPE")) pipe = atoi(getenv("PSTM_HIP_PIPE")) != 0;
-    if (pipe < 0) pipe = sh.aa ? 0 : 1;
+    // off unless asked for: direct loads at the batch start let the batch
+    // grow to what the LDS holds (f32 rows through registers cap it at kPF
+    // dwords per thread: 6 traces of 2001 samples) and measured faster
+    if (pipe < 0) pipe = 0;
     const int rl_dw = sh.rowlen * (int)(sh.elem / 4);
     const int pipe_batch = sh.aa ? kMaxBatch : kPF * kThreads / rl_dw;
     // (the learned traveltime always stages by direct loads at the batch start)
