@@ -2192,8 +2192,7 @@ Shape launch_shape(const PstmMigrateArgs& a) {
     sh.mlp_terms = sh.learned && a.mlp_weight_terms == 2 ? 2 : 1;
     // pipelined staging: f32 rows through kPF registers per thread (the batch's
     // rows must fit), AA rows by direct global -> LDS loads into a second row
-    // buffer (LDS holds two buffers, so the batch shrinks); automatic: on for
-    // f32 rows, off for AA rows (measured: the larger single-buffered batch wins)
+    // buffer (LDS holds two buffers, so the batch shrinks); automatic: off
     int pipe = g_tune.pipe;
     if (pipe < 0 && getenv("PSTM_HIP_PIPE")) pipe = atoi(getenv("PSTM_HIP_PIPE")) != 0;
     // off unless asked for: direct loads at the batch start let the batch
