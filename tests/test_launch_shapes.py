@@ -1,8 +1,9 @@
 """Launch shapes of the gather: sample blocks per wave (MW), traces per LDS
 batch, and pipelined staging.  Pipelined staging keeps the next batch's f32
 rows in registers; for the f64 anti-aliasing rows it loads them straight from
-global memory into a second LDS buffer.  Unpipelined AA staging loads the
-current batch the same direct way into the one buffer.  A shape changes only
+global memory into a second LDS buffer.  Unpipelined staging (the default)
+loads the current batch's rows, f32 or f64, the same direct way into the one
+buffer at the batch start.  A shape changes only
 how the f32 per-batch sums are grouped, so every shape must give the FP64
 oracle's image within the FP32 bar and the oracle's fold exactly."""
 
